@@ -1,0 +1,214 @@
+/* nmlmc: MI355X (gfx950) level-l nested Monte Carlo sampler, C ABI.
+ *
+ * Drop-in for the reference's extern "C" entry points (the libnested.so built by
+ * the reference makefile:72-73) with the reference's -DREAL_DOUBLE data layouts:
+ * every real_t / prob_t field below is a double.  Sums and work are double, as in
+ * the reference (common.hpp:17-18).  The reference functions are void and abort
+ * on error; here they keep their void signatures and record a status that
+ * nmlmc_last_error() returns (0 = success).  The nmlmc_* entry points return that
+ * status directly.
+ *
+ * Threading: like the reference, one problem per process and not re-entrant;
+ * call from one host thread.  Outputs of a sample call are overwritten, not
+ * accumulated (nested_sampler.hpp:655).
+ */
+#ifndef NMLMC_H
+#define NMLMC_H
+#include <stddef.h>
+#include <stdint.h>
+#ifndef __cplusplus
+#include <stdbool.h>
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- data layouts crossing the ABI ---------------------------------------- */
+
+/* nested_sampler_params (nested_sampler.hpp:14-34), REAL_DOUBLE layout. */
+typedef struct nested_sampler_params {
+    uint32_t N0;
+    uint32_t beta;
+    bool antithetic;
+    uint32_t fine_moments;
+    uint32_t diff_moments;
+    double adaptive_r; /* adaptive iff 1 < r < 2 */
+} nested_sampler_params;
+
+/* sample_output (common.hpp:37-55): the caller owns fsums[phi*fine_moments] and
+ * diffsums[phi*diff_moments]. */
+typedef struct sample_output {
+    uint64_t M;
+    double* fsums;
+    double* diffsums;
+    double total_work;
+    uint64_t inner_stats[6]; /* {inner samples, outer, at max N, at base N, #uniforms, #normals} */
+} sample_output;
+
+/* model::data (model_sampler.cpp:21-39).  phi_type: 0 Indicator, 1 Max. */
+typedef struct model_data {
+    double stau;     /* sqrt(tau) */
+    double s1tau;    /* sqrt(1 - tau) */
+    double max_loss;
+    int32_t phi_type;
+} model_data;
+
+/* span<T> (common.hpp:69-131): pointer + size. */
+typedef struct nmlmc_span {
+    void* data;
+    size_t size;
+} nmlmc_span;
+
+/* portfolio::option_params / stock_params (portfolio_sampler.cpp:42-55).
+ * type: 1 call, -1 put; extension: 2 / -2 cash-or-nothing digital call / put. */
+typedef struct option_params {
+    double strike;
+    double maturity;
+    double start;     /* option value at t=0 (portfolio_comp_params) */
+    double w;         /* weight */
+    double delta_est; /* delta estimate (portfolio_comp_params) */
+    uint32_t stock_ind;
+    int32_t type;
+    double prob;      /* sub-sampling weight; options sorted by descending prob per method */
+    double W_est;     /* work estimate */
+} option_params;
+
+typedef struct stock_params {
+    double S0, mu, sigma;
+} stock_params;
+
+/* portfolio::data (portfolio_sampler.cpp:62-92).  options[m] for m = BS, Exact,
+ * Unbiased; spans of option_params / stock_params.  phi_type: 0 Indicator,
+ * 1 Max, 2 Moments, 3 FMoments.  subsample: 0 None, 1 Random, -1 Random_Uniform. */
+typedef struct portfolio_data {
+    nmlmc_span options[3];
+    nmlmc_span stocks;
+    double r, rho, s1_rho2;
+    double max_loss;
+    double tau;
+    uint32_t dt_h0inv;
+    uint32_t dt_beta;
+    double dt_prob_expo;
+    int32_t phi_type;
+    uint32_t _phi_size;
+    int32_t subsample;
+    bool use_delta_cv;
+    bool use_coarse_delta_cv;
+    bool use_antithetic_risk;
+    bool use_milstein;
+} portfolio_data;
+
+/* ---- reference entry points ------------------------------------------------ */
+
+/* common.cpp:20-26.  cores = number of virtual threads of the parity keying. */
+void init_nestedmc_sampler(uint32_t cores, const nested_sampler_params* params);
+/* common.cpp:28-31. */
+void cleanup_nestedmc_sampler(void);
+
+/* model_sampler.cpp:94-97. */
+void model_update_data(const model_data* data);
+/* model_sampler.cpp:99-104. */
+void model_init(const model_data* data, uint64_t seed);
+/* model_sampler.cpp:106-114. */
+void model_sample(uint32_t ell, uint64_t M, sample_output* out);
+
+/* portfolio_sampler.cpp:618-629. */
+void portfolio_cleanup(void);
+/* portfolio_sampler.cpp:631-678: builds per-method CDFs, compute_prob, prob_norm and
+ * delta_coalesce on the host and copies every table to device memory. */
+void portfolio_update_data(const portfolio_data* data);
+/* portfolio_sampler.cpp:680-686. */
+void portfolio_init(const portfolio_data* data, uint64_t seed);
+/* portfolio_sampler.cpp:687-700: start = Black-Scholes value, delta_est = BS delta. */
+void portfolio_comp_params(portfolio_data* data);
+/* portfolio_sampler.cpp:749-756.  override_dyn_inner > 0 forces the number of lanes
+ * that share one outer sample's inner loop (rounded to a power of two <= 64). */
+void portfolio_sample(uint32_t ell, uint64_t override_dyn_inner, uint64_t M, sample_output* out);
+
+/* ---- extensions (no reference counterpart) -------------------------------- */
+
+enum {
+    NMLMC_OK = 0,
+    NMLMC_ERR_STATE = 1,   /* call order: not initialised, or no problem staged */
+    NMLMC_ERR_ARG = 2,     /* invalid argument */
+    NMLMC_ERR_HIP = 3,     /* HIP runtime error */
+    NMLMC_ERR_DEVICE = 4   /* device-side check failed */
+};
+
+/* Forms: PLAIN = FP64, reference keying (per-virtual-thread streams keyed
+ * {seed, t}, bit-compatible with the reference OpenMP build); BATCHED = FP64,
+ * per-outer-sample keys, persistent work-queue kernel; MIXED = the batched form
+ * with FP32 inner arithmetic + compensated sums and FP64 re-evaluation near the
+ * Heaviside threshold. */
+enum { NMLMC_FORM_PLAIN = 0, NMLMC_FORM_BATCHED = 1, NMLMC_FORM_MIXED = 2 };
+
+int nmlmc_last_error(void);
+const char* nmlmc_last_error_string(void);
+int nmlmc_set_form(int form);
+int nmlmc_get_form(void);
+
+/* Adaptive rule: 0 = reference (nested_sampler.hpp:219-274); 1 = start at
+ * N0*beta^l and double while |mean| < 3 stderr, up to (N0*beta^l) << cap_log2. */
+int nmlmc_set_adaptive_rule(uint32_t rule, uint32_t cap_log2);
+
+/* Model problem with a variant: 0 = reference model (model_sampler.cpp:66-79);
+ * 1 = shift model X = Y + Z (one normal per inner sample), threshold = max_loss. */
+int nmlmc_model_init(int variant, int phi_type, double tau, double max_loss, uint64_t seed);
+
+/* General correlation for the portfolio outer draw: W_tau = sqrt(tau) * L z with
+ * L a d x d row-major lower-triangular factor (NULL restores the reference's
+ * one-factor draw).  Call after portfolio_init / portfolio_update_data. */
+int nmlmc_portfolio_set_correlation(const double* chol, uint32_t d);
+
+/* One level call with a status.  outer_offset (batched forms) selects the outer
+ * sample ids [outer_offset, outer_offset + M); pass UINT64_MAX to continue after
+ * the previous call of this level.  Optional per-outer-sample trace arrays of
+ * length M (any may be NULL): phi_f, phi_c (first phi component), nf, nc. */
+int nmlmc_sample(uint32_t ell, uint64_t M, uint64_t outer_offset, sample_output* out,
+                 double* tr_phi_f, double* tr_phi_c, uint64_t* tr_nf, uint64_t* tr_nc);
+
+/* Mixed form diagnostics of the last call: outer samples re-evaluated in FP64,
+ * and indicator flips (FP32 decision differing from the FP64 one). */
+int nmlmc_mixed_stats(uint64_t* reevaluated, uint64_t* flips);
+
+/* Device time (ms) of the last sample call's kernels, from HIP events on the
+ * library's stream, and the name of its dominant kernel. */
+double nmlmc_last_kernel_ms(void);
+const char* nmlmc_last_kernel_name(void);
+
+/* Lanes per outer sample chosen for the last call (autotuned or overridden). */
+uint32_t nmlmc_last_group_lanes(void);
+
+/* ---- MLMC driver (mlmc.hpp:207-337) over the current problem -------------- */
+
+/* Output of nmlmc_mlmc_run: per level sums as in mlmc_t::mc_diff_output_t
+ * (mlmc.hpp:21-27), 4 moments. */
+typedef struct nmlmc_level_stats {
+    double sums_fine[4];
+    double sums_diff[4];
+    double work;
+    double time;
+    uint64_t M;
+} nmlmc_level_stats;
+
+/* Runs mlmc_t<double,4>::run with the library's sample call as sample_level.
+ * levels_out has room for max_levels entries; *n_levels receives the count.
+ * summary_out[5] = {starting_level, bias_est, stat_error, final TOL, wall seconds}. */
+int nmlmc_mlmc_run(double targetTOL, uint64_t M0, double theta, double Ca,
+                   double starting_level_coeff, int verbose, uint32_t max_levels,
+                   nmlmc_level_stats* levels_out, uint32_t* n_levels, double* summary_out);
+
+/* Same driver over a caller-supplied level sampler (mlmc_input_t::sample_level,
+ * mlmc.hpp:224): fn fills *out for M fresh samples of level ell and returns 0, or
+ * a non-zero code that aborts the run. */
+typedef int (*nmlmc_sample_level_fn)(void* user, uint32_t ell, uint64_t M, nmlmc_level_stats* out);
+int nmlmc_mlmc_run_cb(nmlmc_sample_level_fn fn, void* user, double targetTOL, uint64_t M0,
+                      double theta, double Ca, double starting_level_coeff, int verbose,
+                      uint32_t max_levels, nmlmc_level_stats* levels_out, uint32_t* n_levels,
+                      double* summary_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NMLMC_H */

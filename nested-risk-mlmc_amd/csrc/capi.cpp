@@ -1,0 +1,475 @@
+// C ABI of the MI355X nested sampler: the reference's extern "C" entry points
+// (common.cpp:20-31, model_sampler.cpp:94-114, portfolio_sampler.cpp:618-756)
+// and the status-returning nmlmc_* extensions declared in include/nmlmc.h.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "../../include/nmlmc.h"
+#include "engine.hpp"
+#include "host.hpp"
+#include "model.hpp"
+#include "portfolio.hpp"
+
+namespace nmlmc {
+
+Engine g_engine;
+
+static thread_local int t_last_error = NMLMC_OK;
+static thread_local char t_last_msg[512] = "";
+
+int set_error(int code, const char* fmt, ...) {
+    t_last_error = code;
+    if (code == NMLMC_OK) {
+        t_last_msg[0] = 0;
+        return code;
+    }
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(t_last_msg, sizeof(t_last_msg), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+    return set_error(NMLMC_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+// uint_pow (common.hpp:146-157); x == 1 returns 1 (the reference's power-of-two
+// branch evaluates clz(0) there, which is undefined).
+uint64_t uint_pow_u64(uint64_t x, uint64_t n) {
+    if (n == 0 || x == 1) return 1;
+    if ((x & (x - 1)) == 0) return (uint64_t)1 << (n * (64 - __builtin_clzll(x - 1)));
+    uint64_t r = 1;
+    while (n--) r *= x;
+    return r;
+}
+
+// get_samples_count constructor (nested_sampler.hpp:225-239) and its rule-1
+// extension, evaluated with the host libm.
+LevelCounts level_counts(const EngineParams& ep, uint32_t ell) {
+    LevelCounts lc;
+    lc.rule = (int)ep.adapt_rule;
+    const uint64_t N0 = ep.N0;
+    if (ep.adapt_rule == 1) {
+        lc.base = N0 * uint_pow_u64(ep.beta, ell);
+        lc.max = lc.base << ep.cap_log2;
+        lc.tmp = 9.0;
+        lc.adaptive = 1;
+        return lc;
+    }
+    lc.max = N0 * uint_pow_u64(ep.beta, ell);
+    const double r = ep.adaptive_r;
+    lc.adaptive = (r > 1 && r < 2);
+    if (lc.adaptive) {
+        lc.base = N0 * uint_pow_u64(ep.beta >> 1, ell);
+        const double c_conf2 = 9.;
+        lc.tmp = (double)lc.max * std::pow(c_conf2 / (double)lc.max, r / 2);
+    } else {
+        lc.base = lc.max;
+        lc.tmp = 0;
+    }
+    return lc;
+}
+
+static uint32_t pow2_ceil_u32(uint64_t x) {
+    uint32_t g = 1;
+    while (g < x && g < 64) g <<= 1;
+    return g;
+}
+
+int Engine::ensure_device() {
+    if (stream) return NMLMC_OK;
+    hipError_t e = hipGetDevice(&device);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+    e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamCreate");
+    if ((e = hipEventCreate(&ev0)) != hipSuccess) return hip_fail(e, "hipEventCreate");
+    if ((e = hipEventCreate(&ev1)) != hipSuccess) return hip_fail(e, "hipEventCreate");
+    if ((e = hipMalloc(&d_out, sizeof(Partial))) != hipSuccess) return hip_fail(e, "hipMalloc");
+    if ((e = hipHostMalloc(&h_out, sizeof(Partial))) != hipSuccess) return hip_fail(e, "hipHostMalloc");
+    return NMLMC_OK;
+}
+
+void Engine::free_samplers() {
+    if (d_states) hipFree(d_states);
+    if (d_partials) hipFree(d_partials);
+    if (d_ws) hipFree(d_ws);
+    d_states = nullptr;
+    d_partials = nullptr;
+    d_ws = nullptr;
+    partial_cap = 0;
+    ws_cap = 0;
+}
+
+int Engine::ensure_partials(size_t n) {
+    if (n <= partial_cap) return NMLMC_OK;
+    if (d_partials) hipFree(d_partials);
+    d_partials = nullptr;
+    partial_cap = 0;
+    hipError_t e = hipMalloc(&d_partials, sizeof(Partial) * n);
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc partials");
+    partial_cap = n;
+    return NMLMC_OK;
+}
+
+int Engine::ensure_ws(size_t n) {
+    if (n <= ws_cap) return NMLMC_OK;
+    if (d_ws) hipFree(d_ws);
+    d_ws = nullptr;
+    ws_cap = 0;
+    hipError_t e = hipMalloc(&d_ws, sizeof(double) * n);
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc workspace");
+    ws_cap = n;
+    return NMLMC_OK;
+}
+
+// init_samplers (nested_sampler.hpp:618-644): both streams of thread t keyed
+// {seed, t} at position 0 (the OpenMP seeding, 640-641).
+int Engine::seed_samplers(uint64_t s) {
+    seed = s;
+    int rc = ensure_device();
+    if (rc) return rc;
+    free_samplers();
+    hipError_t e = hipMalloc(&d_states, sizeof(StreamState) * 2 * (size_t)cores);
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc states");
+    e = hipMemsetAsync(d_states, 0, sizeof(StreamState) * 2 * (size_t)cores, stream);
+    if (e != hipSuccess) return hip_fail(e, "hipMemset states");
+    next_outer.assign(64, 0);
+    e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+    return NMLMC_OK;
+}
+
+uint32_t Engine::phi_size() const {
+    if (problem == PROBLEM_PORTFOLIO) return pf_phi_size;
+    return 1;
+}
+
+// Copies the reduced record into a sample_output (device_to_host,
+// nested_sampler.hpp:562-577).
+void Engine::copy_out(uint32_t ell, sample_output* out) {
+    const Partial& p = *h_out;
+    const uint32_t ps = phi_size();
+    if (out->fsums)
+        for (uint32_t i = 0; i < ps * ep.fine_moments; i++) out->fsums[i] = p.f[i];
+    if (out->diffsums)
+        for (uint32_t i = 0; i < ps * ep.diff_moments; i++) out->diffsums[i] = ell == 0 ? 0. : p.d[i];
+    out->total_work = p.work;
+    for (int i = 0; i < kStats; i++) out->inner_stats[i] = p.stats[i];
+}
+
+int Engine::sample(uint32_t ell, uint64_t M, uint64_t outer_offset, uint32_t lanes_override,
+                   sample_output* out, double* tr_phi_f, double* tr_phi_c, uint64_t* tr_nf,
+                   uint64_t* tr_nc) {
+    if (!inited) return set_error(NMLMC_ERR_STATE, "init_nestedmc_sampler has not been called");
+    if (problem < 0) return set_error(NMLMC_ERR_STATE, "no problem staged (model_init / portfolio_init)");
+    if (!out) return set_error(NMLMC_ERR_ARG, "sample_output is NULL");
+    if (ell > 40) return set_error(NMLMC_ERR_ARG, "level %u out of range", ell);
+    out->M = M;
+    if (M == 0) {
+        std::memset(h_out, 0, sizeof(Partial));
+        copy_out(ell, out);
+        return set_error(NMLMC_OK, "");
+    }
+    if (form == NMLMC_FORM_PLAIN) {
+        if (problem == PROBLEM_MODEL)
+            return sample_model_parity(ell, M, lanes_override, out, tr_phi_f, tr_phi_c, tr_nf, tr_nc);
+        return sample_portfolio_parity(ell, M, lanes_override, out, tr_phi_f, tr_phi_c, tr_nf, tr_nc);
+    }
+    if (ell >= next_outer.size()) next_outer.resize(ell + 1, 0);
+    const uint64_t off = (outer_offset == UINT64_MAX) ? next_outer[ell] : outer_offset;
+    int rc = sample_batched(ell, M, off, lanes_override, out, tr_phi_f, tr_phi_c, tr_nf, tr_nc);
+    if (rc == NMLMC_OK) next_outer[ell] = off + M;
+    return rc;
+}
+
+// Device trace buffers for one call (test/diagnostic path only).
+int Engine::trace_alloc(uint64_t M, bool want, TraceBufs& tb) {
+    tb = TraceBufs{};
+    if (!want) return NMLMC_OK;
+    hipError_t e;
+    if ((e = hipMalloc(&tb.phi_f, 8 * M)) != hipSuccess) return hip_fail(e, "hipMalloc trace");
+    if ((e = hipMalloc(&tb.phi_c, 8 * M)) != hipSuccess) return hip_fail(e, "hipMalloc trace");
+    if ((e = hipMalloc(&tb.nf, 8 * M)) != hipSuccess) return hip_fail(e, "hipMalloc trace");
+    if ((e = hipMalloc(&tb.nc, 8 * M)) != hipSuccess) return hip_fail(e, "hipMalloc trace");
+    return NMLMC_OK;
+}
+
+int Engine::trace_finish(uint64_t M, TraceBufs& tb, double* tr_phi_f, double* tr_phi_c,
+                         uint64_t* tr_nf, uint64_t* tr_nc) {
+    hipError_t e = hipSuccess;
+    if (tb.phi_f) {
+        if (tr_phi_f && e == hipSuccess) e = hipMemcpy(tr_phi_f, tb.phi_f, 8 * M, hipMemcpyDeviceToHost);
+        if (tr_phi_c && e == hipSuccess) e = hipMemcpy(tr_phi_c, tb.phi_c, 8 * M, hipMemcpyDeviceToHost);
+        if (tr_nf && e == hipSuccess) e = hipMemcpy(tr_nf, tb.nf, 8 * M, hipMemcpyDeviceToHost);
+        if (tr_nc && e == hipSuccess) e = hipMemcpy(tr_nc, tb.nc, 8 * M, hipMemcpyDeviceToHost);
+        hipFree(tb.phi_f);
+        hipFree(tb.phi_c);
+        hipFree(tb.nf);
+        hipFree(tb.nc);
+        tb = TraceBufs{};
+    }
+    if (e != hipSuccess) return hip_fail(e, "trace copy");
+    return NMLMC_OK;
+}
+
+int Engine::finish_call(uint32_t ell, sample_output* out) {
+    hipError_t e = hipEventRecord(ev1, stream);
+    if (e != hipSuccess) return hip_fail(e, "hipEventRecord");
+    e = hipMemcpyAsync(h_out, d_out, sizeof(Partial), hipMemcpyDeviceToHost, stream);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync result");
+    e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) return hip_fail(e, "kernel execution");
+    float ms = 0;
+    hipEventElapsedTime(&ms, ev0, ev1);
+    last_ms = ms;
+    copy_out(ell, out);
+    return NMLMC_OK;
+}
+
+int Engine::sample_model_parity(uint32_t ell, uint64_t M, uint32_t lanes_override,
+                                sample_output* out, double* tr_phi_f, double* tr_phi_c,
+                                uint64_t* tr_nf, uint64_t* tr_nc) {
+    ModelParityArgsHost H;
+    H.mp = mp;
+    H.ep = ep;
+    H.cf = level_counts(ep, ell);
+    H.cc = level_counts(ep, ell == 0 ? 0 : ell - 1);
+    H.seed = seed;
+    H.M = M;
+    H.T = cores;
+    H.ell = ell;
+    const int nps = mp.variant == MODEL_REF ? 2 : 1;
+    const uint64_t pairs = (H.cf.base * nps + 1) / 2;
+    H.G = lanes_override ? pow2_ceil_u32(lanes_override) : pow2_ceil_u32(pairs);
+    if (const char* s = getenv("NMLMC_GROUP_LANES")) H.G = pow2_ceil_u32(strtoul(s, nullptr, 10));
+    last_G = H.G;
+    const uint64_t nblocks = parity_blocks(cores, H.G);
+    int rc = ensure_partials(nblocks);
+    if (rc) return rc;
+    H.states = d_states;
+    H.partials = d_partials;
+    TraceBufs tb;
+    rc = trace_alloc(M, tr_phi_f || tr_phi_c || tr_nf || tr_nc, tb);
+    if (rc) return rc;
+    H.tr_phi_f = tb.phi_f;
+    H.tr_phi_c = tb.phi_c;
+    H.tr_nf = tb.nf;
+    H.tr_nc = tb.nc;
+    hipError_t e = hipEventRecord(ev0, stream);
+    if (e != hipSuccess) return hip_fail(e, "hipEventRecord");
+    e = launch_model_parity(H, stream);
+    if (e != hipSuccess) return hip_fail(e, "launch k_model_parity");
+    e = launch_reduce_partials(d_partials, (uint32_t)nblocks, d_out, stream);
+    if (e != hipSuccess) return hip_fail(e, "launch k_reduce_partials");
+    last_kernel = "k_model_parity";
+    rc = finish_call(ell, out);
+    if (rc) return rc;
+    rc = trace_finish(M, tb, tr_phi_f, tr_phi_c, tr_nf, tr_nc);
+    if (rc) return rc;
+    return set_error(NMLMC_OK, "");
+}
+
+static uint32_t pow2_floor_u32(uint64_t x) {
+    uint32_t g = 1;
+    while ((uint64_t)g * 2 <= x && g < 64) g <<= 1;
+    return g;
+}
+
+int Engine::device_cus() {
+    if (cus > 0) return cus;
+    int n = 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || n <= 0) n = 256;
+    cus = n;
+    return cus;
+}
+
+// Batched forms: persistent grid over a device queue of outer samples.
+int Engine::sample_batched(uint32_t ell, uint64_t M, uint64_t outer_offset, uint32_t lanes_override,
+                           sample_output* out, double* tr_phi_f, double* tr_phi_c, uint64_t* tr_nf,
+                           uint64_t* tr_nc) {
+    if (problem != PROBLEM_MODEL) return sample_portfolio_batched(ell, M, outer_offset, lanes_override, out,
+                                                                   tr_phi_f, tr_phi_c, tr_nf, tr_nc);
+    if (!d_queue) {
+        hipError_t e = hipMalloc(&d_queue, 64);
+        if (e != hipSuccess) return hip_fail(e, "hipMalloc queue");
+    }
+    ModelBatchedArgs A;
+    A.mp = mp;
+    A.ep = ep;
+    A.cf = level_counts(ep, ell);
+    A.cc = level_counts(ep, ell == 0 ? 0 : ell - 1);
+    A.seed = seed;
+    A.M = M;
+    A.offset = outer_offset;
+    A.ell = ell;
+    A.queue = (unsigned long long*)d_queue;
+    // Lanes per outer sample: spread the resident lanes over the outer samples.
+    const uint64_t lanes = (uint64_t)device_cus() * 64 * 16;
+    uint32_t G = pow2_floor_u32(lanes / M);
+    if (lanes_override) G = pow2_ceil_u32(lanes_override);
+    if (const char* sG = getenv("NMLMC_GROUP_LANES")) G = pow2_ceil_u32(strtoul(sG, nullptr, 10));
+    last_G = G;
+    uint64_t blocks = (M * G + 255) / 256;
+    const uint64_t max_blocks = (uint64_t)device_cus() * 8;
+    if (blocks > max_blocks) blocks = max_blocks;
+    int rc = ensure_partials(blocks);
+    if (rc) return rc;
+    A.partials = d_partials;
+    TraceBufs tb;
+    rc = trace_alloc(M, tr_phi_f || tr_phi_c || tr_nf || tr_nc, tb);
+    if (rc) return rc;
+    A.tr_phi_f = tb.phi_f;
+    A.tr_phi_c = tb.phi_c;
+    A.tr_nf = tb.nf;
+    A.tr_nc = tb.nc;
+    hipError_t e = hipEventRecord(ev0, stream);
+    if (e != hipSuccess) return hip_fail(e, "hipEventRecord");
+    e = launch_model_batched(A, G, (uint32_t)blocks, form == NMLMC_FORM_MIXED, stream);
+    if (e != hipSuccess) return hip_fail(e, "launch k_model_batched");
+    e = launch_reduce_partials(d_partials, (uint32_t)blocks, d_out, stream);
+    if (e != hipSuccess) return hip_fail(e, "launch k_reduce_partials");
+    last_kernel = "k_model_batched";
+    rc = finish_call(ell, out);
+    if (rc) return rc;
+    mixed_reeval = h_out->stats[6];
+    mixed_flips = h_out->stats[7];
+    rc = trace_finish(M, tb, tr_phi_f, tr_phi_c, tr_nf, tr_nc);
+    if (rc) return rc;
+    return set_error(NMLMC_OK, "");
+}
+
+}  // namespace nmlmc
+
+using nmlmc::g_engine;
+using nmlmc::set_error;
+
+extern "C" {
+
+int nmlmc_last_error(void) { return nmlmc::t_last_error; }
+const char* nmlmc_last_error_string(void) { return nmlmc::t_last_msg; }
+
+// init_nestedmc_sampler (common.cpp:20-26).
+void init_nestedmc_sampler(uint32_t cores, const nested_sampler_params* params) {
+    if (!params || cores == 0) {
+        set_error(NMLMC_ERR_ARG, "init_nestedmc_sampler: cores must be > 0 and params non-NULL");
+        return;
+    }
+    if (params->fine_moments > nmlmc::kMaxMoments || params->diff_moments > nmlmc::kMaxMoments ||
+        params->N0 == 0 || params->beta == 0) {
+        set_error(NMLMC_ERR_ARG, "init_nestedmc_sampler: invalid parameters");
+        return;
+    }
+    g_engine.free_samplers();
+    g_engine.cores = cores;
+    g_engine.ep.N0 = params->N0;
+    g_engine.ep.beta = params->beta;
+    g_engine.ep.antithetic = params->antithetic ? 1 : 0;
+    g_engine.ep.fine_moments = params->fine_moments;
+    g_engine.ep.diff_moments = params->diff_moments;
+    g_engine.ep.adaptive_r = params->adaptive_r;
+    g_engine.ep.adapt_rule = 0;
+    g_engine.ep.cap_log2 = 0;
+    g_engine.problem = -1;
+    g_engine.inited = true;
+    int rc = g_engine.ensure_device();
+    if (rc) return;
+    set_error(NMLMC_OK, "");
+}
+
+// cleanup_nestedmc_sampler (common.cpp:28-31).
+void cleanup_nestedmc_sampler(void) {
+    g_engine.free_samplers();
+    g_engine.free_portfolio();
+    g_engine.inited = false;
+    g_engine.problem = -1;
+    set_error(NMLMC_OK, "");
+}
+
+int nmlmc_set_form(int form) {
+    if (form < NMLMC_FORM_PLAIN || form > NMLMC_FORM_MIXED) return set_error(NMLMC_ERR_ARG, "unknown form %d", form);
+    g_engine.form = form;
+    return set_error(NMLMC_OK, "");
+}
+int nmlmc_get_form(void) { return g_engine.form; }
+
+int nmlmc_set_adaptive_rule(uint32_t rule, uint32_t cap_log2) {
+    if (!g_engine.inited) return set_error(NMLMC_ERR_STATE, "init_nestedmc_sampler has not been called");
+    if (rule > 1 || cap_log2 > 24) return set_error(NMLMC_ERR_ARG, "invalid adaptive rule");
+    g_engine.ep.adapt_rule = rule;
+    g_engine.ep.cap_log2 = cap_log2;
+    return set_error(NMLMC_OK, "");
+}
+
+// model_update_data (model_sampler.cpp:94-97).
+void model_update_data(const model_data* data) {
+    if (!data) {
+        set_error(NMLMC_ERR_ARG, "model_update_data: NULL data");
+        return;
+    }
+    g_engine.mp.stau = data->stau;
+    g_engine.mp.s1tau = data->s1tau;
+    g_engine.mp.max_loss = data->max_loss;
+    g_engine.mp.phi_type = data->phi_type;
+    g_engine.mp.variant = nmlmc::MODEL_REF;
+    g_engine.problem = nmlmc::PROBLEM_MODEL;
+    set_error(NMLMC_OK, "");
+}
+
+// model_init (model_sampler.cpp:99-104).
+void model_init(const model_data* data, uint64_t seed) {
+    if (!g_engine.inited) {
+        set_error(NMLMC_ERR_STATE, "init_nestedmc_sampler has not been called");
+        return;
+    }
+    model_update_data(data);
+    if (nmlmc_last_error()) return;
+    g_engine.seed_samplers(seed);
+}
+
+int nmlmc_model_init(int variant, int phi_type, double tau, double max_loss, uint64_t seed) {
+    if (!g_engine.inited) return set_error(NMLMC_ERR_STATE, "init_nestedmc_sampler has not been called");
+    if (variant != nmlmc::MODEL_REF && variant != nmlmc::MODEL_SHIFT)
+        return set_error(NMLMC_ERR_ARG, "unknown model variant %d", variant);
+    if (phi_type != 0 && phi_type != 1) return set_error(NMLMC_ERR_ARG, "model phi_type must be 0 or 1");
+    if (!(tau > 0 && tau < 1)) return set_error(NMLMC_ERR_ARG, "tau must lie in (0, 1)");
+    model_data d;
+    d.stau = std::sqrt(tau);
+    d.s1tau = std::sqrt(1 - tau);
+    d.max_loss = max_loss;
+    d.phi_type = phi_type;
+    model_update_data(&d);
+    g_engine.mp.variant = variant;
+    return g_engine.seed_samplers(seed) ? nmlmc_last_error() : set_error(NMLMC_OK, "");
+}
+
+// model_sample (model_sampler.cpp:106-114).
+void model_sample(uint32_t ell, uint64_t M, sample_output* out) {
+    if (g_engine.problem != nmlmc::PROBLEM_MODEL) {
+        set_error(NMLMC_ERR_STATE, "model_sample: no model problem staged");
+        return;
+    }
+    g_engine.sample(ell, M, UINT64_MAX, 0, out, nullptr, nullptr, nullptr, nullptr);
+}
+
+int nmlmc_sample(uint32_t ell, uint64_t M, uint64_t outer_offset, sample_output* out,
+                 double* tr_phi_f, double* tr_phi_c, uint64_t* tr_nf, uint64_t* tr_nc) {
+    return g_engine.sample(ell, M, outer_offset, 0, out, tr_phi_f, tr_phi_c, tr_nf, tr_nc);
+}
+
+int nmlmc_mixed_stats(uint64_t* reevaluated, uint64_t* flips) {
+    if (reevaluated) *reevaluated = g_engine.mixed_reeval;
+    if (flips) *flips = g_engine.mixed_flips;
+    return set_error(NMLMC_OK, "");
+}
+
+double nmlmc_last_kernel_ms(void) { return g_engine.last_ms; }
+const char* nmlmc_last_kernel_name(void) { return g_engine.last_kernel; }
+uint32_t nmlmc_last_group_lanes(void) { return g_engine.last_G; }
+
+}  // extern "C"

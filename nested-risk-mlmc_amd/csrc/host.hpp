@@ -1,0 +1,94 @@
+// Host-side engine state: the one-problem-per-process globals of the reference
+// (gc_sampler_data, gc_phi_size, <problem>::gc_data, g_p_nestedmc_data;
+// common.cpp:15-17) gathered in one object.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <vector>
+
+#include "../../include/nmlmc.h"
+#include "engine.hpp"
+#include "model.hpp"
+#include "portfolio.hpp"
+
+namespace nmlmc {
+
+enum { PROBLEM_MODEL = 0, PROBLEM_PORTFOLIO = 1 };
+
+struct TraceBufs {
+    double* phi_f = nullptr;
+    double* phi_c = nullptr;
+    uint64_t* nf = nullptr;
+    uint64_t* nc = nullptr;
+};
+
+struct Engine {
+    bool inited = false;
+    uint32_t cores = 0;
+    EngineParams ep{};
+    int form = NMLMC_FORM_PLAIN;
+    int problem = -1;
+    ModelParams mp{};
+    uint64_t seed = 0;
+    int device = 0;
+
+    // Portfolio tables (device) and host mirrors.
+    PortfolioDevice pf{};
+    uint32_t pf_phi_size = 1;
+    std::vector<void*> pf_allocs;
+
+    StreamState* d_states = nullptr;  // [2 cores]: parity streams (sync, inner)
+    Partial* d_partials = nullptr;    // [cores] or [batched groups]
+    size_t partial_cap = 0;
+    Partial* d_out = nullptr;
+    Partial* h_out = nullptr;
+    double* d_ws = nullptr;           // per-group workspace (portfolio R_tau)
+    size_t ws_cap = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    float last_ms = 0;
+    const char* last_kernel = "";
+    uint32_t last_G = 0;
+    std::vector<uint64_t> next_outer;   // batched keying: next outer id per level
+    uint64_t* d_queue = nullptr;        // batched: work-queue counters
+    uint64_t mixed_reeval = 0, mixed_flips = 0;
+
+    int ensure_device();
+    int seed_samplers(uint64_t s);
+    void free_samplers();
+    void free_portfolio();
+    uint32_t phi_size() const;
+    void copy_out(uint32_t ell, sample_output* out);
+    int ensure_partials(size_t n);
+    int ensure_ws(size_t n);
+    int trace_alloc(uint64_t M, bool want, TraceBufs& tb);
+    int trace_finish(uint64_t M, TraceBufs& tb, double* tr_phi_f, double* tr_phi_c,
+                     uint64_t* tr_nf, uint64_t* tr_nc);
+    int finish_call(uint32_t ell, sample_output* out);
+    int sample(uint32_t ell, uint64_t M, uint64_t outer_offset, uint32_t lanes_override,
+               sample_output* out, double* tr_phi_f, double* tr_phi_c, uint64_t* tr_nf,
+               uint64_t* tr_nc);
+    int sample_model_parity(uint32_t ell, uint64_t M, uint32_t lanes_override, sample_output* out,
+                            double* tr_phi_f, double* tr_phi_c, uint64_t* tr_nf, uint64_t* tr_nc);
+    int sample_portfolio_parity(uint32_t ell, uint64_t M, uint32_t lanes_override,
+                                sample_output* out, double* tr_phi_f, double* tr_phi_c,
+                                uint64_t* tr_nf, uint64_t* tr_nc);
+    int cus = 0;
+    int device_cus();
+    int sample_portfolio_batched(uint32_t ell, uint64_t M, uint64_t outer_offset,
+                                 uint32_t lanes_override, sample_output* out, double* tr_phi_f,
+                                 double* tr_phi_c, uint64_t* tr_nf, uint64_t* tr_nc);
+    int sample_batched(uint32_t ell, uint64_t M, uint64_t outer_offset, uint32_t lanes_override,
+                       sample_output* out, double* tr_phi_f, double* tr_phi_c, uint64_t* tr_nf,
+                       uint64_t* tr_nc);
+};
+
+extern Engine g_engine;
+
+int set_error(int code, const char* fmt, ...);
+int hip_fail(hipError_t e, const char* what);
+uint64_t uint_pow_u64(uint64_t x, uint64_t n);
+LevelCounts level_counts(const EngineParams& ep, uint32_t ell);
+
+}  // namespace nmlmc

@@ -1,0 +1,130 @@
+// C entry points of the MLMC driver (mlmc.hpp:290-337) over the GPU level call
+// or a caller-supplied level sampler.
+#include <chrono>
+#include <cstring>
+#include <stdexcept>
+
+#include "../../include/nmlmc.h"
+#include "host.hpp"
+#include "mlmc.hpp"
+
+namespace nmlmc {
+
+namespace {
+struct abort_run {
+    int code;
+};
+
+int run_driver(mlmc_real::mlmc_input_t& in, uint32_t max_levels, nmlmc_level_stats* levels_out,
+               uint32_t* n_levels, double* summary_out) {
+    const auto t0 = std::chrono::steady_clock::now();
+    mlmc_real::mlmc_output_t out;
+    try {
+        out = mlmc_real::run(in);
+    } catch (const abort_run& a) {
+        return a.code;
+    }
+    const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    const uint32_t n = (uint32_t)out.levels.size();
+    if (n_levels) *n_levels = n;
+    if (levels_out) {
+        for (uint32_t l = 0; l < n && l < max_levels; l++) {
+            const auto& lv = out.levels[l];
+            for (int k = 0; k < 4; k++) {
+                levels_out[l].sums_fine[k] = lv.sums_fine[k];
+                levels_out[l].sums_diff[k] = lv.sums_diff[k];
+            }
+            levels_out[l].work = lv.work;
+            levels_out[l].time = lv.time;
+            levels_out[l].M = lv.M;
+        }
+    }
+    if (summary_out) {
+        summary_out[0] = out.starting_level;
+        summary_out[1] = out.bias_est;
+        summary_out[2] = out.stat_error;
+        summary_out[3] = out.TOL;
+        summary_out[4] = wall;
+    }
+    return NMLMC_OK;
+}
+
+void fill_input(mlmc_real::mlmc_input_t& in, double targetTOL, uint64_t M0, double theta, double Ca,
+                double slc, int verbose) {
+    in.targetTOL = targetTOL;
+    in.M0 = M0;
+    in.theta = theta;
+    in.Ca = Ca;
+    in.starting_level_coeff = slc;
+    in.verbose = verbose != 0;
+}
+}  // namespace
+
+}  // namespace nmlmc
+
+using namespace nmlmc;
+
+extern "C" {
+
+int nmlmc_mlmc_run(double targetTOL, uint64_t M0, double theta, double Ca, double starting_level_coeff,
+                   int verbose, uint32_t max_levels, nmlmc_level_stats* levels_out, uint32_t* n_levels,
+                   double* summary_out) {
+    if (!g_engine.inited || g_engine.problem < 0)
+        return set_error(NMLMC_ERR_STATE, "nmlmc_mlmc_run: no problem staged");
+    if (!(targetTOL > 0)) return set_error(NMLMC_ERR_ARG, "targetTOL must be positive");
+    mlmc_real::mlmc_input_t in;
+    fill_input(in, targetTOL, M0, theta, Ca, starting_level_coeff, verbose);
+    const uint32_t ps = g_engine.phi_size();
+    // The sample_level lambda of the reference mains (model_sampler.cpp:158-172).
+    in.sample_level = [ps](uint32_t ell, uint64_t M) {
+        mlmc_real::mc_diff_output_t mc;
+        double fs[kMaxPhi * kMaxMoments] = {0}, ds[kMaxPhi * kMaxMoments] = {0};
+        sample_output out;
+        out.fsums = fs;
+        out.diffsums = ds;
+        const auto t0 = std::chrono::steady_clock::now();
+        int rc = g_engine.sample(ell, M, UINT64_MAX, 0, &out, nullptr, nullptr, nullptr, nullptr);
+        if (rc) throw abort_run{rc};
+        mc.time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        (void)ps;
+        for (int k = 0; k < 4; k++) {
+            mc.sums_fine[k] = fs[k];
+            mc.sums_diff[k] = ds[k];
+        }
+        mc.work = out.total_work;
+        mc.M = out.M;
+        return mc;
+    };
+    int rc = run_driver(in, max_levels, levels_out, n_levels, summary_out);
+    if (rc) return rc;
+    return set_error(NMLMC_OK, "");
+}
+
+int nmlmc_mlmc_run_cb(nmlmc_sample_level_fn fn, void* user, double targetTOL, uint64_t M0, double theta,
+                      double Ca, double starting_level_coeff, int verbose, uint32_t max_levels,
+                      nmlmc_level_stats* levels_out, uint32_t* n_levels, double* summary_out) {
+    if (!fn) return set_error(NMLMC_ERR_ARG, "sample_level callback is NULL");
+    if (!(targetTOL > 0)) return set_error(NMLMC_ERR_ARG, "targetTOL must be positive");
+    mlmc_real::mlmc_input_t in;
+    fill_input(in, targetTOL, M0, theta, Ca, starting_level_coeff, verbose);
+    in.sample_level = [fn, user](uint32_t ell, uint64_t M) {
+        nmlmc_level_stats st;
+        std::memset(&st, 0, sizeof(st));
+        int rc = fn(user, ell, M, &st);
+        if (rc) throw abort_run{rc};
+        mlmc_real::mc_diff_output_t mc;
+        for (int k = 0; k < 4; k++) {
+            mc.sums_fine[k] = st.sums_fine[k];
+            mc.sums_diff[k] = st.sums_diff[k];
+        }
+        mc.work = st.work;
+        mc.time = st.time;
+        mc.M = st.M;
+        return mc;
+    };
+    int rc = run_driver(in, max_levels, levels_out, n_levels, summary_out);
+    if (rc) return set_error(NMLMC_ERR_ARG, "sample_level callback returned %d", rc);
+    return set_error(NMLMC_OK, "");
+}
+
+}  // extern "C"

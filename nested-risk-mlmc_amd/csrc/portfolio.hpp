@@ -1,0 +1,46 @@
+// Portfolio problem (portfolio_sampler.cpp:16-615): device tables and the launch
+// interface of its kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "engine.hpp"
+
+namespace nmlmc {
+
+// Option record in device memory: the reference's option_params with real_t =
+// double (64 bytes), one aligned record per gather.
+struct alignas(16) OptionRec {
+    double strike, maturity, start, w, delta_est;
+    uint32_t stock_ind;
+    int32_t type;
+    double prob, W_est;
+};
+static_assert(sizeof(OptionRec) == 64, "option record must stay 64 bytes");
+
+struct StockRec {
+    double S0, mu, sigma;
+};
+
+// data_ex (portfolio_sampler.cpp:94-132) with device pointers.
+struct PortfolioDevice {
+    const OptionRec* options[3];
+    const double* cdfs[3];
+    uint32_t nopt[3];
+    const StockRec* stocks;
+    const double* delta_coalesce;
+    const double* chol;  // d x d row-major lower triangle, or nullptr
+    uint32_t d;
+    double r, rho, s1_rho2, max_loss, tau, stau;
+    uint32_t dt_h0inv, dt_beta;
+    double dt_prob_expo;
+    int phi_type;
+    uint32_t phi_size;
+    int subsample;
+    int use_delta_cv, use_coarse_delta_cv, use_antithetic_risk, use_milstein;
+    double compute_prob[3];
+    uint32_t options_count;
+    double prob_norm, total_std;
+};
+
+}  // namespace nmlmc

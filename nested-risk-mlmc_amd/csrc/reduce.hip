@@ -1,0 +1,52 @@
+// Fixed-order reduction of per-block Partial records into one record: one
+// 256-thread block per field, strided partial sums then an LDS tree, so the
+// result does not depend on the order in which blocks finished.
+#include "engine.hpp"
+
+namespace nmlmc {
+
+__global__ __launch_bounds__(256) void k_reduce_partials(const Partial* __restrict__ P, uint32_t n,
+                                                         Partial* out) {
+    __shared__ double sd[256];
+    __shared__ uint64_t su[256];
+    constexpr int nsum = 2 * kMaxPhi * kMaxMoments;
+    const int field = blockIdx.x;  // sums, then work, then the stats
+    const int tid = threadIdx.x;
+    if (field <= nsum) {
+        double acc = 0.;
+        for (uint32_t t = tid; t < n; t += 256) {
+            const Partial& p = P[t];
+            acc += field < kMaxPhi * kMaxMoments ? p.f[field]
+                   : (field < nsum ? p.d[field - kMaxPhi * kMaxMoments] : p.work);
+        }
+        sd[tid] = acc;
+        __syncthreads();
+        for (int o = 128; o > 0; o >>= 1) {
+            if (tid < o) sd[tid] += sd[tid + o];
+            __syncthreads();
+        }
+        if (tid == 0) {
+            if (field < kMaxPhi * kMaxMoments) out->f[field] = sd[0];
+            else if (field < nsum) out->d[field - kMaxPhi * kMaxMoments] = sd[0];
+            else out->work = sd[0];
+        }
+    } else {
+        const int k = field - nsum - 1;
+        uint64_t acc = 0;
+        for (uint32_t t = tid; t < n; t += 256) acc += P[t].stats[k];
+        su[tid] = acc;
+        __syncthreads();
+        for (int o = 128; o > 0; o >>= 1) {
+            if (tid < o) su[tid] += su[tid + o];
+            __syncthreads();
+        }
+        if (tid == 0) out->stats[k] = su[0];
+    }
+}
+
+hipError_t launch_reduce_partials(const Partial* P, uint32_t n, Partial* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_reduce_partials, dim3(kPartialFields), dim3(256), 0, s, P, n, out);
+    return hipGetLastError();
+}
+
+}  // namespace nmlmc

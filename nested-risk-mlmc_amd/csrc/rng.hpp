@@ -1,0 +1,162 @@
+// Counter-based RNG for gfx950: Threefry2x64-20 with the reference's stream
+// semantics (Random123 as used by par_omp.hpp:72-121).
+//
+// A stream is a key {k0, k1} and a position P = number of u64 words already
+// handed out.  Word q of a stream is element 1 - (q & 1) of the block
+// threefry({q >> 1, 0}, key): random_state::rand() returns cur[1] before cur[0]
+// (par_omp.hpp:92-99).  randn() takes two consecutive words (first, second) and
+// evaluates boxmuller(u0 = second, u1 = first) - the argument order the
+// reference build produces - returning x and caching y.  Because every position
+// is a pure function of (key, P), any lane can compute any draw: the reference's
+// sequential stream becomes a parallel one.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace nmlmc {
+
+struct u64x2 {
+    uint64_t a, b;
+};
+
+__device__ __forceinline__ uint64_t rotl64(uint64_t x, unsigned n) {
+    return (x << n) | (x >> (64 - n));
+}
+
+// Threefry2x64 with 20 rounds (Random123/threefry.h:190-293): Skein rotation
+// constants {16,42,12,31,16,32,24,21}, key-schedule parity 0x1BD11BDAA9FC1A22,
+// key injection every 4 rounds with the injection count added to word 1.
+__device__ __forceinline__ u64x2 threefry2x64_20(uint64_t c0, uint64_t c1, uint64_t k0, uint64_t k1) {
+    const uint64_t k2 = 0x1BD11BDAA9FC1A22ULL ^ k0 ^ k1;
+    uint64_t x0 = c0 + k0, x1 = c1 + k1;
+#define NMLMC_TF_R(R)     \
+    x0 += x1;             \
+    x1 = rotl64(x1, R);   \
+    x1 ^= x0;
+    NMLMC_TF_R(16) NMLMC_TF_R(42) NMLMC_TF_R(12) NMLMC_TF_R(31)
+    x0 += k1; x1 += k2 + 1;
+    NMLMC_TF_R(16) NMLMC_TF_R(32) NMLMC_TF_R(24) NMLMC_TF_R(21)
+    x0 += k2; x1 += k0 + 2;
+    NMLMC_TF_R(16) NMLMC_TF_R(42) NMLMC_TF_R(12) NMLMC_TF_R(31)
+    x0 += k0; x1 += k1 + 3;
+    NMLMC_TF_R(16) NMLMC_TF_R(32) NMLMC_TF_R(24) NMLMC_TF_R(21)
+    x0 += k1; x1 += k2 + 4;
+    NMLMC_TF_R(16) NMLMC_TF_R(42) NMLMC_TF_R(12) NMLMC_TF_R(31)
+    x0 += k2; x1 += k0 + 5;
+#undef NMLMC_TF_R
+    return {x0, x1};
+}
+
+// r123::u01 / uneg11 for W = 64 (Random123/uniform.hpp:144-180).  The scale is
+// a power of two, so the product is exact and fused or unfused evaluation
+// rounds identically.
+__device__ __forceinline__ double u01_f64(uint64_t u) { return (double)u * 0x1p-64 + 0x1p-65; }
+__device__ __forceinline__ double uneg11_f64(uint64_t u) {
+    return (double)(int64_t)u * 0x1p-63 + 0x1p-64;
+}
+__device__ __forceinline__ float u01_f32(uint64_t u) { return (float)u * 0x1p-64f + 0x1p-65f; }
+
+// boxmuller(uint64, uint64) (Random123/boxmuller.hpp:128-137) with the
+// sincospi fallback sincos(PI * x) (boxmuller.hpp:104-107).
+__device__ __forceinline__ void boxmuller_f64(uint64_t u0, uint64_t u1, double& x, double& y) {
+    double s, c;
+    sincos(3.1415926535897932 * uneg11_f64(u0), &s, &c);
+    const double r = sqrt(-2. * log(u01_f64(u1)));
+    x = s * r;
+    y = c * r;
+}
+
+// FP32 Box-Muller for the mixed form: the same words, u1 -> (0,1] in float and
+// u0 -> [-1,1], sin/cos(pi x) and log in hardware-accelerated single precision.
+__device__ __forceinline__ void boxmuller_f32(uint64_t u0, uint64_t u1, float& x, float& y) {
+    const float a = (float)(int64_t)u0 * 0x1p-63f + 0x1p-64f;
+    const float b = (float)u1 * 0x1p-64f + 0x1p-65f;
+    float s, c;
+    __sincosf(3.14159265358979f * a, &s, &c);
+    const float r = __fsqrt_rn(-2.f * __logf(b));
+    x = s * r;
+    y = c * r;
+}
+
+// Word q of the stream keyed {k0, k1}.
+__device__ __forceinline__ uint64_t stream_word(uint64_t k0, uint64_t k1, uint64_t q) {
+    const u64x2 b = threefry2x64_20(q >> 1, 0, k0, k1);
+    return (q & 1) ? b.a : b.b;
+}
+
+// The Box-Muller pair built from words (P, P+1) of a stream.
+__device__ __forceinline__ void stream_bm_f64(uint64_t k0, uint64_t k1, uint64_t P, double& x,
+                                              double& y) {
+    uint64_t first, second;
+    if ((P & 1) == 0) {
+        const u64x2 b = threefry2x64_20(P >> 1, 0, k0, k1);
+        first = b.b;
+        second = b.a;
+    } else {
+        first = stream_word(k0, k1, P);
+        second = stream_word(k0, k1, P + 1);
+    }
+    boxmuller_f64(second, first, x, y);
+}
+
+// Persistent state of one reference stream between calls: P words handed out,
+// plus the cached second normal of the last Box-Muller pair.
+struct StreamState {
+    uint64_t P;
+    double saved;
+    uint32_t is_saved;
+    uint32_t pad;
+};
+
+// Uniform (wave-uniform in practice) sequential draws on a stream state.
+__device__ __forceinline__ double st_randn_f64(uint64_t k0, uint64_t k1, StreamState& s) {
+    if (s.is_saved) {
+        s.is_saved = 0;
+        return s.saved;
+    }
+    double x, y;
+    stream_bm_f64(k0, k1, s.P, x, y);
+    s.P += 2;
+    s.saved = y;
+    s.is_saved = 1;
+    return x;
+}
+
+__device__ __forceinline__ double st_randu_f64(uint64_t k0, uint64_t k1, StreamState& s) {
+    return u01_f64(stream_word(k0, k1, s.P++));
+}
+
+// Normal number n (0-based) of the normal-only draw sequence that starts at
+// state s: normal 0 is the cached one when s.is_saved, the rest come in
+// Box-Muller pairs from words P, P+2, ...
+__device__ __forceinline__ double normal_at_f64(uint64_t k0, uint64_t k1, const StreamState& s,
+                                                uint64_t n) {
+    if (s.is_saved) {
+        if (n == 0) return s.saved;
+        n -= 1;
+    }
+    double x, y;
+    stream_bm_f64(k0, k1, s.P + 2 * (n >> 1), x, y);
+    return (n & 1) ? y : x;
+}
+
+// State after consuming k normals from s (normal-only consumption).
+__device__ __forceinline__ StreamState advance_normals_f64(uint64_t k0, uint64_t k1, StreamState s,
+                                                           uint64_t k) {
+    if (k == 0) return s;
+    if (s.is_saved) {
+        s.is_saved = 0;
+        k -= 1;
+    }
+    const uint64_t pairs = (k + 1) >> 1;
+    if (k & 1) {
+        double x, y;
+        stream_bm_f64(k0, k1, s.P + 2 * (pairs - 1), x, y);
+        s.saved = y;
+        s.is_saved = 1;
+    }
+    s.P += 2 * pairs;
+    return s;
+}
+
+}  // namespace nmlmc

@@ -1,0 +1,289 @@
+"""Python bindings of libnmlmc.so, the MI355X nested sampler.
+
+The C ABI (include/nmlmc.h) mirrors the reference's extern "C" entry points
+(common.cpp:20-31, model_sampler.cpp:94-114, portfolio_sampler.cpp:618-756)
+with its -DREAL_DOUBLE layouts; this module exposes them with the same names
+plus small helpers.  The library is the only compute path: if it is missing,
+loading fails loudly (there is no CPU fallback).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libnmlmc.so")
+
+FORM_PLAIN, FORM_BATCHED, FORM_MIXED = 0, 1, 2
+PHI_INDICATOR, PHI_MAX, PHI_MOMENTS, PHI_FMOMENTS = 0, 1, 2, 3
+MODEL_REF, MODEL_SHIFT = 0, 1
+METHOD_BS, METHOD_EXACT, METHOD_UNBIASED = 0, 1, 2
+CALL, PUT, DIGITAL_CALL, DIGITAL_PUT = 1, -1, 2, -2
+
+
+class NestedSamplerParams(C.Structure):
+    """nested_sampler_params (nested_sampler.hpp:14-34), REAL_DOUBLE layout."""
+    _fields_ = [("N0", C.c_uint32), ("beta", C.c_uint32), ("antithetic", C.c_bool),
+                ("fine_moments", C.c_uint32), ("diff_moments", C.c_uint32),
+                ("adaptive_r", C.c_double)]
+
+
+class SampleOutput(C.Structure):
+    """sample_output (common.hpp:37-55)."""
+    _fields_ = [("M", C.c_uint64), ("fsums", C.POINTER(C.c_double)),
+                ("diffsums", C.POINTER(C.c_double)), ("total_work", C.c_double),
+                ("inner_stats", C.c_uint64 * 6)]
+
+
+class ModelData(C.Structure):
+    """model::data (model_sampler.cpp:21-39)."""
+    _fields_ = [("stau", C.c_double), ("s1tau", C.c_double), ("max_loss", C.c_double),
+                ("phi_type", C.c_int32)]
+
+
+class Span(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("size", C.c_size_t)]
+
+
+class OptionParams(C.Structure):
+    """portfolio::option_params (portfolio_sampler.cpp:42-52), REAL_DOUBLE layout."""
+    _fields_ = [("strike", C.c_double), ("maturity", C.c_double), ("start", C.c_double),
+                ("w", C.c_double), ("delta_est", C.c_double), ("stock_ind", C.c_uint32),
+                ("type", C.c_int32), ("prob", C.c_double), ("W_est", C.c_double)]
+
+
+class StockParams(C.Structure):
+    _fields_ = [("S0", C.c_double), ("mu", C.c_double), ("sigma", C.c_double)]
+
+
+class PortfolioData(C.Structure):
+    """portfolio::data (portfolio_sampler.cpp:62-92), REAL_DOUBLE layout."""
+    _fields_ = [("options", Span * 3), ("stocks", Span), ("r", C.c_double), ("rho", C.c_double),
+                ("s1_rho2", C.c_double), ("max_loss", C.c_double), ("tau", C.c_double),
+                ("dt_h0inv", C.c_uint32), ("dt_beta", C.c_uint32), ("dt_prob_expo", C.c_double),
+                ("phi_type", C.c_int32), ("_phi_size", C.c_uint32), ("subsample", C.c_int32),
+                ("use_delta_cv", C.c_bool), ("use_coarse_delta_cv", C.c_bool),
+                ("use_antithetic_risk", C.c_bool), ("use_milstein", C.c_bool)]
+
+
+class LevelStats(C.Structure):
+    """nmlmc_level_stats: mlmc_t::mc_diff_output_t (mlmc.hpp:21-27), 4 moments."""
+    _fields_ = [("sums_fine", C.c_double * 4), ("sums_diff", C.c_double * 4),
+                ("work", C.c_double), ("time", C.c_double), ("M", C.c_uint64)]
+
+
+SAMPLE_LEVEL_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_uint32, C.c_uint64, C.POINTER(LevelStats))
+
+EXPORTS = [
+    "init_nestedmc_sampler", "cleanup_nestedmc_sampler", "model_update_data", "model_init",
+    "model_sample", "portfolio_cleanup", "portfolio_update_data", "portfolio_init",
+    "portfolio_comp_params", "portfolio_sample", "nmlmc_last_error", "nmlmc_last_error_string",
+    "nmlmc_set_form", "nmlmc_get_form", "nmlmc_set_adaptive_rule", "nmlmc_model_init",
+    "nmlmc_portfolio_set_correlation", "nmlmc_sample", "nmlmc_mixed_stats",
+    "nmlmc_last_kernel_ms", "nmlmc_last_kernel_name", "nmlmc_last_group_lanes",
+    "nmlmc_mlmc_run", "nmlmc_mlmc_run_cb",
+]
+
+_lib = None
+
+
+class NmlmcError(RuntimeError):
+    pass
+
+
+def load(path=LIB_PATH):
+    """Load libnmlmc.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise NmlmcError("%s is missing: build it with `make -C nested-risk-mlmc_amd`" % path)
+    lib = C.CDLL(path)
+    u32, u64, dbl, i32 = C.c_uint32, C.c_uint64, C.c_double, C.c_int
+    dp, u64p = C.POINTER(C.c_double), C.POINTER(C.c_uint64)
+    sig = {
+        "init_nestedmc_sampler": (None, [u32, C.POINTER(NestedSamplerParams)]),
+        "cleanup_nestedmc_sampler": (None, []),
+        "model_update_data": (None, [C.POINTER(ModelData)]),
+        "model_init": (None, [C.POINTER(ModelData), u64]),
+        "model_sample": (None, [u32, u64, C.POINTER(SampleOutput)]),
+        "portfolio_cleanup": (None, []),
+        "portfolio_update_data": (None, [C.POINTER(PortfolioData)]),
+        "portfolio_init": (None, [C.POINTER(PortfolioData), u64]),
+        "portfolio_comp_params": (None, [C.POINTER(PortfolioData)]),
+        "portfolio_sample": (None, [u32, u64, u64, C.POINTER(SampleOutput)]),
+        "nmlmc_last_error": (i32, []),
+        "nmlmc_last_error_string": (C.c_char_p, []),
+        "nmlmc_set_form": (i32, [i32]),
+        "nmlmc_get_form": (i32, []),
+        "nmlmc_set_adaptive_rule": (i32, [u32, u32]),
+        "nmlmc_model_init": (i32, [i32, i32, dbl, dbl, u64]),
+        "nmlmc_portfolio_set_correlation": (i32, [dp, u32]),
+        "nmlmc_sample": (i32, [u32, u64, u64, C.POINTER(SampleOutput), dp, dp, u64p, u64p]),
+        "nmlmc_mixed_stats": (i32, [u64p, u64p]),
+        "nmlmc_last_kernel_ms": (dbl, []),
+        "nmlmc_last_kernel_name": (C.c_char_p, []),
+        "nmlmc_last_group_lanes": (u32, []),
+        "nmlmc_mlmc_run": (i32, [dbl, u64, dbl, dbl, dbl, i32, u32, C.POINTER(LevelStats),
+                                 C.POINTER(C.c_uint32), dp]),
+        "nmlmc_mlmc_run_cb": (i32, [SAMPLE_LEVEL_FN, C.c_void_p, dbl, u64, dbl, dbl, dbl, i32, u32,
+                                    C.POINTER(LevelStats), C.POINTER(C.c_uint32), dp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc=None):
+    """Raise NmlmcError if the last call failed."""
+    lib = load()
+    code = lib.nmlmc_last_error() if rc is None else rc
+    if code:
+        raise NmlmcError("nmlmc error %d: %s" % (code, lib.nmlmc_last_error_string().decode()))
+
+
+class Output:
+    """Owns the arrays of a sample_output."""
+
+    def __init__(self, moments=4, phi_size=1):
+        self.fsums = np.zeros(moments * phi_size)
+        self.diffsums = np.zeros(moments * phi_size)
+        self.c = SampleOutput()
+        self.c.fsums = self.fsums.ctypes.data_as(C.POINTER(C.c_double))
+        self.c.diffsums = self.diffsums.ctypes.data_as(C.POINTER(C.c_double))
+
+    def as_dict(self):
+        return dict(M=self.c.M, fsums=self.fsums.copy(), diffsums=self.diffsums.copy(),
+                    work=self.c.total_work, stats=np.array(list(self.c.inner_stats), np.uint64))
+
+
+# ---- reference entry points ---------------------------------------------------
+
+def init_nestedmc_sampler(cores, N0, beta, adaptive_r=0.0, antithetic=True, fine_moments=4,
+                          diff_moments=4):
+    lib = load()
+    p = NestedSamplerParams(N0, beta, bool(antithetic), fine_moments, diff_moments, adaptive_r)
+    lib.init_nestedmc_sampler(cores, C.byref(p))
+    check()
+
+
+def cleanup_nestedmc_sampler():
+    load().cleanup_nestedmc_sampler()
+
+
+def model_init(phi_type, tau, max_loss, seed):
+    """model_init with model::data(phi_type, tau, max_loss) (model_sampler.cpp:24-29)."""
+    d = ModelData(np.sqrt(tau), np.sqrt(1 - tau), max_loss, phi_type)
+    load().model_init(C.byref(d), seed)
+    check()
+
+
+def model_sample(ell, M, moments=4):
+    o = Output(moments)
+    load().model_sample(ell, M, C.byref(o.c))
+    check()
+    return o.as_dict()
+
+
+# ---- extensions ---------------------------------------------------------------
+
+def set_form(form):
+    check(load().nmlmc_set_form(form))
+
+
+def set_adaptive_rule(rule, cap_log2=0):
+    check(load().nmlmc_set_adaptive_rule(rule, cap_log2))
+
+
+def model_init_ex(variant, phi_type, tau, max_loss, seed):
+    check(load().nmlmc_model_init(variant, phi_type, tau, max_loss, seed))
+
+
+def sample(ell, M, outer_offset=None, trace=False, moments=4, phi_size=1):
+    """nmlmc_sample: one level call; returns sums, work, stats (and the trace)."""
+    lib = load()
+    o = Output(moments, phi_size)
+    off = (1 << 64) - 1 if outer_offset is None else outer_offset
+    tr = None
+    ptrs = [None] * 4
+    if trace:
+        tr = (np.zeros(M), np.zeros(M), np.zeros(M, np.uint64), np.zeros(M, np.uint64))
+        ptrs = [tr[0].ctypes.data_as(C.POINTER(C.c_double)),
+                tr[1].ctypes.data_as(C.POINTER(C.c_double)),
+                tr[2].ctypes.data_as(C.POINTER(C.c_uint64)),
+                tr[3].ctypes.data_as(C.POINTER(C.c_uint64))]
+    check(lib.nmlmc_sample(ell, M, off, C.byref(o.c), *ptrs))
+    out = o.as_dict()
+    if trace:
+        out.update(phi_f=tr[0], phi_c=tr[1], nf=tr[2], nc=tr[3])
+    return out
+
+
+def mixed_stats():
+    a, b = C.c_uint64(), C.c_uint64()
+    check(load().nmlmc_mixed_stats(C.byref(a), C.byref(b)))
+    return a.value, b.value
+
+
+def last_kernel_ms():
+    return load().nmlmc_last_kernel_ms()
+
+
+def last_kernel_name():
+    return load().nmlmc_last_kernel_name().decode()
+
+
+def last_group_lanes():
+    return load().nmlmc_last_group_lanes()
+
+
+def _run_outputs(lib_call, max_levels):
+    levels = (LevelStats * max_levels)()
+    n = C.c_uint32()
+    summ = (C.c_double * 5)()
+    rc = lib_call(levels, n, summ)
+    check(rc)
+    lv = []
+    for i in range(min(n.value, max_levels)):
+        L = levels[i]
+        lv.append(dict(sums_fine=list(L.sums_fine), sums_diff=list(L.sums_diff), work=L.work,
+                       time=L.time, M=L.M))
+    return dict(levels=lv, starting_level=int(summ[0]), bias_est=summ[1], stat_error=summ[2],
+                TOL=summ[3], wall=summ[4])
+
+
+def mlmc_run(targetTOL, M0, theta=0.5, Ca=3.0, starting_level_coeff=1.5, verbose=False,
+             max_levels=64):
+    """mlmc_t<double,4>::run over the staged problem (mlmc.hpp:290-337)."""
+    lib = load()
+    return _run_outputs(lambda lv, n, s: lib.nmlmc_mlmc_run(
+        targetTOL, M0, theta, Ca, starting_level_coeff, int(verbose), max_levels, lv,
+        C.byref(n), s), max_levels)
+
+
+def mlmc_run_cb(fn, targetTOL, M0, theta=0.5, Ca=3.0, starting_level_coeff=1.5, verbose=False,
+                max_levels=64):
+    """The same driver over a Python level sampler fn(ell, M) -> dict(sums_fine, sums_diff,
+    work, M[, time])."""
+    lib = load()
+
+    def _cb(user, ell, M, out):
+        try:
+            r = fn(ell, M)
+            for k in range(4):
+                out.contents.sums_fine[k] = r["sums_fine"][k]
+                out.contents.sums_diff[k] = r["sums_diff"][k]
+            out.contents.work = r["work"]
+            out.contents.time = r.get("time", 0.0)
+            out.contents.M = r["M"]
+            return 0
+        except Exception:  # noqa: BLE001 - reported as an aborted run
+            return 1
+
+    cb = SAMPLE_LEVEL_FN(_cb)
+    return _run_outputs(lambda lv, n, s: lib.nmlmc_mlmc_run_cb(
+        cb, None, targetTOL, M0, theta, Ca, starting_level_coeff, int(verbose), max_levels, lv,
+        C.byref(n), s), max_levels)

@@ -1,0 +1,170 @@
+"""GPU parity of the model problem: HIP library vs the oracle on the same seeded
+inputs (per outer sample), plus the closed-form answer of the shift model.
+
+Tolerances: counts, work and inner_stats are integers and must match exactly;
+phi values and sums are FP64 and must agree to 1e-12 relative (the GPU uses the
+device libm for log/sincos and a parallel summation order)."""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-12
+
+
+def _cmp(g, o, trace=True):
+    assert np.array_equal(g["stats"], o["stats"]), (g["stats"], o["stats"])
+    assert g["work"] == o["work"]
+    np.testing.assert_allclose(g["fsums"], o["fsums"], rtol=RTOL, atol=1e-300)
+    np.testing.assert_allclose(g["diffsums"], o["diffsums"], rtol=RTOL, atol=1e-300)
+    if trace:
+        assert np.array_equal(g["nf"], o["nf"])
+        assert np.array_equal(g["nc"], o["nc"])
+        np.testing.assert_allclose(g["phi_f"], o["phi_f"], rtol=RTOL, atol=1e-300)
+        np.testing.assert_allclose(g["phi_c"], o["phi_c"], rtol=RTOL, atol=1e-300)
+
+
+def _ref_max_loss(orc, tau=0.02, eta=0.025):
+    # model_sampler.cpp:144: max_loss = tau (Phi^-1(eta/2)^2 - 1)
+    return tau * (orc.norm_cdf_inv(eta / 2) ** 2 - 1)
+
+
+@pytest.mark.parametrize("cores,adaptive_r,anti,phi", [
+    (1, 0.0, True, 0), (8, 1.5, True, 0), (64, 1.5, True, 1), (64, 0.0, False, 1),
+    (32, 1.5, False, 0), (3, 1.5, True, 1)])
+def test_plain_reference_model_matches_oracle(nm, oracle_f64, cores, adaptive_r, anti, phi):
+    """Plain FP64 form with the reference keying, reference model
+    (model_sampler.cpp:42-92): per-sample phi_f, phi_c, N_f, N_c and level sums
+    over consecutive calls (stream state persists across calls and levels)."""
+    print("case", cores, adaptive_r, anti, phi, flush=True)
+    N0, beta, tau = 32, 4, 0.02
+    orc = oracle_f64.new(cores, N0, beta, adaptive_r, anti)
+    ml = _ref_max_loss(orc, tau)
+    nm.init_nestedmc_sampler(cores, N0, beta, adaptive_r, anti)
+    nm.model_init(phi, tau, ml, 11)
+    orc.model_init(0, phi, tau, ml, 11)
+    for ell, M in ((0, 300), (1, 200), (3, 77), (2, 130), (0, 5)):
+        _cmp(nm.sample(ell, M, trace=True), orc.sample(ell, M, trace=True))
+    nm.cleanup_nestedmc_sampler()
+
+
+@pytest.mark.parametrize("lanes", [1, 4, 16, 64])
+def test_plain_lane_groups_do_not_change_results(nm, oracle_f64, lanes, monkeypatch):
+    """The lanes-per-virtual-thread split is invisible in the results."""
+    monkeypatch.setenv("NMLMC_GROUP_LANES", str(lanes))
+    orc = oracle_f64.new(16, 32, 4, 1.5, True)
+    ml = _ref_max_loss(orc)
+    nm.init_nestedmc_sampler(16, 32, 4, 1.5, True)
+    nm.model_init(0, 0.02, ml, 3)
+    orc.model_init(0, 0, 0.02, ml, 3)
+    for ell, M in ((0, 100), (2, 60)):
+        _cmp(nm.sample(ell, M, trace=True), orc.sample(ell, M, trace=True))
+        assert nm.last_group_lanes() == lanes
+    nm.cleanup_nestedmc_sampler()
+
+
+def test_drop_in_model_sample_entry(nm, oracle_f64):
+    """model_sample (model_sampler.cpp:106-114) through the reference-named entry."""
+    orc = oracle_f64.new(8, 32, 4, 1.5, True)
+    ml = _ref_max_loss(orc)
+    nm.init_nestedmc_sampler(8, 32, 4, 1.5, True)
+    nm.model_init(0, 0.02, ml, 0)
+    orc.model_init(0, 0, 0.02, ml, 0)
+    for ell, M in ((0, 64), (1, 33)):
+        g = nm.model_sample(ell, M)
+        o = orc.sample(ell, M)
+        assert g["M"] == M
+        _cmp(g, o, trace=False)
+    nm.cleanup_nestedmc_sampler()
+
+
+def _shift_setup(nm, orc, cores, form, seed=1, cap=6):
+    nm.init_nestedmc_sampler(cores, 32, 2, 0.0, True)
+    nm.set_adaptive_rule(1, cap)
+    nm.set_form(form)
+    nm.model_init_ex(nm.MODEL_SHIFT, 0, 0.5, 2.0, seed)
+    o = orc.new(cores, 32, 2, 0.0, True, adapt_rule=1, cap_log2=cap)
+    o.model_init(1, 0, 0.5, 2.0, seed)
+    return o
+
+
+def test_plain_shift_model_capped_rule(nm, oracle_f64):
+    """Shift model X = Y + Z, threshold 2, 3-sigma doubling capped at 2^6."""
+    orc = _shift_setup(nm, oracle_f64, 32, nm.FORM_PLAIN)
+    for ell, M in ((0, 400), (1, 300), (4, 60)):
+        _cmp(nm.sample(ell, M, trace=True), orc.sample(ell, M, trace=True))
+    nm.cleanup_nestedmc_sampler()
+
+
+@pytest.mark.parametrize("lanes", [0, 1, 8, 64])
+def test_batched_fp64_matches_oracle(nm, oracle_f64, lanes, monkeypatch):
+    """Batched FP64 form: per-outer-sample keys; identical per-sample results to
+    the oracle's batched keying for any lane grouping."""
+    if lanes:
+        monkeypatch.setenv("NMLMC_GROUP_LANES", str(lanes))
+    orc = _shift_setup(nm, oracle_f64, 1, nm.FORM_BATCHED)
+    for ell, M, off in ((0, 500, 0), (1, 300, 0), (3, 100, 0), (1, 200, 300)):
+        g = nm.sample(ell, M, outer_offset=off, trace=True)
+        o = orc.sample(ell, M, keying=1, outer_offset=off, trace=True)
+        _cmp(g, o)
+    nm.cleanup_nestedmc_sampler()
+    nm.set_form(nm.FORM_PLAIN)
+
+
+def test_batched_reference_model_nonantithetic(nm, oracle_f64):
+    """Batched keying with the reference model and the plain (non-antithetic)
+    fine/coarse estimator."""
+    nm.init_nestedmc_sampler(1, 32, 4, 1.5, False)
+    nm.set_form(nm.FORM_BATCHED)
+    orc = oracle_f64.new(1, 32, 4, 1.5, False)
+    ml = _ref_max_loss(orc)
+    nm.model_init(1, 0.02, ml, 9)
+    orc.model_init(0, 1, 0.02, ml, 9)
+    for ell, M in ((0, 200), (2, 150)):
+        _cmp(nm.sample(ell, M, outer_offset=0, trace=True),
+             orc.sample(ell, M, keying=1, outer_offset=0, trace=True))
+    nm.cleanup_nestedmc_sampler()
+    nm.set_form(nm.FORM_PLAIN)
+
+
+def test_mixed_form_agrees_with_fp64(nm, oracle_f64):
+    """Mixed FP32+FP64 form against the FP64 oracle with the same keying: every
+    indicator equals the FP64 one (re-evaluation near the threshold), sample
+    counts differ only where an FP32 refinement test flipped."""
+    orc = _shift_setup(nm, oracle_f64, 1, nm.FORM_MIXED)
+    for ell, M in ((0, 20000), (2, 5000), (5, 500)):
+        g = nm.sample(ell, M, outer_offset=0, trace=True)
+        o = orc.sample(ell, M, keying=1, outer_offset=0, trace=True)
+        same = (g["nf"] == o["nf"]) & (g["nc"] == o["nc"])
+        assert same.mean() > 0.999, same.mean()
+        assert np.array_equal(g["phi_f"][same], o["phi_f"][same])
+        assert np.array_equal(g["phi_c"][same], o["phi_c"][same])
+        reev, flips = nm.mixed_stats()
+        print("ell", ell, "count flips", int((~same).sum()), "re-evaluated", reev, "flips", flips)
+    nm.cleanup_nestedmc_sampler()
+    nm.set_form(nm.FORM_PLAIN)
+
+
+@pytest.mark.parametrize("form", [0, 1, 2])
+def test_shift_model_closed_form(nm, form):
+    """P(E[X|Y] > 2) = 1 - Phi(2) for the shift model: the MLMC telescoping sum
+    over levels 0..5 lies within 4 standard errors (plus the level-5 bias bound)
+    of the closed form."""
+    nm.init_nestedmc_sampler(4096 if form == 0 else 1, 32, 2, 0.0, True)
+    nm.set_adaptive_rule(1, 6)
+    nm.set_form(form)
+    nm.model_init_ex(nm.MODEL_SHIFT, 0, 0.5, 2.0, 123)
+    exact = 0.5 * math.erfc(2 / math.sqrt(2))
+    est, var = 0.0, 0.0
+    for ell, M in ((0, 400000), (1, 200000), (2, 100000), (3, 50000), (4, 25000), (5, 12000)):
+        o = nm.sample(ell, M)
+        s = o["fsums"] if ell == 0 else o["diffsums"]
+        mean = s[0] / M
+        est += mean
+        var += (s[1] / M - mean * mean) / M
+        last = abs(mean)
+    assert abs(est - exact) < 4 * math.sqrt(var) + 2 * last, (est, exact, math.sqrt(var), last)
+    nm.cleanup_nestedmc_sampler()
+    nm.set_form(nm.FORM_PLAIN)
