@@ -1,0 +1,238 @@
+#!/usr/bin/env python3
+"""Benchmark of the level-l nested sampler on one MI355X.
+
+Workload (the adaptive model problem): shift model X = Y + Z, Y, Z ~ N(0,1),
+threshold 2 (P = 1 - Phi(2)); levels 0..12; base inner count 32 * 2^l, each
+outer sample refined by doubling while |mean| < 3 stderr up to 2^6 x; antithetic
+fine/coarse estimator; M_l = ceil(1e7 * 2^(-1.5 l)) outer samples per level.
+
+A step is one sweep of sample(l, M_l) over all levels with inputs generated on
+the device (counter-based RNG).  value = outer samples / s of the chosen form
+(batched FP32+FP64, "mixed") over the timed steps; the plain FP64 form (the
+reference keying, 65536 virtual threads) is timed beside it, per level, and
+both forms are driven to RMSE eps by the MLMC driver (time to eps).
+
+roofline: the dominant kernel is VALU-bound (integer Threefry + Box-Muller +
+compensated sums; HBM traffic is a few KB per launch), so achieved = algorithmic
+VALU lane-ops per launch (inner samples x ops per inner sample + outer samples
+x ops per outer sample, counts in nested-risk-mlmc_amd/opcount.json) / the
+kernel's duration from HIP events on the library's stream; peak = 256 CU x 64
+lanes x 2.4 GHz = 39.3 T lane-ops/s.
+
+cpu_baseline: the oracle (C restatement, FP64, same keying and workload) on the
+host cores with OpenMP, over a bounded sample of the sweep (M_l scaled down).
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_VALU_TOPS = 256 * 64 * 2.4e9 / 1e12  # 39.3 T lane-ops/s
+
+
+def level_M(M0, ell):
+    return int(math.ceil(M0 * 2.0 ** (-1.5 * ell)))
+
+
+def setup(nm, form, cores, seed=1):
+    nm.init_nestedmc_sampler(cores, 32, 2, 0.0, True)
+    nm.set_adaptive_rule(1, 6)
+    nm.set_form(form)
+    nm.model_init_ex(nm.MODEL_SHIFT, nm.PHI_INDICATOR, 0.5, 2.0, seed)
+
+
+def sweep(nm, M0, levels, rec=None):
+    for ell in range(levels + 1):
+        M = level_M(M0, ell)
+        t0 = time.perf_counter()
+        o = nm.sample(ell, M)
+        dt = time.perf_counter() - t0
+        if rec is not None:
+            reev, flips = nm.mixed_stats()
+            rec.append(dict(ell=ell, M=M, wall=dt, kernel_ms=nm.last_kernel_ms(),
+                            inner=int(o["stats"][0]), G=nm.last_group_lanes(), reev=reev,
+                            flips=flips))
+
+
+def ops_per_sample(form):
+    with open(os.path.join(ROOT, "nested-risk-mlmc_amd", "opcount.json")) as f:
+        oc = json.load(f)
+    key = "mixed" if form == 2 else "fp64"
+    return oc["shift_model"][key]["inner"], oc["shift_model"][key]["outer"]
+
+
+def roofline(recs, form):
+    oi, oo = ops_per_sample(form)
+    ops = sum(r["inner"] * oi + r["M"] * oo for r in recs)
+    secs = sum(r["kernel_ms"] for r in recs) / 1e3
+    achieved = ops / secs / 1e12
+    return {"bound": "valu", "achieved": round(achieved, 3), "peak": round(PEAK_VALU_TOPS, 1),
+            "unit": "TOP/s", "frac": round(achieved / PEAK_VALU_TOPS, 4), "traffic": None,
+            "kernel": "k_model_batched" if form else "k_model_parity",
+            "launches": len(recs), "avg_launch_ms": round(1e3 * secs / len(recs), 4),
+            "ops_per_inner_sample": oi}
+
+
+def per_level(recs, levels):
+    out = []
+    for ell in range(levels + 1):
+        rs = sorted(r["wall"] for r in recs if r["ell"] == ell)
+        M = next(r["M"] for r in recs if r["ell"] == ell)
+        out.append(round(M / rs[len(rs) // 2], 1))
+    return out
+
+
+def cpu_baseline(M0_cpu, levels):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import Oracle  # the checker, timed as the CPU baseline
+    threads = min(16, os.cpu_count() or 1)
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    orc = Oracle("f64").new(1, 32, 2, 0.0, True, adapt_rule=1, cap_log2=6)
+    orc.model_init(1, 0, 0.5, 2.0, 1)
+    import ctypes
+    try:
+        ctypes.CDLL("libgomp.so.1").omp_set_num_threads(threads)
+    except OSError:
+        pass
+    total = 0
+    t0 = time.perf_counter()
+    for ell in range(levels + 1):
+        M = level_M(M0_cpu, ell)
+        orc.sample(ell, M, keying=1, outer_offset=0)
+        total += M
+    dt = time.perf_counter() - t0
+    return {"value": round(total / dt, 1), "unit": "outer samples/s", "cores": threads,
+            "kind": "port",
+            "sample": "one level sweep l=0..%d with M_l = ceil(%g * 2^-1.5l) (%d outer samples, "
+                      "%.1f s), FP64, batched keying, OpenMP over fixed 256-sample chunks"
+                      % (levels, M0_cpu, total, dt)}
+
+
+def time_to_eps(nm, form, cores, eps, M0):
+    setup(nm, form, cores, seed=7)
+    t0 = time.perf_counter()
+    r = nm.mlmc_run(eps, M0)
+    dt = time.perf_counter() - t0
+    Eg = r["levels"][r["starting_level"]]["sums_fine"][0] / r["levels"][r["starting_level"]]["M"]
+    for lv in r["levels"][r["starting_level"] + 1:]:
+        Eg += lv["sums_diff"][0] / lv["M"]
+    rms = math.sqrt(r["bias_est"] ** 2 + r["stat_error"] ** 2)
+    nm.cleanup_nestedmc_sampler()
+    return {"seconds": round(dt, 3), "estimate": Eg, "rmse_est": rms, "levels": len(r["levels"]),
+            "stat_error": r["stat_error"]}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--M0", type=float, default=1e7)
+    ap.add_argument("--levels", type=int, default=12)
+    ap.add_argument("--plain-cores", type=int, default=65536)
+    ap.add_argument("--eps", type=float, default=1e-5)
+    ap.add_argument("--tte-M0", type=int, default=1024)
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-tte", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-M0", type=float, default=1e6)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (the outer samples "
+              "would shard with one all-reduce of ~300 B per level call); run with --gpus 1.")
+        sys.exit(2)
+
+    import __graft_entry__
+    nm = __graft_entry__.package()
+    nm.load()
+    chosen = nm.FORM_MIXED
+
+    # Chosen form: warmup, then K timed sweeps.
+    setup(nm, chosen, 1)
+    for _ in range(args.warmup):
+        sweep(nm, args.M0, args.levels)
+    recs = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        sweep(nm, args.M0, args.levels, recs)
+    elapsed = time.perf_counter() - t0
+    outer_per_step = sum(level_M(args.M0, l) for l in range(args.levels + 1))
+    value = outer_per_step * args.steps / elapsed
+    reev = sum(r["reev"] for r in recs)
+    flips = sum(r["flips"] for r in recs)
+    rl = roofline(recs, chosen)
+    levels_mixed = per_level(recs, args.levels)
+    print("# mixed: %.3e outer samples/s, %.1f ms/step, roofline %.3f of VALU peak"
+          % (value, 1e3 * elapsed / args.steps, rl["frac"]), file=sys.stderr, flush=True)
+    nm.cleanup_nestedmc_sampler()
+
+    extra = {}
+    if not args.no_plain:
+        setup(nm, nm.FORM_PLAIN, args.plain_cores)
+        sweep(nm, args.M0, args.levels)
+        precs = []
+        t0 = time.perf_counter()
+        sweep(nm, args.M0, args.levels, precs)
+        pel = time.perf_counter() - t0
+        extra["plain_fp64"] = {"outer_samples_per_s": round(outer_per_step / pel, 1),
+                               "per_level": per_level(precs, args.levels),
+                               "roofline_frac": roofline(precs, nm.FORM_PLAIN)["frac"],
+                               "virtual_threads": args.plain_cores}
+        print("# plain: %.3e outer samples/s" % (outer_per_step / pel), file=sys.stderr, flush=True)
+        nm.cleanup_nestedmc_sampler()
+        setup(nm, nm.FORM_BATCHED, 1)
+        sweep(nm, args.M0, args.levels)
+        brecs = []
+        t0 = time.perf_counter()
+        sweep(nm, args.M0, args.levels, brecs)
+        bel = time.perf_counter() - t0
+        extra["batched_fp64"] = {"outer_samples_per_s": round(outer_per_step / bel, 1),
+                                 "per_level": per_level(brecs, args.levels)}
+        nm.cleanup_nestedmc_sampler()
+    if not args.no_tte:
+        tm = time_to_eps(nm, chosen, 1, args.eps, args.tte_M0)
+        print("# time to eps (mixed): %s" % tm, file=sys.stderr, flush=True)
+        tp = time_to_eps(nm, nm.FORM_PLAIN, args.plain_cores, args.eps, args.tte_M0)
+        print("# time to eps (plain): %s" % tp, file=sys.stderr, flush=True)
+        exact = 0.5 * math.erfc(2 / math.sqrt(2))
+        extra["time_to_eps"] = {"eps": args.eps, "mixed": tm, "plain_fp64": tp,
+                                "speedup": round(tp["seconds"] / tm["seconds"], 2),
+                                "exact": exact}
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(args.cpu_M0, args.levels)
+
+    line = {
+        "metric": "outer samples/s per level; time to RMSE eps: batched FP32+FP64 vs plain FP64",
+        "value": round(value, 1),
+        "unit": "outer samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * elapsed / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic (counter-based Threefry draws generated on the device)",
+        "config": {"workload": "adaptive model problem: X=Y+Z, threshold 2, levels 0..%d, "
+                               "N_base=32*2^l, 3-sigma doubling capped at 2^6, "
+                               "M_l=ceil(%g*2^-1.5l)" % (args.levels, args.M0),
+                   "form": "batched FP32+FP64 (mixed)", "outer_samples_per_step": outer_per_step},
+        "per_level_outer_samples_per_s": levels_mixed,
+        "mixed_reevaluated": reev,
+        "mixed_flips": flips,
+        "roofline": rl,
+        "cpu_baseline": cpu,
+    }
+    line.update(extra)
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,548 @@
+// Kernels of the batched engine (see batched.hpp), templated on the problem
+// policy P and on the precision of the inner arithmetic.
+//
+// Launch order of one level call (host: Engine::run_batched_level):
+//   k_b_zero, k_b_init, k_b_test(r) for r = 0..R-1, k_b_final_count,
+//   k_b_final_alloc, k_b_final_fill, k_b_final, [k_b_reeval when MIXED]
+#pragma once
+#include "batched.hpp"
+
+namespace nmlmc {
+
+__device__ __forceinline__ uint32_t class_of(uint64_t n, uint64_t base_min) {
+    uint32_t c = 0;
+    while ((base_min << c) < n && c + 1 < (uint32_t)kMaxClasses) c++;
+    return c;
+}
+
+// sample_inner_delta2 + the refinement test (nested_sampler.hpp:120-129, 252-263):
+// true when the outer sample needs more inner samples than `cur`.
+__device__ __forceinline__ bool refine(const BatchedCommon& c, const LevelCounts& lc, double start,
+                                       uint64_t cur, double s1, double s2) {
+    const double out_var = s2 / (double)(cur - 1) - (s1 * s1) / (double)(cur * (cur - 1));
+    const double dist = start + s1 / (double)cur;
+    const double delta2 = dist * dist / out_var;
+    const double new_count = (lc.rule == 1) ? lc.tmp / delta2 : pow(delta2, -c.ep.adaptive_r / 2) * lc.tmp;
+    return !(new_count <= (double)cur);
+}
+
+__global__ void k_b_zero(BatchedCtl* ctl) {
+    uint32_t* p = (uint32_t*)ctl;
+    const uint32_t n = sizeof(BatchedCtl) / 4;
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
+}
+
+// Outer draws.  Round 0 tests every outer sample, so its queue is implicit
+// (entry e is outer sample e); its accumulators are only written when a draw
+// group spans several segments.
+template <class P>
+__global__ __launch_bounds__(256) void k_b_init(typename P::Args A) {
+    const BatchedCommon& c = A.c;
+    LaneAcc acc;
+    acc.zero();
+    const bool test_f = c.cf.max > 2 * c.cf.base;
+    const bool test_c = c.ell > 0 && c.cc.max > 2 * c.cc.base;
+    const bool acc_f = test_f && (c.cf.base + P::spu - 1) / P::spu > c.S;
+    const bool acc_c = test_c && (c.cc.base + P::spu - 1) / P::spu > c.S;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        c.ctl->cnt_f[0] = test_f ? (uint32_t)c.M : 0;
+        c.ctl->cnt_c[0] = test_c ? (uint32_t)c.M : 0;
+    }
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t m = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; m < c.M; m += stride) {
+        ItemState& it = c.items[m];
+        P::init_item(A, m, it, acc);
+        uint64_t nf = 0, nc = 0;
+        if (!test_f) {
+            nf = c.cf.max;
+            acc.st[1] += 1;
+            acc.st[2] += 1;
+            acc.st[3] += (c.cf.max == c.cf.base);
+        } else {
+            acc.st[0] += c.cf.base;
+        }
+        if (c.ell > 0) {
+            if (!test_c) nc = c.cc.max;
+            else acc.st[0] += c.cc.base;
+        }
+        it.nf = nf;
+        it.nc = nc;
+        if (acc_f) c.tf[0][m] = TestEntry{0., 0., 0., (uint32_t)m, 0};
+        if (acc_c) c.tc[0][m] = TestEntry{0., 0., 0., (uint32_t)m, 0};
+    }
+    block_flush(acc, &c.ctl->acc);
+}
+
+// Test round r of every outer sample still refining its fine or coarse count.
+template <class P, bool MIXED>
+__global__ __launch_bounds__(256) void k_b_test(typename P::Args A, uint32_t r) {
+    const BatchedCommon& c = A.c;
+    LaneAcc acc;
+    acc.zero();
+    const uint32_t nfe = aload_u32(&c.ctl->cnt_f[r]), nce = aload_u32(&c.ctl->cnt_c[r]);
+    const uint64_t cur_f = c.cf.base << r, cur_c = c.cc.base << r;
+    const uint64_t uf = (cur_f + P::spu - 1) / P::spu, uc = (cur_c + P::spu - 1) / P::spu;
+    const uint64_t spe_f = (uf + c.S - 1) / c.S, spe_c = (uc + c.S - 1) / c.S;
+    const uint64_t segs_f = (uint64_t)nfe * spe_f, total = segs_f + (uint64_t)nce * spe_c;
+    TestEntry* const lf = c.tf[r & 1];
+    TestEntry* const lc_ = c.tc[r & 1];
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < total; k += stride) {
+        const bool fine = k < segs_f;
+        const uint64_t kk = fine ? k : k - segs_f;
+        const uint64_t spe = fine ? spe_f : spe_c;
+        const uint64_t e = kk / spe, j = kk % spe;
+        TestEntry& te = fine ? lf[e] : lc_[e];
+        const uint32_t m = r == 0 ? (uint32_t)e : te.item;
+        const ItemState& it = c.items[m];
+        const uint64_t cur = fine ? cur_f : cur_c, units = fine ? uf : uc;
+        const uint64_t u0 = j * c.S, u1 = (u0 + c.S < units) ? u0 + c.S : units;
+        SegAcc s;
+        P::template units<MIXED>(A, m, it, (fine ? TAG_FINE_TEST : TAG_COARSE_TEST) + r, u0, u1, 0, cur, s);
+        acc.work += s.work;
+        acc.st[4] += s.nu;
+        acc.st[5] += s.nn;
+        bool complete = spe == 1;
+        double s1 = s.s1, s2 = s.s2;
+        if (!complete) {
+            aadd(&te.s1, s.s1);
+            aadd(&te.s2, s.s2);
+            wait_acks();
+            if (acount(&te.done, 1u) == spe - 1) {
+                complete = true;
+                s1 = aread(&te.s1);
+                s2 = aread(&te.s2);
+            }
+        }
+        bool append = false;
+        if (complete) {
+            const LevelCounts& lc = fine ? c.cf : c.cc;
+            uint64_t n = cur;
+            bool more = refine(c, lc, it.start, cur, s1, s2);
+            if (more) {
+                n = cur * 2;
+                if (lc.max <= 2 * n) {
+                    n = lc.max;
+                    more = false;
+                }
+            }
+            if (more) {
+                append = true;
+                acc.st[0] += n;
+            } else {
+                ItemState& w = c.items[m];
+                if (fine) {
+                    w.nf = n;
+                    acc.st[1] += 1;
+                    acc.st[2] += (n == lc.max);
+                    acc.st[3] += (n == lc.base);
+                } else {
+                    w.nc = n;
+                }
+            }
+        }
+        // Refined outer samples go to round r + 1 at doubled count.
+        const uint32_t sf = wave_append(&c.ctl->cnt_f[r + 1], append && fine);
+        const uint32_t sc = wave_append(&c.ctl->cnt_c[r + 1], append && !fine);
+        if (append) {
+            if (fine) c.tf[(r + 1) & 1][sf] = TestEntry{0., 0., 0., m, 0};
+            else c.tc[(r + 1) & 1][sc] = TestEntry{0., 0., 0., m, 0};
+        }
+    }
+    block_flush(acc, &c.ctl->acc);
+}
+
+// Final draw groups of an outer sample: count and shape.
+struct FinalShape {
+    uint32_t groups;  // entries
+    uint64_t n;       // samples per antithetic group (kind 0)
+    uint64_t big;
+    bool f_big;
+};
+
+__device__ __forceinline__ FinalShape final_shape(const BatchedCommon& c, const ItemState& it) {
+    FinalShape s;
+    if (c.ell == 0) {
+        s.groups = 1;
+        s.n = s.big = it.nf;
+        s.f_big = true;
+    } else if (c.ep.antithetic) {
+        s.f_big = it.nf > it.nc;
+        s.big = s.f_big ? it.nf : it.nc;
+        s.n = s.f_big ? it.nc : it.nf;
+        s.groups = (uint32_t)(s.big / s.n);
+    } else {
+        s.groups = 2;
+        s.n = 0;
+        s.big = 0;
+        s.f_big = true;
+    }
+    return s;
+}
+
+// Units of an outer sample's whole final (all its final draw groups).
+template <class P>
+__device__ __forceinline__ uint64_t final_units(const BatchedCommon& c, const ItemState& it, const FinalShape& s) {
+    if (c.ell > 0 && !c.ep.antithetic) return (it.nf + P::spu - 1) / P::spu + (it.nc + P::spu - 1) / P::spu;
+    return (uint64_t)s.groups * ((s.n + P::spu - 1) / P::spu);
+}
+
+template <class P>
+__global__ __launch_bounds__(256) void k_b_final_count(typename P::Args A) {
+    const BatchedCommon& c = A.c;
+    __shared__ uint32_t hist[kMaxClasses + 1];
+    for (int i = threadIdx.x; i <= kMaxClasses; i += blockDim.x) hist[i] = 0;
+    __syncthreads();
+    LaneAcc acc;
+    acc.zero();
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t m = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; m < c.M; m += stride) {
+        const ItemState& it = c.items[m];
+        const FinalShape s = final_shape(c, it);
+        const bool plain2 = c.ell > 0 && !c.ep.antithetic;
+        acc.st[0] += plain2 ? it.nf + it.nc : s.big;
+        if (final_units<P>(c, it, s) <= c.S) {
+            atomicAdd(&hist[kMaxClasses], 1u);  // one lane finishes it alone
+        } else if (plain2) {
+            atomicAdd(&hist[class_of(it.nf, c.base_min)], 1u);
+            atomicAdd(&hist[class_of(it.nc, c.base_min)], 1u);
+        } else {
+            atomicAdd(&hist[class_of(s.n, c.base_min)], s.groups);
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kMaxClasses; i += blockDim.x)
+        if (hist[i]) atomicAdd(&c.ctl->class_cnt[i], hist[i]);
+    if (threadIdx.x == 0 && hist[kMaxClasses]) atomicAdd(&c.ctl->n_whole, hist[kMaxClasses]);
+    block_flush(acc, &c.ctl->acc);
+}
+
+__global__ void k_b_final_alloc(BatchedCtl* ctl, uint32_t fcap) {
+    if (threadIdx.x != 0) return;
+    uint32_t off = 0;
+    for (int i = 0; i < kMaxClasses; i++) {
+        ctl->class_off[i] = off;
+        ctl->class_cur[i] = 0;
+        off += ctl->class_cnt[i];
+    }
+    ctl->class_off[kMaxClasses] = off;
+    ctl->final_total = off + ctl->n_whole;
+    ctl->overflow = ctl->final_total > fcap;
+    ctl->n_whole = 0;  // refilled as a cursor by k_b_final_fill
+}
+
+// Writes each outer sample's final entries into its size class's region; block
+// ranks come from LDS counters, one global atomic per class and block.
+// Writes each outer sample's final entries: whole samples after the class
+// regions, split samples into their size class's region; block ranks come
+// from LDS counters, one global atomic per class and block.
+template <class P>
+__global__ __launch_bounds__(256) void k_b_final_fill(typename P::Args A) {
+    const BatchedCommon& c = A.c;
+    if (c.ctl->overflow) return;
+    __shared__ uint32_t cnt[kMaxClasses + 1], base[kMaxClasses + 1];
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint32_t whole_off = c.ctl->class_off[kMaxClasses];
+    for (uint64_t b0 = (uint64_t)blockIdx.x * blockDim.x; b0 < c.M; b0 += stride) {
+        for (int i = threadIdx.x; i <= kMaxClasses; i += blockDim.x) cnt[i] = 0;
+        __syncthreads();
+        const uint64_t m = b0 + threadIdx.x;
+        FinalShape s{};
+        uint32_t cls_a = 0, cls_b = 0, rank_a = 0, rank_b = 0;
+        const bool valid = m < c.M;
+        const bool plain2 = c.ell > 0 && !c.ep.antithetic;
+        bool whole = false;
+        if (valid) {
+            ItemState& it = c.items[m];
+            s = final_shape(c, it);
+            whole = final_units<P>(c, it, s) <= c.S;
+            if (whole) {
+                cls_a = kMaxClasses;
+                rank_a = atomicAdd(&cnt[kMaxClasses], 1u);
+            } else {
+                it.left = plain2 ? 2u : s.groups;
+                it.near = 0;
+                it.phi_acc = it.s1_big = it.sab_big = 0.;
+                if (plain2) {
+                    cls_a = class_of(it.nf, c.base_min);
+                    cls_b = class_of(it.nc, c.base_min);
+                    rank_a = atomicAdd(&cnt[cls_a], 1u);
+                    rank_b = atomicAdd(&cnt[cls_b], 1u);
+                } else {
+                    cls_a = class_of(s.n, c.base_min);
+                    rank_a = atomicAdd(&cnt[cls_a], s.groups);
+                }
+            }
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < kMaxClasses; i += blockDim.x)
+            base[i] = cnt[i] ? atomicAdd(&c.ctl->class_cur[i], cnt[i]) : 0;
+        if (threadIdx.x == 0) base[kMaxClasses] = cnt[kMaxClasses] ? atomicAdd(&c.ctl->n_whole, cnt[kMaxClasses]) : 0;
+        __syncthreads();
+        if (valid) {
+            if (whole) {
+                c.fpool[whole_off + base[kMaxClasses] + rank_a] = FinalEntry{0., 0., (uint32_t)m, 0, 0, 3};
+            } else if (plain2) {
+                c.fpool[c.ctl->class_off[cls_a] + base[cls_a] + rank_a] = FinalEntry{0., 0., (uint32_t)m, 0, 0, 1};
+                c.fpool[c.ctl->class_off[cls_b] + base[cls_b] + rank_b] = FinalEntry{0., 0., (uint32_t)m, 0, 0, 2};
+            } else {
+                FinalEntry* dst = c.fpool + c.ctl->class_off[cls_a] + base[cls_a] + rank_a;
+                for (uint32_t g = 0; g < s.groups; g++) dst[g] = FinalEntry{0., 0., (uint32_t)m, g, 0, 0};
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Records a finished outer sample: moments, trace (the reference's lines 446-451).
+__device__ __forceinline__ void finish_outer(const BatchedCommon& c, uint64_t m, const ItemState& it,
+                                             double phi_f, double phi_c, LaneAcc& acc) {
+    add_moments(&phi_f, 1, acc.f, c.ep.fine_moments);
+    if (c.ell > 0) {
+        const double dphi = phi_f - phi_c;
+        add_moments(&dphi, 1, acc.d, c.ep.diff_moments);
+    }
+    if (c.tr_phi_f) c.tr_phi_f[m] = phi_f;
+    if (c.tr_phi_c) c.tr_phi_c[m] = c.ell == 0 ? 0. : phi_c;
+    if (c.tr_nf) c.tr_nf[m] = it.nf;
+    if (c.tr_nc) c.tr_nc[m] = c.ell == 0 ? 0 : it.nc;
+}
+
+// phi of an outer sample from its finished final groups (sample_phi /
+// sample_phi_antithetic, nested_sampler.hpp:164-216).
+__device__ __forceinline__ void outer_phi(const BatchedCommon& c, const ItemState& it, double s1_big,
+                                          double phi_acc, double phi_f_na, double phi_c_na,
+                                          double& phi_f, double& phi_c) {
+    if (c.ell == 0) {
+        phi_f = phi_scalar(c.phi_type, it.start + s1_big / (double)it.nf, 1., 0.);
+        phi_c = 0.;
+    } else if (c.ep.antithetic) {
+        const FinalShape s = final_shape(c, it);
+        const double phi_big = phi_scalar(c.phi_type, it.start + s1_big / (double)s.big, 1., 0.);
+        phi_f = s.f_big ? phi_big : phi_acc;
+        phi_c = s.f_big ? phi_acc : phi_big;
+    } else {
+        phi_f = phi_f_na;
+        phi_c = phi_c_na;
+    }
+}
+
+// Sum of inner samples [i0, i1) of draw group `tag`, by one lane.
+template <class P, bool F32>
+__device__ __forceinline__ SegAcc lane_group(const typename P::Args& A, uint32_t m, const ItemState& it,
+                                             uint32_t tag, uint64_t i0, uint64_t i1) {
+    SegAcc s;
+    P::template units<F32>(A, m, it, tag, i0 / P::spu, (i1 + P::spu - 1) / P::spu, i0, i1, s);
+    return s;
+}
+
+template <class P, bool MIXED>
+__global__ __launch_bounds__(256) void k_b_final(typename P::Args A) {
+    const BatchedCommon& c = A.c;
+    LaneAcc acc;
+    acc.zero();
+    if (c.ctl->overflow) {
+        block_flush(acc, &c.ctl->acc);
+        return;
+    }
+    // Split entries (per size class, largest first in memory order of classes)
+    // then whole outer samples.
+    uint64_t total_split = 0;
+    for (int i = 0; i < kMaxClasses; i++) {
+        const uint64_t units = ((c.base_min << i) + P::spu - 1) / P::spu;
+        total_split += (uint64_t)c.ctl->class_cnt[i] * ((units + c.S - 1) / c.S);
+    }
+    const uint32_t n_whole = c.ctl->n_whole;
+    const uint32_t whole_off = c.ctl->class_off[kMaxClasses];
+    const uint64_t total = total_split + n_whole;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const bool plain2 = c.ell > 0 && !c.ep.antithetic;
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < total; k += stride) {
+        bool reeval = false;
+        uint32_t m_done = 0;
+        if (k >= total_split) {
+            // ---- a whole outer sample on this lane: no atomics ----
+            const FinalEntry& fe = c.fpool[whole_off + (k - total_split)];
+            const uint32_t m = fe.item;
+            ItemState& it = c.items[m];
+            const FinalShape sh = final_shape(c, it);
+            double phi_f, phi_c, s1_big = 0., sab_big = 0., phi_small = 0.;
+            bool near = false;
+            if (plain2) {
+                const SegAcc a = lane_group<P, MIXED>(A, m, it, TAG_FINAL, 0, it.nf);
+                const SegAcc b = lane_group<P, MIXED>(A, m, it, TAG_FINAL_COARSE, 0, it.nc);
+                acc.work += a.work + b.work;
+                acc.st[4] += a.nu + b.nu;
+                acc.st[5] += a.nn + b.nn;
+                const double mf = it.start + a.s1 / (double)it.nf, mc = it.start + b.s1 / (double)it.nc;
+                near = MIXED && (fabs(mf) <= kMixedBound * (a.sab / (double)it.nf + fabs(it.start)) ||
+                                 fabs(mc) <= kMixedBound * (b.sab / (double)it.nc + fabs(it.start)));
+                outer_phi(c, it, 0., 0., phi_scalar(c.phi_type, mf, 1., 0.), phi_scalar(c.phi_type, mc, 1., 0.),
+                          phi_f, phi_c);
+            } else {
+                const double coeff = (double)(1.f / (float)sh.groups);
+                for (uint32_t g = 0; g < sh.groups; g++) {
+                    const SegAcc a = lane_group<P, MIXED>(A, m, it, TAG_FINAL, (uint64_t)g * sh.n,
+                                                          (uint64_t)(g + 1) * sh.n);
+                    acc.work += a.work;
+                    acc.st[4] += a.nu;
+                    acc.st[5] += a.nn;
+                    const double mean = it.start + a.s1 / (double)sh.n;
+                    if (MIXED) near |= fabs(mean) <= kMixedBound * (a.sab / (double)sh.n + fabs(it.start));
+                    phi_small = phi_scalar(c.phi_type, mean, coeff, phi_small);
+                    s1_big += a.s1;
+                    sab_big += a.sab;
+                }
+                if (MIXED)
+                    near |= fabs(it.start + s1_big / (double)sh.big) <=
+                            kMixedBound * (sab_big / (double)sh.big + fabs(it.start));
+                outer_phi(c, it, s1_big, phi_small, 0., 0., phi_f, phi_c);
+            }
+            if (MIXED && near && c.phi_type == PHI_INDICATOR) {
+                it.phi32_f = phi_f;
+                it.phi32_c = phi_c;
+                reeval = true;
+                m_done = m;
+            } else {
+                finish_outer(c, m, it, phi_f, phi_c, acc);
+            }
+        } else {
+            // ---- one segment of a split final group ----
+            uint64_t kk = k;
+            int cls = 0;
+            uint64_t spe = 1, n_cls = 0;
+            for (; cls < kMaxClasses; cls++) {
+                n_cls = c.base_min << cls;
+                const uint64_t units = (n_cls + P::spu - 1) / P::spu;
+                spe = (units + c.S - 1) / c.S;
+                const uint64_t segs = (uint64_t)c.ctl->class_cnt[cls] * spe;
+                if (kk < segs) break;
+                kk -= segs;
+            }
+            const uint64_t e = kk / spe, j = kk % spe;
+            FinalEntry& fe = c.fpool[c.ctl->class_off[cls] + e];
+            const uint32_t m = fe.item;
+            ItemState& it = c.items[m];
+            const uint64_t i0 = (fe.kind == 0) ? (uint64_t)fe.group * n_cls : 0;
+            const uint64_t i1 = i0 + n_cls;
+            const uint64_t ub = i0 / P::spu, ue = (i1 + P::spu - 1) / P::spu;
+            const uint64_t u0 = ub + j * c.S, u1 = (u0 + c.S < ue) ? u0 + c.S : ue;
+            SegAcc s;
+            P::template units<MIXED>(A, m, it, fe.kind == 2 ? TAG_FINAL_COARSE : TAG_FINAL, u0, u1, i0, i1, s);
+            acc.work += s.work;
+            acc.st[4] += s.nu;
+            acc.st[5] += s.nn;
+            bool complete = spe == 1;
+            double s1 = s.s1, sab = s.sab;
+            if (!complete) {
+                aadd(&fe.s1, s.s1);
+                if (MIXED) aadd(&fe.sab, s.sab);
+                wait_acks();
+                if (acount(&fe.done, 1u) == spe - 1) {
+                    complete = true;
+                    s1 = aread(&fe.s1);
+                    sab = MIXED ? aread(&fe.sab) : 0.;
+                }
+            }
+            if (complete) {
+                const double mean = it.start + s1 / (double)n_cls;
+                const bool near = MIXED && fabs(mean) <= kMixedBound * (sab / (double)n_cls + fabs(it.start));
+                if (near) __hip_atomic_fetch_or(&it.near, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (fe.kind == 0) {
+                    const FinalShape sh = final_shape(c, it);
+                    const double coeff = (double)(1.f / (float)sh.groups);
+                    if (c.ell > 0) aadd(&it.phi_acc, phi_scalar(c.phi_type, mean, coeff, 0.));
+                    aadd(&it.s1_big, s1);
+                    if (MIXED) aadd(&it.sab_big, sab);
+                } else if (fe.kind == 1) {
+                    astore(&it.phi_f, phi_scalar(c.phi_type, mean, 1., 0.));
+                } else {
+                    astore(&it.phi_c, phi_scalar(c.phi_type, mean, 1., 0.));
+                }
+                wait_acks();
+                if (__hip_atomic_fetch_sub(&it.left, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 1u) {
+                    const double s1b = aread(&it.s1_big), pacc = aread(&it.phi_acc);
+                    double phi_f, phi_c;
+                    outer_phi(c, it, s1b, pacc, aread(&it.phi_f), aread(&it.phi_c), phi_f, phi_c);
+                    bool is_near = MIXED && aread_u32(&it.near) != 0;
+                    if (MIXED && fe.kind == 0) {
+                        const FinalShape sh = final_shape(c, it);
+                        const double mb = it.start + s1b / (double)sh.big;
+                        is_near |= fabs(mb) <= kMixedBound * (aread(&it.sab_big) / (double)sh.big + fabs(it.start));
+                    }
+                    if (MIXED && is_near && c.phi_type == PHI_INDICATOR) {
+                        it.phi32_f = phi_f;
+                        it.phi32_c = phi_c;
+                        reeval = true;
+                        m_done = m;
+                    } else {
+                        finish_outer(c, m, it, phi_f, phi_c, acc);
+                    }
+                }
+            }
+        }
+        const uint32_t slot = wave_append(&c.ctl->nreeval, reeval);
+        if (reeval) c.reeval[slot] = m_done;
+    }
+    block_flush(acc, &c.ctl->acc);
+}
+
+// FP64 re-evaluation of the final draws of outer samples whose FP32 mean came
+// within the bound of the Heaviside threshold: one wave per outer sample.
+template <class P>
+__global__ __launch_bounds__(256) void k_b_reeval(typename P::Args A) {
+    const BatchedCommon& c = A.c;
+    LaneAcc acc;
+    acc.zero();
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t n = aload_u32(&c.ctl->nreeval);
+    const uint32_t waves = gridDim.x * (blockDim.x / 64);
+    for (uint32_t w = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6); w < n; w += waves) {
+        const uint32_t m = c.reeval[w];
+        const ItemState& it = c.items[m];
+        const FinalShape sh = final_shape(c, it);
+        double phi_f = 0., phi_c = 0.;
+        auto group_sum64 = [&](uint32_t tag, uint64_t i0, uint64_t i1) {
+            SegAcc s;
+            const uint64_t ub = i0 / P::spu, ue = (i1 + P::spu - 1) / P::spu;
+            for (uint64_t u = ub + lane; u < ue; u += 64) P::template units<false>(A, m, it, tag, u, u + 1, i0, i1, s);
+            return wave_sum(s.s1);
+        };
+        if (c.ell > 0 && !c.ep.antithetic) {
+            phi_f = phi_scalar(c.phi_type, it.start + group_sum64(TAG_FINAL, 0, it.nf) / (double)it.nf, 1., 0.);
+            phi_c = phi_scalar(c.phi_type, it.start + group_sum64(TAG_FINAL_COARSE, 0, it.nc) / (double)it.nc, 1., 0.);
+        } else {
+            const double coeff = (double)(1.f / (float)sh.groups);
+            double s1_big = 0., phi_small = 0.;
+            for (uint32_t g = 0; g < sh.groups; g++) {
+                const double s1 = group_sum64(TAG_FINAL, (uint64_t)g * sh.n, (uint64_t)(g + 1) * sh.n);
+                phi_small = phi_scalar(c.phi_type, it.start + s1 / (double)sh.n, coeff, phi_small);
+                s1_big += s1;
+            }
+            outer_phi(c, it, s1_big, phi_small, 0., 0., phi_f, phi_c);
+        }
+        if (lane == 0) {
+            acc.st[6] += 1;
+            acc.st[7] += (phi_f != it.phi32_f) || (phi_c != it.phi32_c);
+            finish_outer(c, m, it, phi_f, phi_c, acc);
+        }
+    }
+    block_flush(acc, &c.ctl->acc);
+}
+
+// The kernel sequence of one level call on stream s.
+template <class P, bool MIXED>
+hipError_t launch_batched_level(const typename P::Args& A, uint32_t blocks, hipStream_t s) {
+    hipLaunchKernelGGL(k_b_zero, dim3(1), dim3(256), 0, s, A.c.ctl);
+    hipLaunchKernelGGL((k_b_init<P>), dim3(blocks), dim3(256), 0, s, A);
+    for (uint32_t r = 0; r < A.c.rounds; r++)
+        hipLaunchKernelGGL((k_b_test<P, MIXED>), dim3(blocks), dim3(256), 0, s, A, r);
+    hipLaunchKernelGGL((k_b_final_count<P>), dim3(blocks), dim3(256), 0, s, A);
+    hipLaunchKernelGGL(k_b_final_alloc, dim3(1), dim3(64), 0, s, A.c.ctl, A.c.fcap);
+    hipLaunchKernelGGL((k_b_final_fill<P>), dim3(blocks), dim3(256), 0, s, A);
+    hipLaunchKernelGGL((k_b_final<P, MIXED>), dim3(blocks), dim3(256), 0, s, A);
+    if (MIXED) hipLaunchKernelGGL((k_b_reeval<P>), dim3(blocks), dim3(256), 0, s, A);
+    return hipGetLastError();
+}
+
+}  // namespace nmlmc

@@ -6,7 +6,9 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
+#include <functional>
 #include <vector>
 
 #include "../../include/nmlmc.h"
@@ -219,9 +221,7 @@ int Engine::trace_finish(uint64_t M, TraceBufs& tb, double* tr_phi_f, double* tr
 }
 
 int Engine::finish_call(uint32_t ell, sample_output* out) {
-    hipError_t e = hipEventRecord(ev1, stream);
-    if (e != hipSuccess) return hip_fail(e, "hipEventRecord");
-    e = hipMemcpyAsync(h_out, d_out, sizeof(Partial), hipMemcpyDeviceToHost, stream);
+    hipError_t e = hipMemcpyAsync(h_out, d_out, sizeof(Partial), hipMemcpyDeviceToHost, stream);
     if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync result");
     e = hipStreamSynchronize(stream);
     if (e != hipSuccess) return hip_fail(e, "kernel execution");
@@ -265,6 +265,7 @@ int Engine::sample_model_parity(uint32_t ell, uint64_t M, uint32_t lanes_overrid
     if (e != hipSuccess) return hip_fail(e, "hipEventRecord");
     e = launch_model_parity(H, stream);
     if (e != hipSuccess) return hip_fail(e, "launch k_model_parity");
+    if ((e = hipEventRecord(ev1, stream)) != hipSuccess) return hip_fail(e, "hipEventRecord");
     e = launch_reduce_partials(d_partials, (uint32_t)nblocks, d_out, stream);
     if (e != hipSuccess) return hip_fail(e, "launch k_reduce_partials");
     last_kernel = "k_model_parity";
@@ -289,59 +290,164 @@ int Engine::device_cus() {
     return cus;
 }
 
-// Batched forms: persistent grid over a device queue of outer samples.
-int Engine::sample_batched(uint32_t ell, uint64_t M, uint64_t outer_offset, uint32_t lanes_override,
-                           sample_output* out, double* tr_phi_f, double* tr_phi_c, uint64_t* tr_nf,
-                           uint64_t* tr_nc) {
-    if (problem != PROBLEM_MODEL) return sample_portfolio_batched(ell, M, outer_offset, lanes_override, out,
-                                                                   tr_phi_f, tr_phi_c, tr_nf, tr_nc);
-    if (!d_queue) {
-        hipError_t e = hipMalloc(&d_queue, 64);
-        if (e != hipSuccess) return hip_fail(e, "hipMalloc queue");
+// Buffers of the batched engine for M outer samples (grown, never shrunk).
+int Engine::prepare_batched(uint64_t M, BatchedCommon& c) {
+    hipError_t e;
+    if (M > bat_cap) {
+        uint64_t cap = bat_cap ? bat_cap : 1024;
+        while (cap < M) cap *= 2;
+        for (void* p : {(void*)bat_items, (void*)bat_tf[0], (void*)bat_tf[1], (void*)bat_tc[0],
+                        (void*)bat_tc[1], (void*)bat_reeval})
+            if (p) hipFree(p);
+        bat_items = nullptr;
+        bat_tf[0] = bat_tf[1] = bat_tc[0] = bat_tc[1] = nullptr;
+        bat_reeval = nullptr;
+        bat_cap = 0;
+        if ((e = hipMalloc(&bat_items, sizeof(ItemState) * cap)) != hipSuccess) return hip_fail(e, "hipMalloc items");
+        for (int i = 0; i < 2; i++) {
+            if ((e = hipMalloc(&bat_tf[i], sizeof(TestEntry) * cap)) != hipSuccess) return hip_fail(e, "hipMalloc queue");
+            if ((e = hipMalloc(&bat_tc[i], sizeof(TestEntry) * cap)) != hipSuccess) return hip_fail(e, "hipMalloc queue");
+        }
+        if ((e = hipMalloc(&bat_reeval, sizeof(uint32_t) * cap)) != hipSuccess) return hip_fail(e, "hipMalloc reeval");
+        bat_cap = cap;
     }
-    ModelBatchedArgs A;
-    A.mp = mp;
-    A.ep = ep;
-    A.cf = level_counts(ep, ell);
-    A.cc = level_counts(ep, ell == 0 ? 0 : ell - 1);
-    A.seed = seed;
-    A.M = M;
-    A.offset = outer_offset;
-    A.ell = ell;
-    A.queue = (unsigned long long*)d_queue;
-    // Lanes per outer sample: spread the resident lanes over the outer samples.
-    const uint64_t lanes = (uint64_t)device_cus() * 64 * 16;
-    uint32_t G = pow2_floor_u32(lanes / M);
-    if (lanes_override) G = pow2_ceil_u32(lanes_override);
-    if (const char* sG = getenv("NMLMC_GROUP_LANES")) G = pow2_ceil_u32(strtoul(sG, nullptr, 10));
-    last_G = G;
-    uint64_t blocks = (M * G + 255) / 256;
-    const uint64_t max_blocks = (uint64_t)device_cus() * 8;
-    if (blocks > max_blocks) blocks = max_blocks;
-    int rc = ensure_partials(blocks);
-    if (rc) return rc;
-    A.partials = d_partials;
+    const uint64_t want_f = 3 * M + 4096;
+    if (want_f > bat_fcap) {
+        if (bat_fpool) hipFree(bat_fpool);
+        bat_fpool = nullptr;
+        bat_fcap = 0;
+        if ((e = hipMalloc(&bat_fpool, sizeof(FinalEntry) * want_f)) != hipSuccess) return hip_fail(e, "hipMalloc finals");
+        bat_fcap = want_f;
+    }
+    if (!bat_ctl) {
+        if ((e = hipMalloc(&bat_ctl, sizeof(BatchedCtl))) != hipSuccess) return hip_fail(e, "hipMalloc ctl");
+        if ((e = hipHostMalloc(&bat_ctl_h, sizeof(BatchedCtl))) != hipSuccess) return hip_fail(e, "hipHostMalloc ctl");
+    }
+    c.items = bat_items;
+    c.tf[0] = bat_tf[0];
+    c.tf[1] = bat_tf[1];
+    c.tc[0] = bat_tc[0];
+    c.tc[1] = bat_tc[1];
+    c.fpool = bat_fpool;
+    c.fcap = (uint32_t)std::min<uint64_t>(bat_fcap, 0xffffffffull);
+    c.reeval = bat_reeval;
+    c.ctl = bat_ctl;
+    return NMLMC_OK;
+}
+
+// Common arguments of a batched level call; checks that every draw-group size
+// is the smallest one times a power of two (finals are bucketed that way).
+int Engine::batched_common(uint32_t ell, uint64_t M, uint64_t outer_offset, BatchedCommon& c) {
+    std::memset(&c, 0, sizeof(c));
+    c.ep = ep;
+    c.cf = level_counts(ep, ell);
+    c.cc = level_counts(ep, ell == 0 ? 0 : ell - 1);
+    c.seed = seed;
+    c.M = M;
+    c.offset = outer_offset;
+    c.ell = ell;
+    auto rounds_of = [](const LevelCounts& lc) {
+        uint32_t n = 0;
+        for (uint64_t cur = lc.base; lc.max > 2 * cur; cur *= 2) n++;
+        return n;
+    };
+    c.rounds = rounds_of(c.cf);
+    uint64_t bmin = c.cf.base;
+    if (ell > 0) {
+        c.rounds = std::max(c.rounds, rounds_of(c.cc));
+        bmin = std::min(c.cf.base, c.cc.base);
+    }
+    if (c.rounds > (uint32_t)kMaxRounds) return set_error(NMLMC_ERR_ARG, "too many refinement rounds");
+    auto pow2_multiple = [bmin](uint64_t n) {
+        if (n % bmin) return false;
+        const uint64_t q = n / bmin;
+        return (q & (q - 1)) == 0 && q < (1ull << (kMaxClasses - 1));
+    };
+    bool ok = pow2_multiple(c.cf.base) && pow2_multiple(c.cf.max);
+    if (ell > 0) ok = ok && pow2_multiple(c.cc.base) && pow2_multiple(c.cc.max);
+    if (!ok && !(ell > 0 && ep.antithetic))
+        return set_error(NMLMC_ERR_ARG, "batched forms need inner counts that are the smallest count "
+                                        "times a power of two (beta a power of two)");
+    if (!ok) {
+        // Antithetic finals only use the smaller count as the group size.
+        bmin = std::min(c.cf.max, c.cc.max);
+        if (c.cf.base != c.cf.max || c.cc.base != c.cc.max)
+            return set_error(NMLMC_ERR_ARG, "batched forms need power-of-two inner count ratios");
+    }
+    c.base_min = bmin;
+    c.fcap = 0;
+    return prepare_batched(M, c);
+}
+
+int Engine::run_batched(uint32_t ell, uint64_t M, BatchedCommon& c, sample_output* out,
+                        const std::function<hipError_t()>& launch, double* tr_phi_f, double* tr_phi_c,
+                        uint64_t* tr_nf, uint64_t* tr_nc) {
     TraceBufs tb;
-    rc = trace_alloc(M, tr_phi_f || tr_phi_c || tr_nf || tr_nc, tb);
+    int rc = trace_alloc(M, tr_phi_f || tr_phi_c || tr_nf || tr_nc, tb);
     if (rc) return rc;
-    A.tr_phi_f = tb.phi_f;
-    A.tr_phi_c = tb.phi_c;
-    A.tr_nf = tb.nf;
-    A.tr_nc = tb.nc;
-    hipError_t e = hipEventRecord(ev0, stream);
-    if (e != hipSuccess) return hip_fail(e, "hipEventRecord");
-    e = launch_model_batched(A, G, (uint32_t)blocks, form == NMLMC_FORM_MIXED, stream);
-    if (e != hipSuccess) return hip_fail(e, "launch k_model_batched");
-    e = launch_reduce_partials(d_partials, (uint32_t)blocks, d_out, stream);
-    if (e != hipSuccess) return hip_fail(e, "launch k_reduce_partials");
-    last_kernel = "k_model_batched";
-    rc = finish_call(ell, out);
-    if (rc) return rc;
+    c.tr_phi_f = tb.phi_f;
+    c.tr_phi_c = tb.phi_c;
+    c.tr_nf = tb.nf;
+    c.tr_nc = tb.nc;
+    for (int attempt = 0; attempt < 4; attempt++) {
+        hipError_t e = hipEventRecord(ev0, stream);
+        if (e != hipSuccess) return hip_fail(e, "hipEventRecord");
+        e = launch();
+        if (e != hipSuccess) return hip_fail(e, "launch batched level");
+        if ((e = hipEventRecord(ev1, stream)) != hipSuccess) return hip_fail(e, "hipEventRecord");
+        e = hipMemcpyAsync(bat_ctl_h, bat_ctl, sizeof(BatchedCtl), hipMemcpyDeviceToHost, stream);
+        if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync ctl");
+        if ((e = hipStreamSynchronize(stream)) != hipSuccess) return hip_fail(e, "batched kernels");
+        if (!bat_ctl_h->overflow) break;
+        // More final groups than the pool holds: grow it and redo the call (the
+        // draws are keyed, so the redo is identical).
+        const uint64_t need = (uint64_t)bat_ctl_h->final_total + 4096;
+        if (bat_fpool) hipFree(bat_fpool);
+        bat_fpool = nullptr;
+        bat_fcap = 0;
+        if ((e = hipMalloc(&bat_fpool, sizeof(FinalEntry) * need)) != hipSuccess) return hip_fail(e, "hipMalloc finals");
+        bat_fcap = need;
+        c.fpool = bat_fpool;
+        c.fcap = (uint32_t)std::min<uint64_t>(need, 0xffffffffull);
+    }
+    if (bat_ctl_h->overflow) return set_error(NMLMC_ERR_DEVICE, "final-group pool overflow");
+    float ms = 0;
+    hipEventElapsedTime(&ms, ev0, ev1);
+    last_ms = ms;
+    std::memcpy(h_out, &bat_ctl_h->acc, sizeof(Partial));
+    copy_out(ell, out);
     mixed_reeval = h_out->stats[6];
     mixed_flips = h_out->stats[7];
     rc = trace_finish(M, tb, tr_phi_f, tr_phi_c, tr_nf, tr_nc);
     if (rc) return rc;
     return set_error(NMLMC_OK, "");
+}
+
+uint32_t Engine::batched_blocks() { return (uint32_t)device_cus() * 8; }
+
+// Batched forms: per-outer-sample keys, segment queues, one launch per round.
+int Engine::sample_batched(uint32_t ell, uint64_t M, uint64_t outer_offset, uint32_t lanes_override,
+                           sample_output* out, double* tr_phi_f, double* tr_phi_c, uint64_t* tr_nf,
+                           uint64_t* tr_nc) {
+    if (problem != PROBLEM_MODEL) return sample_portfolio_batched(ell, M, outer_offset, lanes_override, out,
+                                                                   tr_phi_f, tr_phi_c, tr_nf, tr_nc);
+    ModelBatchedHost H;
+    int rc = batched_common(ell, M, outer_offset, H.c);
+    if (rc) return rc;
+    H.mp = mp;
+    H.c.phi_type = mp.phi_type;
+    H.c.nps = mp.variant == MODEL_REF ? 2 : 1;
+    H.c.ups = 0;
+    // Segment: 16 Box-Muller pairs per lane (32 samples of the shift model).
+    H.c.S = lanes_override ? lanes_override : 16;
+    if (const char* sS = getenv("NMLMC_SEGMENT")) H.c.S = (uint32_t)strtoul(sS, nullptr, 10);
+    if (H.c.S == 0) H.c.S = 16;
+    last_G = H.c.S;
+    H.blocks = batched_blocks();
+    H.mixed = form == NMLMC_FORM_MIXED;
+    last_kernel = "k_b_final";
+    return run_batched(ell, M, H.c, out, [&]() { return launch_model_batched(H, stream); }, tr_phi_f, tr_phi_c,
+                       tr_nf, tr_nc);
 }
 
 }  // namespace nmlmc

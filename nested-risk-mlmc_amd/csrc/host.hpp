@@ -5,9 +5,11 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <functional>
 #include <vector>
 
 #include "../../include/nmlmc.h"
+#include "batched.hpp"
 #include "engine.hpp"
 #include "model.hpp"
 #include "portfolio.hpp"
@@ -51,7 +53,16 @@ struct Engine {
     const char* last_kernel = "";
     uint32_t last_G = 0;
     std::vector<uint64_t> next_outer;   // batched keying: next outer id per level
-    uint64_t* d_queue = nullptr;        // batched: work-queue counters
+    uint64_t* d_queue = nullptr;
+    // Batched engine buffers.
+    ItemState* bat_items = nullptr;
+    TestEntry* bat_tf[2] = {nullptr, nullptr};
+    TestEntry* bat_tc[2] = {nullptr, nullptr};
+    FinalEntry* bat_fpool = nullptr;
+    uint32_t* bat_reeval = nullptr;
+    BatchedCtl* bat_ctl = nullptr;
+    BatchedCtl* bat_ctl_h = nullptr;
+    uint64_t bat_cap = 0, bat_fcap = 0;
     uint64_t mixed_reeval = 0, mixed_flips = 0;
 
     int ensure_device();
@@ -76,6 +87,12 @@ struct Engine {
                                 uint64_t* tr_nf, uint64_t* tr_nc);
     int cus = 0;
     int device_cus();
+    uint32_t batched_blocks();
+    int prepare_batched(uint64_t M, BatchedCommon& c);
+    int batched_common(uint32_t ell, uint64_t M, uint64_t outer_offset, BatchedCommon& c);
+    int run_batched(uint32_t ell, uint64_t M, BatchedCommon& c, sample_output* out,
+                    const std::function<hipError_t()>& launch, double* tr_phi_f, double* tr_phi_c,
+                    uint64_t* tr_nf, uint64_t* tr_nc);
     int sample_portfolio_batched(uint32_t ell, uint64_t M, uint64_t outer_offset,
                                  uint32_t lanes_override, sample_output* out, double* tr_phi_f,
                                  double* tr_phi_c, uint64_t* tr_nf, uint64_t* tr_nc);

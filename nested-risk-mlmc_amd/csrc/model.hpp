@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "batched.hpp"
 #include "engine.hpp"
 
 namespace nmlmc {
@@ -44,24 +45,13 @@ struct ModelParityArgsHost {
     uint64_t* tr_nc;
 };
 
-struct ModelBatchedArgs {
+struct ModelBatchedHost {
+    BatchedCommon c;
     ModelParams mp;
-    EngineParams ep;
-    LevelCounts cf, cc;
-    uint64_t seed;
-    uint64_t M;
-    uint64_t offset;          // first outer-sample id of this call
-    uint32_t ell;
-    unsigned long long* queue;  // next outer sample to hand out
-    Partial* partials;        // one per block
-    double* tr_phi_f;
-    double* tr_phi_c;
-    uint64_t* tr_nf;
-    uint64_t* tr_nc;
+    uint32_t blocks;
+    bool mixed;
 };
-
-hipError_t launch_model_batched(const ModelBatchedArgs& A, uint32_t G, uint32_t blocks, bool mixed,
-                                hipStream_t s);
+hipError_t launch_model_batched(const ModelBatchedHost& H, hipStream_t s);
 
 // Blocks of 256 threads covering T groups of G lanes; one Partial per block.
 inline uint64_t parity_blocks(uint64_t T, uint32_t G) { return (T * G + 255) / 256; }

@@ -19,8 +19,23 @@ struct u64x2 {
     uint64_t a, b;
 };
 
-__device__ __forceinline__ uint64_t rotl64(uint64_t x, unsigned n) {
-    return (x << n) | (x >> (64 - n));
+// 64-bit rotate by a compile-time amount as two v_alignbit_b32 funnel shifts
+// (one per 32-bit half); a rotate by 32 is a register swap.
+template <unsigned N>
+__device__ __forceinline__ uint64_t rotl64(uint64_t x) {
+    const uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
+    uint32_t rlo, rhi;
+    if constexpr (N == 32) {
+        rlo = hi;
+        rhi = lo;
+    } else if constexpr (N < 32) {
+        rhi = __builtin_amdgcn_alignbit(hi, lo, 32 - N);
+        rlo = __builtin_amdgcn_alignbit(lo, hi, 32 - N);
+    } else {
+        rhi = __builtin_amdgcn_alignbit(lo, hi, 64 - N);
+        rlo = __builtin_amdgcn_alignbit(hi, lo, 64 - N);
+    }
+    return ((uint64_t)rhi << 32) | rlo;
 }
 
 // Threefry2x64 with 20 rounds (Random123/threefry.h:190-293): Skein rotation
@@ -31,7 +46,7 @@ __device__ __forceinline__ u64x2 threefry2x64_20(uint64_t c0, uint64_t c1, uint6
     uint64_t x0 = c0 + k0, x1 = c1 + k1;
 #define NMLMC_TF_R(R)     \
     x0 += x1;             \
-    x1 = rotl64(x1, R);   \
+    x1 = rotl64<R>(x1);   \
     x1 ^= x0;
     NMLMC_TF_R(16) NMLMC_TF_R(42) NMLMC_TF_R(12) NMLMC_TF_R(31)
     x0 += k1; x1 += k2 + 1;

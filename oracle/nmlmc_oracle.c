@@ -22,6 +22,7 @@
 #define _GNU_SOURCE
 #include "nmlmc_oracle.h"
 #include <math.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -457,6 +458,10 @@ struct orc_ctx {
     orc_portfolio pf;
     orc_sampler* samplers;
     real_t* workspace;
+    orc_sampler* workers; /* batched keying: one sampler per OpenMP worker */
+    real_t* worker_ws;
+    int nworkers;
+    uint32_t ws_reals;
 };
 
 static inline real_t s_randn(orc_sampler* s, orc_rs* st) {
@@ -951,9 +956,59 @@ static void process_outer(orc_ctx* c, draw_env* e, uint32_t ell, int fine_only,
     }
 }
 
+static void orc_ensure_workers(orc_ctx* c);
+
+/* Per-worker accumulators of kernel_sample (nested_sampler.hpp:379-382). */
+typedef struct {
+    sum_t f[MAX_PHI * MAX_MOM], d[MAX_PHI * MAX_MOM];
+    work_t tw;
+    uint64_t st[NSTATS];
+} orc_acc;
+
+/* Outer samples [s0, e0) on sampler s: the m-loop of kernel_sample. */
+static void run_range(orc_ctx* c, orc_sampler* s, int keying, uint32_t ell, uint64_t s0, uint64_t e0,
+                      uint64_t outer_offset, const samples_count_t* sc_f, const samples_count_t* sc_c,
+                      orc_acc* a, double* tr_phi_f, double* tr_phi_c, uint64_t* tr_nf, uint64_t* tr_nc) {
+    const int fine_only = (ell == 0);
+    real_t f_phi[MAX_PHI], c_phi[MAX_PHI];
+    s->randn_count = s->rand_count = 0;
+    for (uint64_t m = s0; m < e0; m++) {
+        draw_env e;
+        e.keying = keying;
+        e.s = s;
+        e.spu = (c->problem == ORC_MODEL_SHIFT) ? 2 : 1;
+        orc_rs outer;
+        if (keying == ORC_KEY_BATCHED) {
+            e.key[0] = c->seed;
+            e.key[1] = ((uint64_t)ell << 46) | (outer_offset + m);
+            rs_seed(&outer, e.key[0], e.key[1]);
+            s->sync = &outer;
+            s->inner = &outer;
+        }
+        for (uint32_t i = 0; i < c->phi_size; i++) c_phi[i] = 0;
+        uint64_t nf, nc;
+        process_outer(c, &e, ell, fine_only, sc_f, sc_c, a->f, a->d, &a->tw, a->st, f_phi, c_phi, &nf,
+                      &nc);
+        if (tr_phi_f) tr_phi_f[m] = f_phi[0];
+        if (tr_phi_c) tr_phi_c[m] = fine_only ? 0 : c_phi[0];
+        if (tr_nf) tr_nf[m] = nf;
+        if (tr_nc) tr_nc[m] = nc;
+        if (keying == ORC_KEY_BATCHED) {
+            s->sync = &s->own_sync;
+            s->inner = &s->own_inner;
+        }
+    }
+    a->st[4] += s->rand_count;
+    a->st[5] += s->randn_count;
+}
+
+#define ORC_BATCH_CHUNK 256
+
 /* kernel_sample (369-485) + sample (646-694).  Parity keying restates the
  * OpenMP build: thread t takes the libgomp static-schedule block of [0, M)
- * and consumes its own persistent streams in order. */
+ * and consumes its own persistent streams in order.  The virtual threads (or,
+ * for the batched keying, fixed chunks of outer samples) run on OpenMP
+ * workers; their accumulators are added in index order. */
 void orc_sample(orc_ctx* c, int keying, uint32_t ell, uint64_t M, uint64_t outer_offset,
                 double* fsums, double* diffsums, double* work, uint64_t* stats,
                 double* tr_phi_f, double* tr_phi_c, uint64_t* tr_nf, uint64_t* tr_nc) {
@@ -968,13 +1023,13 @@ void orc_sample(orc_ctx* c, int keying, uint32_t ell, uint64_t M, uint64_t outer
     sc_init(c, ell, &sc_f);
     sc_init(c, fine_only ? 0 : ell - 1, &sc_c);
 
-    const uint32_t T = (keying == ORC_KEY_PARITY) ? c->cores : 1;
-    for (uint32_t t = 0; t < T; t++) {
-        sum_t f_sums[MAX_PHI * MAX_MOM] = {0}, d_sums[MAX_PHI * MAX_MOM] = {0};
-        uint64_t st[NSTATS] = {0};
-        work_t tw = 0.;
-        real_t f_phi[MAX_PHI], c_phi[MAX_PHI];
+    const uint64_t T = (keying == ORC_KEY_PARITY) ? c->cores : (M + ORC_BATCH_CHUNK - 1) / ORC_BATCH_CHUNK;
+    orc_acc* acc = (orc_acc*)calloc(T ? T : 1, sizeof(orc_acc));
+    if (keying == ORC_KEY_BATCHED) orc_ensure_workers(c);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (uint64_t t = 0; t < T; t++) {
         uint64_t s0, e0;
+        orc_sampler* s;
         if (keying == ORC_KEY_PARITY) {
             uint64_t q = M / T, rr = M % T;
             if (t < rr) {
@@ -983,46 +1038,23 @@ void orc_sample(orc_ctx* c, int keying, uint32_t ell, uint64_t M, uint64_t outer
             }
             s0 = q * t + rr;
             e0 = s0 + q;
+            s = &c->samplers[t];
         } else {
-            s0 = 0;
-            e0 = M;
+            s0 = t * ORC_BATCH_CHUNK;
+            e0 = s0 + ORC_BATCH_CHUNK < M ? s0 + ORC_BATCH_CHUNK : M;
+            s = &c->workers[omp_get_thread_num()];
         }
-        orc_sampler* s = &c->samplers[t];
-        s->randn_count = s->rand_count = 0;
-        for (uint64_t m = s0; m < e0; m++) {
-            draw_env e;
-            e.keying = keying;
-            e.s = s;
-            e.spu = (c->problem == ORC_MODEL_SHIFT) ? 2 : 1;
-            orc_rs outer;
-            if (keying == ORC_KEY_BATCHED) {
-                e.key[0] = c->seed;
-                e.key[1] = ((uint64_t)ell << 46) | (outer_offset + m);
-                rs_seed(&outer, e.key[0], e.key[1]);
-                s->sync = &outer;
-                s->inner = &outer;
-            }
-            for (uint32_t i = 0; i < c->phi_size; i++) c_phi[i] = 0;
-            uint64_t nf, nc;
-            process_outer(c, &e, ell, fine_only, &sc_f, &sc_c, f_sums, d_sums, &tw, st, f_phi, c_phi,
-                          &nf, &nc);
-            if (tr_phi_f) tr_phi_f[m] = f_phi[0];
-            if (tr_phi_c) tr_phi_c[m] = fine_only ? 0 : c_phi[0];
-            if (tr_nf) tr_nf[m] = nf;
-            if (tr_nc) tr_nc[m] = nc;
-            if (keying == ORC_KEY_BATCHED) {
-                s->sync = &s->own_sync;
-                s->inner = &s->own_inner;
-            }
-        }
-        st[4] = s->rand_count;
-        st[5] = s->randn_count;
-        for (uint32_t i = 0; i < nf_sums; i++) fsums[i] += f_sums[i];
-        if (!fine_only)
-            for (uint32_t i = 0; i < nd_sums; i++) diffsums[i] += d_sums[i];
-        *work += tw;
-        for (int i = 0; i < NSTATS; i++) stats[i] += st[i];
+        run_range(c, s, keying, ell, s0, e0, outer_offset, &sc_f, &sc_c, &acc[t], tr_phi_f, tr_phi_c,
+                  tr_nf, tr_nc);
     }
+    for (uint64_t t = 0; t < T; t++) {
+        for (uint32_t i = 0; i < nf_sums; i++) fsums[i] += acc[t].f[i];
+        if (!fine_only)
+            for (uint32_t i = 0; i < nd_sums; i++) diffsums[i] += acc[t].d[i];
+        *work += acc[t].tw;
+        for (int i = 0; i < NSTATS; i++) stats[i] += acc[t].st[i];
+    }
+    free(acc);
 }
 
 /* ---------------------------------------------------------------------------
@@ -1056,11 +1088,40 @@ void orc_free(orc_ctx* c) {
     free_portfolio(&c->pf);
     free(c->samplers);
     free(c->workspace);
+    free(c->workers);
+    free(c->worker_ws);
     free(c);
+}
+
+/* Per-worker samplers of the batched keying (their streams are re-keyed per
+ * outer sample; only the portfolio workspace matters). */
+static void orc_ensure_workers(orc_ctx* c) {
+    int n = omp_get_max_threads();
+    if (c->workers && c->nworkers >= n) return;
+    free(c->workers);
+    free(c->worker_ws);
+    c->workers = (orc_sampler*)calloc(n, sizeof(orc_sampler));
+    c->worker_ws = c->ws_reals ? (real_t*)calloc((size_t)c->ws_reals * n, sizeof(real_t)) : NULL;
+    for (int i = 0; i < n; i++) {
+        orc_sampler* s = &c->workers[i];
+        s->sync = &s->own_sync;
+        s->inner = &s->own_inner;
+        if (c->ws_reals) {
+            s->W_tau = c->worker_ws + (size_t)i * c->ws_reals;
+            s->R_tau = s->W_tau + c->ws_reals / 2;
+        }
+    }
+    c->nworkers = n;
 }
 
 static void seed_samplers(orc_ctx* c, uint64_t seed, uint32_t ws_reals) {
     c->seed = seed;
+    c->ws_reals = ws_reals;
+    free(c->workers);
+    free(c->worker_ws);
+    c->workers = NULL;
+    c->worker_ws = NULL;
+    c->nworkers = 0;
     free(c->workspace);
     c->workspace = ws_reals ? (real_t*)calloc((size_t)ws_reals * c->cores, sizeof(real_t)) : NULL;
     for (uint32_t t = 0; t < c->cores; t++) {

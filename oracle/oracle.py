@@ -257,7 +257,14 @@ class Ref(_Common):
                                            C.c_double, C.c_uint32, _dp, _dp, _dp, _dp, _dp,
                                            _u32p, _u64p, C.c_int, _dp]
         lib.refp_mlmc_fake_run.restype = C.c_int
+        lib.refp_layout.argtypes = [_u64p]
+        lib.refp_layout.restype = C.c_int
         self.real = real
+
+    def layout(self):
+        o = np.zeros(64, np.uint64)
+        n = self.lib.refp_layout(_ptr(o, _u64p))
+        return [int(x) for x in o[:n]]
 
     def init(self, cores, N0, beta, adaptive_r=0.0, antithetic=True, moments=4):
         self.lib.refp_set_threads(cores)

@@ -5,6 +5,7 @@
 // stream consumption, sub-sampling searches, Black-Scholes helpers, the MLMC
 // driver's call sequence).  TEST INFRASTRUCTURE ONLY: never linked into, or
 // called by, the product library.
+#include <cstddef>
 #include <cstdint>
 #include <cstring>
 #include <vector>
@@ -24,6 +25,54 @@
 extern "C" {
 
 int refp_real_bytes() { return REFP_REAL_BYTES; }
+
+// sizeof / offsetof of the structs that cross the reference's C ABI.
+int refp_layout(uint64_t* o) {
+    int n = 0;
+    o[n++] = sizeof(nested_sampler_params);
+    o[n++] = offsetof(nested_sampler_params, beta);
+    o[n++] = offsetof(nested_sampler_params, antithetic);
+    o[n++] = offsetof(nested_sampler_params, fine_moments);
+    o[n++] = offsetof(nested_sampler_params, diff_moments);
+    o[n++] = offsetof(nested_sampler_params, adaptive_r);
+    o[n++] = sizeof(sample_output);
+    o[n++] = offsetof(sample_output, fsums);
+    o[n++] = offsetof(sample_output, diffsums);
+    o[n++] = offsetof(sample_output, total_work);
+    o[n++] = offsetof(sample_output, inner_stats);
+    o[n++] = sizeof(model::data);
+    o[n++] = offsetof(model::data, s1tau);
+    o[n++] = offsetof(model::data, max_loss);
+    o[n++] = offsetof(model::data, phi_type);
+    o[n++] = sizeof(portfolio::option_params);
+    o[n++] = offsetof(portfolio::option_params, maturity);
+    o[n++] = offsetof(portfolio::option_params, start);
+    o[n++] = offsetof(portfolio::option_params, w);
+    o[n++] = offsetof(portfolio::option_params, delta_est);
+    o[n++] = offsetof(portfolio::option_params, stock_ind);
+    o[n++] = offsetof(portfolio::option_params, type);
+    o[n++] = offsetof(portfolio::option_params, prob);
+    o[n++] = offsetof(portfolio::option_params, W_est);
+    o[n++] = sizeof(portfolio::stock_params);
+    o[n++] = sizeof(portfolio::data);
+    o[n++] = offsetof(portfolio::data, stocks);
+    o[n++] = offsetof(portfolio::data, r);
+    o[n++] = offsetof(portfolio::data, rho);
+    o[n++] = offsetof(portfolio::data, s1_rho2);
+    o[n++] = offsetof(portfolio::data, max_loss);
+    o[n++] = offsetof(portfolio::data, tau);
+    o[n++] = offsetof(portfolio::data, dt_h0inv);
+    o[n++] = offsetof(portfolio::data, dt_beta);
+    o[n++] = offsetof(portfolio::data, dt_prob_expo);
+    o[n++] = offsetof(portfolio::data, phi_type);
+    o[n++] = offsetof(portfolio::data, _phi_size);
+    o[n++] = offsetof(portfolio::data, subsample);
+    o[n++] = offsetof(portfolio::data, use_delta_cv);
+    o[n++] = offsetof(portfolio::data, use_coarse_delta_cv);
+    o[n++] = offsetof(portfolio::data, use_antithetic_risk);
+    o[n++] = offsetof(portfolio::data, use_milstein);
+    return n;
+}
 
 void refp_set_threads(int n) { omp_set_num_threads(n); }
 

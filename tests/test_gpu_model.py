@@ -98,12 +98,13 @@ def test_plain_shift_model_capped_rule(nm, oracle_f64):
     nm.cleanup_nestedmc_sampler()
 
 
-@pytest.mark.parametrize("lanes", [0, 1, 8, 64])
-def test_batched_fp64_matches_oracle(nm, oracle_f64, lanes, monkeypatch):
+@pytest.mark.parametrize("segment", [0, 1, 3, 16, 64])
+def test_batched_fp64_matches_oracle(nm, oracle_f64, segment, monkeypatch):
     """Batched FP64 form: per-outer-sample keys; identical per-sample results to
-    the oracle's batched keying for any lane grouping."""
-    if lanes:
-        monkeypatch.setenv("NMLMC_GROUP_LANES", str(lanes))
+    the oracle's batched keying for any segment size (Box-Muller pairs per lane
+    per work item), i.e. however the draw groups are split across lanes."""
+    if segment:
+        monkeypatch.setenv("NMLMC_SEGMENT", str(segment))
     orc = _shift_setup(nm, oracle_f64, 1, nm.FORM_BATCHED)
     for ell, M, off in ((0, 500, 0), (1, 300, 0), (3, 100, 0), (1, 200, 300)):
         g = nm.sample(ell, M, outer_offset=off, trace=True)
