@@ -205,4 +205,9 @@ struct BatchedCommon {
 };
 
 
+// Defined once (reduce.hip): clears the control block; turns the final-group
+// class counts into offsets and flags an over-full pool.
+__global__ void k_b_zero(BatchedCtl* ctl);
+__global__ void k_b_final_alloc(BatchedCtl* ctl, uint32_t fcap);
+
 }  // namespace nmlmc

@@ -26,12 +26,6 @@ __device__ __forceinline__ bool refine(const BatchedCommon& c, const LevelCounts
     return !(new_count <= (double)cur);
 }
 
-__global__ void k_b_zero(BatchedCtl* ctl) {
-    uint32_t* p = (uint32_t*)ctl;
-    const uint32_t n = sizeof(BatchedCtl) / 4;
-    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
-}
-
 // Outer draws.  Round 0 tests every outer sample, so its queue is implicit
 // (entry e is outer sample e); its accumulators are only written when a draw
 // group spans several segments.
@@ -116,6 +110,7 @@ __global__ __launch_bounds__(256) void k_b_test(typename P::Args A, uint32_t r) 
         }
         bool append = false;
         if (complete) {
+            acc.work += P::group_work(A);
             const LevelCounts& lc = fine ? c.cf : c.cc;
             uint64_t n = cur;
             bool more = refine(c, lc, it.start, cur, s1, s2);
@@ -215,20 +210,6 @@ __global__ __launch_bounds__(256) void k_b_final_count(typename P::Args A) {
         if (hist[i]) atomicAdd(&c.ctl->class_cnt[i], hist[i]);
     if (threadIdx.x == 0 && hist[kMaxClasses]) atomicAdd(&c.ctl->n_whole, hist[kMaxClasses]);
     block_flush(acc, &c.ctl->acc);
-}
-
-__global__ void k_b_final_alloc(BatchedCtl* ctl, uint32_t fcap) {
-    if (threadIdx.x != 0) return;
-    uint32_t off = 0;
-    for (int i = 0; i < kMaxClasses; i++) {
-        ctl->class_off[i] = off;
-        ctl->class_cur[i] = 0;
-        off += ctl->class_cnt[i];
-    }
-    ctl->class_off[kMaxClasses] = off;
-    ctl->final_total = off + ctl->n_whole;
-    ctl->overflow = ctl->final_total > fcap;
-    ctl->n_whole = 0;  // refilled as a cursor by k_b_final_fill
 }
 
 // Writes each outer sample's final entries into its size class's region; block
@@ -371,7 +352,7 @@ __global__ __launch_bounds__(256) void k_b_final(typename P::Args A) {
             if (plain2) {
                 const SegAcc a = lane_group<P, MIXED>(A, m, it, TAG_FINAL, 0, it.nf);
                 const SegAcc b = lane_group<P, MIXED>(A, m, it, TAG_FINAL_COARSE, 0, it.nc);
-                acc.work += a.work + b.work;
+                acc.work += a.work + b.work + 2 * P::group_work(A);
                 acc.st[4] += a.nu + b.nu;
                 acc.st[5] += a.nn + b.nn;
                 const double mf = it.start + a.s1 / (double)it.nf, mc = it.start + b.s1 / (double)it.nc;
@@ -384,7 +365,7 @@ __global__ __launch_bounds__(256) void k_b_final(typename P::Args A) {
                 for (uint32_t g = 0; g < sh.groups; g++) {
                     const SegAcc a = lane_group<P, MIXED>(A, m, it, TAG_FINAL, (uint64_t)g * sh.n,
                                                           (uint64_t)(g + 1) * sh.n);
-                    acc.work += a.work;
+                    acc.work += a.work + P::group_work(A);
                     acc.st[4] += a.nu;
                     acc.st[5] += a.nn;
                     const double mean = it.start + a.s1 / (double)sh.n;
@@ -445,6 +426,7 @@ __global__ __launch_bounds__(256) void k_b_final(typename P::Args A) {
                 }
             }
             if (complete) {
+                acc.work += P::group_work(A);
                 const double mean = it.start + s1 / (double)n_cls;
                 const bool near = MIXED && fabs(mean) <= kMixedBound * (sab / (double)n_cls + fabs(it.start));
                 if (near) __hip_atomic_fetch_or(&it.near, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

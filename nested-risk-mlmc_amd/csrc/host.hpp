@@ -38,7 +38,10 @@ struct Engine {
     // Portfolio tables (device) and host mirrors.
     PortfolioDevice pf{};
     uint32_t pf_phi_size = 1;
+    double pf_bs_work = 0.;        // work of one init_inner_sampler call
     std::vector<void*> pf_allocs;
+    double* pf_R = nullptr;        // batched: outer states [M x d]
+    size_t pf_R_cap = 0;
 
     StreamState* d_states = nullptr;  // [2 cores]: parity streams (sync, inner)
     Partial* d_partials = nullptr;    // [cores] or [batched groups]
@@ -66,6 +69,7 @@ struct Engine {
     uint64_t mixed_reeval = 0, mixed_flips = 0;
 
     int ensure_device();
+    int stage_portfolio(const portfolio_data* data);
     int seed_samplers(uint64_t s);
     void free_samplers();
     void free_portfolio();

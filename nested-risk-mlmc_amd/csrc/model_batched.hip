@@ -3,8 +3,8 @@
 // Keying: outer sample g of level l owns the key {seed, (l << 46) | g}.  Its
 // outer draw is the first normal of the stream at counter {0, 0}; inner sample
 // i of draw group `tag` takes its Box-Muller pair from counter
-// {i / spu, tag << 40} (spu = inner samples per pair: 2 for the shift model,
-// 1 for the reference model).  Tags: fine test round a -> 1 + a, coarse test
+// {0, (tag << 40) | (i / spu)} (spu = inner samples per pair: 2 for the shift
+// model, 1 for the reference model).  Tags: fine test round a -> 1 + a, coarse test
 // round a -> 33 + a, final draws -> 65 (66 for the coarse final of the plain
 // fine/coarse estimator).
 #include "batched_kernels.hpp"
@@ -19,6 +19,9 @@ struct ModelPolicy {
         ModelParams mp;
     };
     static constexpr uint64_t spu = (VAR == MODEL_SHIFT) ? 2 : 1;
+
+    // init_inner_sampler does no work for the model (model_sampler.cpp:64).
+    __device__ static double group_work(const Args&) { return 0.; }
 
     __device__ static uint64_t key1(const Args& A, uint64_t m) {
         return ((uint64_t)A.c.ell << 46) | (A.c.offset + m);
@@ -47,7 +50,7 @@ struct ModelPolicy {
         if (!F32) {
             double a1 = 0., a2 = 0.;
             for (uint64_t u = u0; u < u1; u++) {
-                const u64x2 w = threefry2x64_20(u, (uint64_t)tag << 40, A.c.seed, k1);
+                const u64x2 w = threefry2x64_20(0, ((uint64_t)tag << 40) | u, A.c.seed, k1);
                 double x, y;
                 boxmuller_f64(w.a, w.b, x, y);
                 if (VAR == MODEL_REF) {
@@ -78,7 +81,7 @@ struct ModelPolicy {
             const float sa = (float)A.mp.stau, sb = (float)A.mp.s1tau;
             Kahan b1, b2, bab;
             for (uint64_t u = u0; u < u1; u++) {
-                const u64x2 w = threefry2x64_20(u, (uint64_t)tag << 40, A.c.seed, k1);
+                const u64x2 w = threefry2x64_20(0, ((uint64_t)tag << 40) | u, A.c.seed, k1);
                 float x, y;
                 boxmuller_f32(w.a, w.b, x, y);
                 if (VAR == MODEL_REF) {

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "batched.hpp"
 #include "engine.hpp"
 
 namespace nmlmc {
@@ -42,5 +43,31 @@ struct PortfolioDevice {
     uint32_t options_count;
     double prob_norm, total_std;
 };
+
+struct PortfolioParityHost {
+    PortfolioDevice pf;
+    EngineParams ep;
+    LevelCounts cf, cc;
+    uint64_t seed, M;
+    uint32_t T, ell;
+    double bs_work;
+    StreamState* states;
+    Partial* partials;
+    double* tr_phi_f;
+    double* tr_phi_c;
+    uint64_t* tr_nf;
+    uint64_t* tr_nc;
+};
+hipError_t launch_portfolio_parity(const PortfolioParityHost& H, hipStream_t s);
+
+struct PortfolioBatchedHost {
+    BatchedCommon c;
+    PortfolioDevice pf;
+    double* R_tau;
+    double bs_work;
+    uint32_t blocks;
+    bool mixed;
+};
+hipError_t launch_portfolio_batched(const PortfolioBatchedHost& H, hipStream_t s);
 
 }  // namespace nmlmc

@@ -1,5 +1,5 @@
 // Portfolio entry points (portfolio_sampler.cpp:618-756): host staging of the
-// option tables and the level call.
+// option tables and the level calls of the plain and batched forms.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -11,20 +11,220 @@
 
 namespace nmlmc {
 
+namespace detail {
+
+// Host restatement of calc_option_blackscholes / calc_delta_blackscholes
+// (portfolio_sampler.cpp:140-160) with real_t = double, plus the
+// cash-or-nothing digitals of the extension.
+const double kSqrt2H = 1.414213562373095;
+const double kSqrt2PiH = 2.506628274631;
+double norm_cdf_h(double x) { return 0.5 * std::erfc(-x / kSqrt2H); }
+double norm_pdf_h(double x) { return std::exp(-0.5 * (x * x)) / kSqrt2PiH; }
+
+double bs_price_h(int type, double S0, double strike, double r, double t, double sigma) {
+    const double v1 = std::log(S0 / strike);
+    const double v2 = (r + (sigma * sigma) / 2.0) * t;
+    const double v3 = sigma * std::sqrt(t);
+    const double d1 = (v1 + v2) / v3;
+    const double d2 = d1 - sigma * std::sqrt(t);
+    if (type == 2 || type == -2) return std::exp(-r * t) * norm_cdf_h((double)(type / 2) * d2);
+    const double mod = (double)type;
+    return mod * S0 * norm_cdf_h(mod * d1) - mod * strike * std::exp(-r * t) * norm_cdf_h(mod * d2);
+}
+
+double bs_delta_h(int type, double S0, double strike, double r, double t, double sigma) {
+    const double v1 = std::log(S0 / strike);
+    const double v2 = (r + (sigma * sigma) / 2.0) * t;
+    const double v3 = sigma * std::sqrt(t);
+    const double d1 = (v1 + v2) / v3;
+    if (type == 2 || type == -2) {
+        const double d2 = d1 - sigma * std::sqrt(t);
+        return (double)(type / 2) * std::exp(-r * t) * norm_pdf_h(d2) / (S0 * sigma * std::sqrt(t));
+    }
+    return norm_cdf_h(d1) - (double)(type == -1);
+}
+
+template <class T>
+int upload(Engine& e, const T* src, size_t n, const T** dst) {
+    void* p = nullptr;
+    hipError_t err = hipMalloc(&p, sizeof(T) * (n ? n : 1));
+    if (err != hipSuccess) return hip_fail(err, "hipMalloc portfolio table");
+    e.pf_allocs.push_back(p);
+    if (n) {
+        err = hipMemcpy(p, src, sizeof(T) * n, hipMemcpyHostToDevice);
+        if (err != hipSuccess) return hip_fail(err, "hipMemcpy portfolio table");
+    }
+    *dst = (const T*)p;
+    return NMLMC_OK;
+}
+
+}  // namespace detail
+
+using detail::upload;
+
 void Engine::free_portfolio() {
     for (void* p : pf_allocs) hipFree(p);
     pf_allocs.clear();
     std::memset(&pf, 0, sizeof(pf));
 }
 
-int Engine::sample_portfolio_parity(uint32_t, uint64_t, uint32_t, sample_output*, double*, double*,
-                                    uint64_t*, uint64_t*) {
-    return set_error(NMLMC_ERR_STATE, "portfolio plain form not available");
+// portfolio_update_data (portfolio_sampler.cpp:631-678).
+int Engine::stage_portfolio(const portfolio_data* data) {
+    if (!inited) return set_error(NMLMC_ERR_STATE, "init_nestedmc_sampler has not been called");
+    if (!data) return set_error(NMLMC_ERR_ARG, "portfolio data is NULL");
+    const uint32_t d = (uint32_t)data->stocks.size;
+    if (d == 0 || !data->stocks.data) return set_error(NMLMC_ERR_ARG, "portfolio has no stocks");
+    if (data->subsample < -1 || data->subsample > 1) return set_error(NMLMC_ERR_ARG, "unknown sub-sampling mode");
+    if (data->phi_type < 0 || data->phi_type > 3) return set_error(NMLMC_ERR_ARG, "unknown phi type");
+    const uint32_t ps = (data->phi_type == 0 || data->phi_type == 1) ? 1u : data->_phi_size;
+    if (ps == 0 || ps > (uint32_t)kMaxPhi) return set_error(NMLMC_ERR_ARG, "phi size must be 1..4");
+    if (ensure_device()) return nmlmc_last_error();
+    free_portfolio();
+    PortfolioDevice P;
+    std::memset(&P, 0, sizeof(P));
+    const stock_params* stocks = (const stock_params*)data->stocks.data;
+    std::vector<double> delta(d, 0.);
+    P.prob_norm = 0;
+    P.total_std = 0;
+    P.options_count = 0;
+    for (int m = 0; m < 3; m++) {
+        const option_params* ops = (const option_params*)data->options[m].data;
+        const size_t n = data->options[m].size;
+        if (n && !ops) return set_error(NMLMC_ERR_ARG, "options[%d] is NULL", m);
+        std::vector<double> cdf(n);
+        double c = 0;
+        double prev = 1.7976931348623157e308;
+        P.compute_prob[m] = 0;
+        for (size_t j = 0; j < n; j++) {
+            const option_params& o = ops[j];
+            if (!(o.prob > 0 && prev >= o.prob))
+                return set_error(NMLMC_ERR_ARG, "options[%d][%zu]: probabilities must be positive and "
+                                                "sorted in descending order", m, j);
+            if (o.stock_ind >= d) return set_error(NMLMC_ERR_ARG, "options[%d][%zu]: stock index out of range", m, j);
+            if (o.type != 1 && o.type != -1 && o.type != 2 && o.type != -2)
+                return set_error(NMLMC_ERR_ARG, "options[%d][%zu]: unknown option type", m, j);
+            cdf[j] = c;
+            prev = o.prob;
+            c += o.prob;
+            delta[o.stock_ind] += o.w * o.delta_est;
+            P.prob_norm += o.prob;
+            P.total_std += o.prob * o.W_est;
+            P.compute_prob[m] += o.prob;
+        }
+        static_assert(sizeof(option_params) == sizeof(OptionRec), "option record layout");
+        int rc = upload(*this, (const OptionRec*)ops, n, &P.options[m]);
+        if (rc) return rc;
+        if ((rc = upload(*this, cdf.data(), n, &P.cdfs[m]))) return rc;
+        P.nopt[m] = (uint32_t)n;
+        P.options_count += (uint32_t)n;
+    }
+    if (P.options_count == 0) return set_error(NMLMC_ERR_ARG, "portfolio has no options");
+    static_assert(sizeof(stock_params) == sizeof(StockRec), "stock record layout");
+    int rc = upload(*this, (const StockRec*)stocks, d, &P.stocks);
+    if (rc) return rc;
+    if ((rc = upload(*this, delta.data(), d, &P.delta_coalesce))) return rc;
+    P.chol = nullptr;
+    P.d = d;
+    P.r = data->r;
+    P.rho = data->rho;
+    P.s1_rho2 = data->s1_rho2;
+    P.max_loss = data->max_loss;
+    P.tau = data->tau;
+    P.stau = std::sqrt(data->tau);
+    P.dt_h0inv = data->dt_h0inv;
+    P.dt_beta = data->dt_beta;
+    P.dt_prob_expo = data->dt_prob_expo;
+    P.phi_type = data->phi_type;
+    P.phi_size = ps;
+    P.subsample = data->subsample;
+    P.use_delta_cv = data->use_delta_cv;
+    P.use_coarse_delta_cv = data->use_coarse_delta_cv;
+    P.use_antithetic_risk = data->use_antithetic_risk;
+    P.use_milstein = data->use_milstein;
+    if (data->options[2].size && (data->dt_beta < 2 || !(data->dt_prob_expo < 0) || data->dt_h0inv == 0))
+        return set_error(NMLMC_ERR_ARG, "Unbiased options need dt_beta >= 2, dt_prob_expo < 0, dt_h0inv > 0");
+    // Work of one init_inner_sampler call without sub-sampling: the BS options.
+    pf_bs_work = 0.;
+    if (data->subsample == 0) {
+        const option_params* bs = (const option_params*)data->options[0].data;
+        for (size_t j = 0; j < data->options[0].size; j++) pf_bs_work += bs[j].W_est;
+    }
+    pf = P;
+    pf_phi_size = ps;
+    problem = PROBLEM_PORTFOLIO;
+    return set_error(NMLMC_OK, "");
 }
 
-int Engine::sample_portfolio_batched(uint32_t, uint64_t, uint64_t, uint32_t, sample_output*, double*,
-                                     double*, uint64_t*, uint64_t*) {
-    return set_error(NMLMC_ERR_STATE, "portfolio batched form not available");
+int Engine::sample_portfolio_parity(uint32_t ell, uint64_t M, uint32_t, sample_output* out,
+                                    double* tr_phi_f, double* tr_phi_c, uint64_t* tr_nf, uint64_t* tr_nc) {
+    PortfolioParityHost H;
+    H.pf = pf;
+    H.ep = ep;
+    H.cf = level_counts(ep, ell);
+    H.cc = level_counts(ep, ell == 0 ? 0 : ell - 1);
+    H.seed = seed;
+    H.M = M;
+    H.T = cores;
+    H.ell = ell;
+    H.bs_work = pf_bs_work;
+    int rc = ensure_partials(cores);
+    if (rc) return rc;
+    H.states = d_states;
+    H.partials = d_partials;
+    TraceBufs tb;
+    rc = trace_alloc(M, tr_phi_f || tr_phi_c || tr_nf || tr_nc, tb);
+    if (rc) return rc;
+    H.tr_phi_f = tb.phi_f;
+    H.tr_phi_c = tb.phi_c;
+    H.tr_nf = tb.nf;
+    H.tr_nc = tb.nc;
+    last_G = 64;
+    hipError_t e = hipEventRecord(ev0, stream);
+    if (e != hipSuccess) return hip_fail(e, "hipEventRecord");
+    if ((e = launch_portfolio_parity(H, stream)) != hipSuccess) return hip_fail(e, "launch k_portfolio_parity");
+    if ((e = hipEventRecord(ev1, stream)) != hipSuccess) return hip_fail(e, "hipEventRecord");
+    if ((e = launch_reduce_partials(d_partials, cores, d_out, stream)) != hipSuccess)
+        return hip_fail(e, "launch k_reduce_partials");
+    last_kernel = "k_portfolio_parity";
+    rc = finish_call(ell, out);
+    if (rc) return rc;
+    rc = trace_finish(M, tb, tr_phi_f, tr_phi_c, tr_nf, tr_nc);
+    if (rc) return rc;
+    return set_error(NMLMC_OK, "");
+}
+
+int Engine::sample_portfolio_batched(uint32_t ell, uint64_t M, uint64_t outer_offset, uint32_t lanes_override,
+                                     sample_output* out, double* tr_phi_f, double* tr_phi_c, uint64_t* tr_nf,
+                                     uint64_t* tr_nc) {
+    if (pf.phi_type != PHI_INDICATOR && pf.phi_type != PHI_MAX)
+        return set_error(NMLMC_ERR_ARG, "batched forms support the Indicator and Max phi types");
+    PortfolioBatchedHost H;
+    int rc = batched_common(ell, M, outer_offset, H.c);
+    if (rc) return rc;
+    H.pf = pf;
+    H.c.phi_type = pf.phi_type;
+    H.c.S = lanes_override ? lanes_override : 32;
+    if (const char* sS = getenv("NMLMC_SEGMENT")) H.c.S = (uint32_t)strtoul(sS, nullptr, 10);
+    if (H.c.S == 0) H.c.S = 32;
+    last_G = H.c.S;
+    const size_t need = (size_t)M * pf.d;
+    if (need > pf_R_cap) {
+        if (pf_R) hipFree(pf_R);
+        pf_R = nullptr;
+        pf_R_cap = 0;
+        size_t cap = pf_R_cap ? pf_R_cap : 1024;
+        while (cap < need) cap *= 2;
+        hipError_t e = hipMalloc(&pf_R, sizeof(double) * cap);
+        if (e != hipSuccess) return hip_fail(e, "hipMalloc outer states");
+        pf_R_cap = cap;
+    }
+    H.R_tau = pf_R;
+    H.bs_work = pf_bs_work;
+    H.blocks = batched_blocks();
+    H.mixed = form == NMLMC_FORM_MIXED;
+    last_kernel = "k_b_final";
+    return run_batched(ell, M, H.c, out, [&]() { return launch_portfolio_batched(H, stream); }, tr_phi_f,
+                       tr_phi_c, tr_nf, tr_nc);
 }
 
 }  // namespace nmlmc
@@ -34,20 +234,66 @@ using nmlmc::set_error;
 
 extern "C" {
 
+// portfolio_cleanup (portfolio_sampler.cpp:618-629).
 void portfolio_cleanup(void) {
     g_engine.free_portfolio();
     if (g_engine.problem == nmlmc::PROBLEM_PORTFOLIO) g_engine.problem = -1;
     set_error(NMLMC_OK, "");
 }
 
-void portfolio_update_data(const portfolio_data*) { set_error(NMLMC_ERR_STATE, "portfolio not available"); }
-void portfolio_init(const portfolio_data*, uint64_t) { set_error(NMLMC_ERR_STATE, "portfolio not available"); }
-void portfolio_comp_params(portfolio_data*) { set_error(NMLMC_ERR_STATE, "portfolio not available"); }
-void portfolio_sample(uint32_t, uint64_t, uint64_t, sample_output*) {
-    set_error(NMLMC_ERR_STATE, "portfolio not available");
+// portfolio_update_data (portfolio_sampler.cpp:631-678).
+void portfolio_update_data(const portfolio_data* data) { g_engine.stage_portfolio(data); }
+
+// portfolio_init (portfolio_sampler.cpp:680-686).
+void portfolio_init(const portfolio_data* data, uint64_t seed) {
+    if (g_engine.stage_portfolio(data)) return;
+    g_engine.seed_samplers(seed);
 }
-int nmlmc_portfolio_set_correlation(const double*, uint32_t) {
-    return set_error(NMLMC_ERR_STATE, "portfolio not available");
+
+// portfolio_comp_params (portfolio_sampler.cpp:687-700).  A cash-or-nothing
+// digital priced by simulation (Exact / Unbiased) has a zero pathwise delta,
+// so its delta estimate is 0: the control variate's constant then matches the
+// zero correction its inner samples add.
+void portfolio_comp_params(portfolio_data* data) {
+    if (!data) {
+        set_error(NMLMC_ERR_ARG, "portfolio data is NULL");
+        return;
+    }
+    const stock_params* stocks = (const stock_params*)data->stocks.data;
+    for (int m = 0; m < 3; m++) {
+        option_params* ops = (option_params*)data->options[m].data;
+        for (size_t j = 0; j < data->options[m].size; j++) {
+            option_params& o = ops[j];
+            const stock_params& st = stocks[o.stock_ind];
+            o.start = nmlmc::detail::bs_price_h(o.type, st.S0, o.strike, data->r, o.maturity, st.sigma);
+            o.delta_est = nmlmc::detail::bs_delta_h(o.type, st.S0, o.strike, data->r, o.maturity, st.sigma);
+            if (m > 0 && (o.type == 2 || o.type == -2)) o.delta_est = 0.;
+        }
+    }
+    set_error(NMLMC_OK, "");
+}
+
+// portfolio_sample (portfolio_sampler.cpp:749-756).
+void portfolio_sample(uint32_t ell, uint64_t override_dyn_inner, uint64_t M, sample_output* out) {
+    if (g_engine.problem != nmlmc::PROBLEM_PORTFOLIO) {
+        set_error(NMLMC_ERR_STATE, "portfolio_sample: no portfolio staged");
+        return;
+    }
+    g_engine.sample(ell, M, UINT64_MAX, (uint32_t)override_dyn_inner, out, nullptr, nullptr, nullptr, nullptr);
+}
+
+int nmlmc_portfolio_set_correlation(const double* chol, uint32_t d) {
+    if (g_engine.problem != nmlmc::PROBLEM_PORTFOLIO) return set_error(NMLMC_ERR_STATE, "no portfolio staged");
+    if (!chol) {
+        g_engine.pf.chol = nullptr;
+        return set_error(NMLMC_OK, "");
+    }
+    if (d != g_engine.pf.d) return set_error(NMLMC_ERR_ARG, "correlation factor must be %u x %u", g_engine.pf.d, g_engine.pf.d);
+    const double* p = nullptr;
+    int rc = nmlmc::detail::upload(g_engine, chol, (size_t)d * d, &p);
+    if (rc) return rc;
+    g_engine.pf.chol = p;
+    return set_error(NMLMC_OK, "");
 }
 
 }  // extern "C"

@@ -1,0 +1,313 @@
+// Plain FP64 form of the portfolio problem with the reference's keying.
+//
+// kernel_sample<false, portfolio::sampler> (nested_sampler.hpp:369-485) for the
+// OpenMP semantics: T virtual threads, thread t owning a sync and an inner
+// stream keyed {seed, t} that persist across calls, its libgomp static share
+// of [0, M) walked in order.  One wavefront per virtual thread:
+//   - the outer draw (portfolio_sampler.cpp:293-311) takes its normals from
+//     known positions of the sync stream, one stock per lane, into LDS;
+//   - an inner loop with sub-sampling and no Unbiased options draws, for
+//     sample k, the method uniform from sync word P_s + k and then 1 (BS) or 3
+//     (Exact: option uniform + one Box-Muller pair) inner words; lanes take
+//     chunks of 64 samples and find their inner word offsets by a ballot
+//     prefix count, so every draw matches the sequential reference;
+//   - other configurations (no sub-sampling, Unbiased options, a cached inner
+//     normal) run the reference loop on lane 0.
+#include "portfolio.hpp"
+#include "portfolio_dev.hpp"
+
+namespace nmlmc {
+
+struct PortfolioParityArgs {
+    PortfolioDevice pf;
+    EngineParams ep;
+    LevelCounts cf, cc;
+    uint64_t seed, M;
+    uint32_t T, ell;
+    double bs_work;
+    StreamState* states;
+    Partial* partials;  // one per virtual thread (block)
+    double* tr_phi_f;
+    double* tr_phi_c;
+    uint64_t* tr_nf;
+    uint64_t* tr_nc;
+};
+
+struct InnerSums {
+    double s1, s2, work;
+    uint64_t nu, nn;
+};
+
+__device__ __forceinline__ double phi_value(int phi_type, double u, double coeff, double phi) {
+    return phi_scalar(phi_type, u, coeff, phi);
+}
+
+// inner_sampler::sample_inner (nested_sampler.hpp:131-161) over `count` samples
+// from the thread's streams (ss: sync, si: inner), wave-uniform result.
+__device__ InnerSums parity_inner(const PortfolioParityArgs& A, const double* R, uint64_t k1, StreamState& ss,
+                                  StreamState& si, uint64_t count) {
+    const PortfolioDevice& pf = A.pf;
+    const uint32_t lane = threadIdx.x;
+    InnerSums r{0., 0., 0., 0, 0};
+    const bool fast = pf.subsample != 0 && !(pf.compute_prob[2] > 0) && !si.is_saved;
+    if (fast) {
+        double s1 = 0., s2 = 0., work = 0.;
+        uint64_t nn = 0;
+        uint64_t base = 0;
+        const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+        for (uint64_t c0 = 0; c0 < count; c0 += 64) {
+            const uint64_t k = c0 + lane;
+            const bool valid = k < count;
+            uint32_t method = 0;
+            if (valid) method = pick_method(pf, u01_f64(stream_word(A.seed, k1, ss.P + k)) * pf.prob_norm);
+            const uint64_t exact = __ballot(valid && method == 1);
+            const uint64_t nvalid = count - c0 < 64 ? count - c0 : 64;
+            const uint64_t q = si.P + base + (valid ? lane + 2 * __popcll(exact & below) : 0);
+            if (valid) {
+                const double u = u01_f64(stream_word(A.seed, k1, q));
+                uint32_t idx;
+                double prob;
+                if (pf.subsample == -1) {
+                    idx = (uint32_t)(ceil(u * (double)pf.nopt[method]) - 1);
+                    prob = 1. / pf.options_count;
+                } else {
+                    idx = pick_cdf(pf.cdfs[method], pf.nopt[method], u * pf.compute_prob[method]);
+                    prob = pf.options[method][idx].prob / pf.prob_norm;
+                }
+                const OptionRec o = pf.options[method][idx];
+                double v;
+                if (method == 1) {
+                    double x, y;
+                    stream_bm_f64(A.seed, k1, q + 1, x, y);
+                    v = inner_exact_d(pf, o, R, x, y, work);
+                    nn += 2;
+                } else {
+                    v = inner_bs_d(pf, o, R, work);
+                }
+                v = v / prob;
+                s1 += v;
+                s2 += v * v;
+            }
+            base += nvalid + 2 * (uint64_t)__popcll(exact);
+        }
+        r.s1 = wave_sum(s1);
+        r.s2 = wave_sum(s2);
+        r.work = wave_sum(work);
+        r.nn = wave_sum_u64(nn);
+        r.nu = 2 * count;
+        ss.P += count;
+        si.P += base;
+        return r;
+    }
+    // Reference loop on lane 0.
+    if (lane == 0) {
+        DStream sy, in;
+        sy.init(A.seed, k1, 0, ss.P, ss.saved, ss.is_saved);
+        in.init(A.seed, k1, 0, si.P, si.saved, si.is_saved);
+        double s1 = 0., s2 = 0., work = 0.;
+        for (uint64_t i = 0; i < count; i++) {
+            const double d = inner_sample_d(pf, R, sy, in, work);
+            s1 += d;
+            s2 += d * d;
+        }
+        r = InnerSums{s1, s2, work, sy.nu + in.nu, sy.nn + in.nn};
+        ss.P = sy.P;
+        ss.saved = sy.saved;
+        ss.is_saved = sy.is_saved;
+        si.P = in.P;
+        si.saved = in.saved;
+        si.is_saved = in.is_saved;
+    }
+    r.s1 = __shfl(r.s1, 0, 64);
+    r.s2 = __shfl(r.s2, 0, 64);
+    r.work = __shfl(r.work, 0, 64);
+    r.nu = __shfl(r.nu, 0, 64);
+    r.nn = __shfl(r.nn, 0, 64);
+    ss.P = __shfl(ss.P, 0, 64);
+    ss.saved = __shfl(ss.saved, 0, 64);
+    ss.is_saved = __shfl(ss.is_saved, 0, 64);
+    si.P = __shfl(si.P, 0, 64);
+    si.saved = __shfl(si.saved, 0, 64);
+    si.is_saved = __shfl(si.is_saved, 0, 64);
+    return r;
+}
+
+__global__ __launch_bounds__(64) void k_portfolio_parity(PortfolioParityArgs A) {
+    extern __shared__ double smem[];  // R_tau[d], z[d]
+    const PortfolioDevice& pf = A.pf;
+    const EngineParams& ep = A.ep;
+    const uint32_t t = blockIdx.x, lane = threadIdx.x;
+    const uint64_t k1 = t;
+    const uint32_t d = pf.d;
+    double* R = smem;
+    double* z = smem + d;
+
+    uint64_t q = A.M / A.T, rr = A.M % A.T;
+    if (t < rr) {
+        q++;
+        rr = 0;
+    }
+    const uint64_t s0 = q * t + rr, e0 = s0 + q;
+    StreamState ss = A.states[2 * t], si = A.states[2 * t + 1];
+    double f_sums[kMaxPhi * kMaxMoments] = {}, d_sums[kMaxPhi * kMaxMoments] = {};
+    uint64_t stats[kStatsExt] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double work = 0.;
+    const bool fine_only = A.ell == 0;
+    const uint32_t ps = pf.phi_size;
+
+    for (uint64_t m = s0; m < e0; m++) {
+        // ---- portfolio::sampler::sample_outer ----
+        uint64_t used;
+        if (pf.chol) {
+            for (uint32_t j = lane; j < d; j += 64) z[j] = normal_at_f64(A.seed, k1, ss, j);
+            __syncthreads();
+            for (uint32_t i = lane; i < d; i += 64) {
+                double a = 0.;
+                for (uint32_t j = 0; j <= i; j++) a += pf.chol[(size_t)i * d + j] * z[j];
+                const StockRec st = pf.stocks[i];
+                R[i] = simulate_stock_d(st.S0, st.mu, st.sigma, pf.tau, pf.stau * a);
+            }
+            used = d;
+        } else {
+            const double dW_sys = pf.stau * normal_at_f64(A.seed, k1, ss, 0);
+            if (d == 1) {
+                if (lane == 0) R[0] = simulate_stock_d(pf.stocks[0].S0, pf.stocks[0].mu, pf.stocks[0].sigma, pf.tau, dW_sys);
+                used = 1;
+            } else {
+                for (uint32_t i = lane; i < d; i += 64) {
+                    const double dW_ind = pf.stau * normal_at_f64(A.seed, k1, ss, 1 + i);
+                    const double W = pf.rho * dW_sys + pf.s1_rho2 * dW_ind;
+                    const StockRec st = pf.stocks[i];
+                    R[i] = simulate_stock_d(st.S0, st.mu, st.sigma, pf.tau, W);
+                }
+                used = 1 + d;
+            }
+        }
+        __syncthreads();
+        ss = advance_normals_f64(A.seed, k1, ss, used);
+        stats[5] += used;
+        // constant = -max_loss - compute_delta_cv() (sequential sum, as the reference).
+        double cv = 0.;
+        if (pf.use_delta_cv)
+            for (uint32_t i = 0; i < d; i++) cv += (R[i] - pf.stocks[i].S0) * pf.delta_coalesce[i];
+        const double constant = -pf.max_loss - cv;
+        // init_inner_sampler: the pre-summed BS options without sub-sampling.
+        double w_det = 0.;
+        const double start = (pf.subsample == 0) ? det_bs_sum(pf, R, w_det) + constant : constant;
+
+        auto test_round = [&](uint64_t cur) {
+            const InnerSums s = parity_inner(A, R, k1, ss, si, cur);
+            work += s.work + A.bs_work;
+            stats[4] += s.nu;
+            stats[5] += s.nn;
+            const double out_var = s.s2 / (double)(cur - 1) - (s.s1 * s.s1) / (double)(cur * (cur - 1));
+            const double dist = start + s.s1 / (double)cur;
+            return dist * dist / out_var;
+        };
+        auto count_of = [&](const LevelCounts& lc, bool with_stats) {
+            uint64_t cur = lc.base;
+            while (true) {
+                if (lc.max <= 2 * cur) {
+                    cur = lc.max;
+                    break;
+                }
+                stats[0] += cur;
+                const double delta2 = test_round(cur);
+                const double nc = (lc.rule == 1) ? lc.tmp / delta2 : pow(delta2, -ep.adaptive_r / 2) * lc.tmp;
+                if (nc <= (double)cur) break;
+                cur *= 2;
+            }
+            if (with_stats) {
+                stats[1] += 1;
+                stats[2] += (cur == lc.max);
+                stats[3] += (cur == lc.base);
+            }
+            return cur;
+        };
+        auto mean_of = [&](uint64_t n) {
+            const InnerSums s = parity_inner(A, R, k1, ss, si, n);
+            work += s.work + A.bs_work;
+            stats[4] += s.nu;
+            stats[5] += s.nn;
+            return s.s1;
+        };
+
+        const uint64_t nf = count_of(A.cf, true);
+        uint64_t nc = 0;
+        double f_phi[kMaxPhi] = {}, c_phi[kMaxPhi] = {};
+        if (fine_only) {
+            stats[0] += nf;
+            phi_add(pf, start + mean_of(nf) / (double)nf, 1., f_phi);
+        } else {
+            nc = count_of(A.cc, false);
+            if (ep.antithetic) {
+                const bool f_big = nf > nc;
+                const uint64_t big = f_big ? nf : nc, small = f_big ? nc : nf;
+                const uint64_t groups = big / small;
+                const double coeff = (double)(1.f / (float)groups);
+                double s1_big = 0.;
+                double* out_small = f_big ? c_phi : f_phi;
+                double* out_big = f_big ? f_phi : c_phi;
+                for (uint64_t g = 0; g < groups; g++) {
+                    const double s1 = mean_of(small);
+                    phi_add(pf, start + s1 / (double)small, coeff, out_small);
+                    s1_big += s1;
+                }
+                phi_add(pf, start + s1_big / (double)big, 1., out_big);
+                stats[0] += big;
+            } else {
+                stats[0] += nf + nc;
+                phi_add(pf, start + mean_of(nf) / (double)nf, 1., f_phi);
+                phi_add(pf, start + mean_of(nc) / (double)nc, 1., c_phi);
+            }
+        }
+        add_moments(f_phi, ps, f_sums, ep.fine_moments);
+        if (!fine_only) {
+            double dphi[kMaxPhi];
+            for (uint32_t i = 0; i < ps; i++) dphi[i] = f_phi[i] - c_phi[i];
+            add_moments(dphi, ps, d_sums, ep.diff_moments);
+        }
+        if (lane == 0) {
+            if (A.tr_phi_f) A.tr_phi_f[m] = f_phi[0];
+            if (A.tr_phi_c) A.tr_phi_c[m] = fine_only ? 0. : c_phi[0];
+            if (A.tr_nf) A.tr_nf[m] = nf;
+            if (A.tr_nc) A.tr_nc[m] = nc;
+        }
+        __syncthreads();
+    }
+    if (lane == 0) {
+        A.states[2 * t] = ss;
+        A.states[2 * t + 1] = si;
+        Partial& p = A.partials[t];
+        for (int i = 0; i < kMaxPhi * kMaxMoments; i++) {
+            p.f[i] = f_sums[i];
+            p.d[i] = d_sums[i];
+        }
+        p.work = work;
+        for (int i = 0; i < kStatsExt; i++) p.stats[i] = stats[i];
+    }
+}
+
+hipError_t launch_portfolio_parity(const PortfolioParityHost& H, hipStream_t s) {
+    PortfolioParityArgs A;
+    A.pf = H.pf;
+    A.ep = H.ep;
+    A.cf = H.cf;
+    A.cc = H.cc;
+    A.seed = H.seed;
+    A.M = H.M;
+    A.T = H.T;
+    A.ell = H.ell;
+    A.bs_work = H.bs_work;
+    A.states = H.states;
+    A.partials = H.partials;
+    A.tr_phi_f = H.tr_phi_f;
+    A.tr_phi_c = H.tr_phi_c;
+    A.tr_nf = H.tr_nf;
+    A.tr_nc = H.tr_nc;
+    const size_t lds = sizeof(double) * 2 * (size_t)H.pf.d;
+    hipLaunchKernelGGL(k_portfolio_parity, dim3(H.T), dim3(64), lds, s, A);
+    return hipGetLastError();
+}
+
+}  // namespace nmlmc

@@ -1,6 +1,7 @@
 // Fixed-order reduction of per-block Partial records into one record: one
 // 256-thread block per field, strided partial sums then an LDS tree, so the
 // result does not depend on the order in which blocks finished.
+#include "batched.hpp"
 #include "engine.hpp"
 
 namespace nmlmc {
@@ -47,6 +48,26 @@ __global__ __launch_bounds__(256) void k_reduce_partials(const Partial* __restri
 hipError_t launch_reduce_partials(const Partial* P, uint32_t n, Partial* out, hipStream_t s) {
     hipLaunchKernelGGL(k_reduce_partials, dim3(kPartialFields), dim3(256), 0, s, P, n, out);
     return hipGetLastError();
+}
+
+__global__ void k_b_zero(BatchedCtl* ctl) {
+    uint32_t* p = (uint32_t*)ctl;
+    const uint32_t n = sizeof(BatchedCtl) / 4;
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
+}
+
+__global__ void k_b_final_alloc(BatchedCtl* ctl, uint32_t fcap) {
+    if (threadIdx.x != 0) return;
+    uint32_t off = 0;
+    for (int i = 0; i < kMaxClasses; i++) {
+        ctl->class_off[i] = off;
+        ctl->class_cur[i] = 0;
+        off += ctl->class_cnt[i];
+    }
+    ctl->class_off[kMaxClasses] = off;
+    ctl->final_total = off + ctl->n_whole;
+    ctl->overflow = ctl->final_total > fcap;
+    ctl->n_whole = 0;  // refilled as a cursor by k_b_final_fill
 }
 
 }  // namespace nmlmc

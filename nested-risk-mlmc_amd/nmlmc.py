@@ -287,3 +287,96 @@ def mlmc_run_cb(fn, targetTOL, M0, theta=0.5, Ca=3.0, starting_level_coeff=1.5, 
     return _run_outputs(lambda lv, n, s: lib.nmlmc_mlmc_run_cb(
         cb, None, targetTOL, M0, theta, Ca, starting_level_coeff, int(verbose), max_levels, lv,
         C.byref(n), s), max_levels)
+
+
+# ---- portfolio ----------------------------------------------------------------
+
+class Portfolio:
+    """Owns the C arrays behind a portfolio_data (portfolio_sampler.cpp:62-92).
+
+    Options are given flat, method-major (BS, Exact, Unbiased) with ``counts``
+    per method, each method's options sorted by descending ``prob``."""
+
+    SCALARS = dict(r=0.05, rho=0.2, max_loss=0.0, tau=0.02, dt_h0inv=8, dt_beta=4,
+                   dt_prob_expo=-1.5, phi_type=0, phi_size=1, subsample=1, use_delta_cv=True,
+                   use_coarse_delta_cv=False, use_antithetic_risk=True, use_milstein=True)
+
+    def __init__(self, S0, mu, sigma, counts, strike, maturity, w, stock_ind, type, prob,
+                 W_est=None, start=None, delta_est=None, **scalars):
+        P = int(np.sum(counts))
+        self.counts = [int(c) for c in counts]
+        self.stocks = (StockParams * len(S0))()
+        for i in range(len(S0)):
+            self.stocks[i] = StockParams(float(S0[i]), float(mu[i]), float(sigma[i]))
+        self.ops = (OptionParams * max(P, 1))()
+        W_est = np.ones(P) if W_est is None else W_est
+        start = np.zeros(P) if start is None else start
+        delta_est = np.zeros(P) if delta_est is None else delta_est
+        for j in range(P):
+            self.ops[j] = OptionParams(float(strike[j]), float(maturity[j]), float(start[j]),
+                                       float(w[j]), float(delta_est[j]), int(stock_ind[j]),
+                                       int(type[j]), float(prob[j]), float(W_est[j]))
+        sc = dict(self.SCALARS)
+        sc.update(scalars)
+        self.scalars = sc
+        self.c = PortfolioData()
+        base = C.addressof(self.ops)
+        off = 0
+        for m in range(3):
+            self.c.options[m] = Span(base + off * C.sizeof(OptionParams), self.counts[m])
+            off += self.counts[m]
+        self.c.stocks = Span(C.addressof(self.stocks), len(S0))
+        self.c.r, self.c.rho = sc["r"], sc["rho"]
+        self.c.s1_rho2 = float(np.sqrt(1 - sc["rho"] ** 2))
+        self.c.max_loss, self.c.tau = sc["max_loss"], sc["tau"]
+        self.c.dt_h0inv, self.c.dt_beta, self.c.dt_prob_expo = (sc["dt_h0inv"], sc["dt_beta"],
+                                                                 sc["dt_prob_expo"])
+        self.c.phi_type, self.c._phi_size, self.c.subsample = (sc["phi_type"], sc["phi_size"],
+                                                               sc["subsample"])
+        self.c.use_delta_cv = bool(sc["use_delta_cv"])
+        self.c.use_coarse_delta_cv = bool(sc["use_coarse_delta_cv"])
+        self.c.use_antithetic_risk = bool(sc["use_antithetic_risk"])
+        self.c.use_milstein = bool(sc["use_milstein"])
+
+    @property
+    def phi_size(self):
+        return 1 if self.scalars["phi_type"] in (0, 1) else self.scalars["phi_size"]
+
+    def column(self, name):
+        return np.array([getattr(self.ops[j], name) for j in range(sum(self.counts))])
+
+    def comp_params(self):
+        """portfolio_comp_params: start = BS value, delta_est = BS delta."""
+        load().portfolio_comp_params(C.byref(self.c))
+        check()
+        return self.column("start"), self.column("delta_est")
+
+
+def portfolio_init(pf, seed):
+    load().portfolio_init(C.byref(pf.c), seed)
+    check()
+
+
+def portfolio_update_data(pf):
+    load().portfolio_update_data(C.byref(pf.c))
+    check()
+
+
+def portfolio_sample(ell, M, override_dyn_inner=0, moments=4, phi_size=1):
+    o = Output(moments, phi_size)
+    load().portfolio_sample(ell, override_dyn_inner, M, C.byref(o.c))
+    check()
+    return o.as_dict()
+
+
+def portfolio_set_correlation(chol):
+    lib = load()
+    if chol is None:
+        check(lib.nmlmc_portfolio_set_correlation(None, 0))
+        return
+    a = np.ascontiguousarray(chol, np.float64)
+    check(lib.nmlmc_portfolio_set_correlation(a.ctypes.data_as(C.POINTER(C.c_double)), a.shape[0]))
+
+
+def portfolio_cleanup():
+    load().portfolio_cleanup()

@@ -750,9 +750,10 @@ typedef struct {
 
 /* inner_sampler::sample_inner (131-161) over samples [first, first+count) of
  * one draw group.  Parity keying: the thread's persistent streams in order.
- * Batched keying: inner sample i draws from the keyed unit stream
- * ctr = {i / spu, tag << 40}, key = the outer sample's key, both the
- * sync-role and inner-role draws of that sample coming from that stream. */
+ * Batched keying: inner sample i draws from the keyed unit stream whose
+ * blocks are ctr = {j, (tag << 40) | (i / spu)}, j = 0, 1, ..., key = the outer
+ * sample's key, both the sync-role and inner-role draws of that sample coming
+ * from that stream. */
 static void inner_loop(orc_ctx* c, draw_env* e, uint32_t tag, uint64_t first, uint64_t count,
                        sum_t* out_s1, sum_t* out_s2, work_t* out_tw) {
     sum_t s1 = 0, s2 = 0;
@@ -772,8 +773,8 @@ static void inner_loop(orc_ctx* c, draw_env* e, uint32_t tag, uint64_t first, ui
             uint64_t i = first + ii;
             if (ii == 0 || i % e->spu == 0) {
                 rs_seed(&unit, e->key[0], e->key[1]);
-                unit.ctr[0] = i / e->spu;
-                unit.ctr[1] = (uint64_t)tag << 40;
+                unit.ctr[0] = 0;
+                unit.ctr[1] = ((uint64_t)tag << 40) | (i / e->spu);
                 if (i % e->spu) (void)rs_randn(&unit); /* skip the pair's first normal */
             }
             real_t d = sampler_inner(c, s, &tw);

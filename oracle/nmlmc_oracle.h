@@ -61,7 +61,9 @@ void orc_portfolio_init(orc_ctx* c, uint32_t Q, const double* S0, const double* 
 
 /* One level call.  keying = ORC_KEY_PARITY (reference OpenMP semantics with
  * `cores` virtual threads, persistent per-thread streams) or ORC_KEY_BATCHED
- * (per-outer-sample keys {seed, (ell<<46) | (outer_offset+m)}).
+ * (per-outer-sample keys {seed, (ell<<46) | (outer_offset+m)}; outer draws from
+ * counters {j, 0}; inner unit u of draw group `tag` from counters
+ * {j, (tag << 40) | u}).
  * Optional per-outer-sample trace (indexed by m): phi_f[m], phi_c[m], nf[m], nc[m]. */
 void orc_sample(orc_ctx* c, int keying, uint32_t ell, uint64_t M, uint64_t outer_offset,
                 double* fsums, double* diffsums, double* work, uint64_t* stats,

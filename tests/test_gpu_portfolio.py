@@ -1,0 +1,154 @@
+"""GPU parity of the portfolio problem: the HIP library against the oracle on
+the same portfolios and seeds.
+
+Plain FP64 form (reference keying): per-sample phi_f, phi_c, N_f, N_c and the
+level sums over consecutive calls, on the reference's own generated portfolio
+(BS, Exact and Unbiased/Milstein options) and on synthetic ones (digitals,
+Cholesky correlation), for every sub-sampling mode and phi type.  Batched FP64
+form: the same against the oracle's batched keying.  Mixed form: indicators
+equal to FP64 wherever the refinement decisions agree.
+
+Tolerances: counts, N_f, N_c and inner_stats exact.  FP64 sums and phi values:
+1e-12 relative for BS/Exact portfolios (device libm and summation order), with
+an absolute floor of 1e-12 times the largest sum.  Portfolios holding Unbiased
+(Milstein) options get 1e-9 relative: their estimator divides a fine-minus-
+coarse path difference by the sampled level's probability, so a 1-ulp
+difference between the device's and glibc's exp/log/sqrt is amplified by that
+cancellation and by 1/p (observed: up to 4e-11)."""
+import numpy as np
+import pytest
+
+from portfolio_util import reference_generated, synthetic, to_library, to_oracle
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-12
+RTOL_UNBIASED = 1e-9
+
+
+def _cmp(g, o, trace=True, rtol=RTOL, exact_work=False):
+    assert np.array_equal(g["stats"], o["stats"]), (g["stats"], o["stats"])
+    if exact_work:
+        assert g["work"] == o["work"]
+    else:
+        np.testing.assert_allclose(g["work"], o["work"], rtol=1e-12)
+    scale = max(1.0, float(np.max(np.abs(o["fsums"]))))
+    atol = rtol * scale
+    np.testing.assert_allclose(g["fsums"], o["fsums"], rtol=rtol, atol=atol)
+    np.testing.assert_allclose(g["diffsums"], o["diffsums"], rtol=rtol, atol=atol)
+    if trace:
+        assert np.array_equal(g["nf"], o["nf"])
+        assert np.array_equal(g["nc"], o["nc"])
+        np.testing.assert_allclose(g["phi_f"], o["phi_f"], rtol=rtol, atol=0.1 * rtol)
+        np.testing.assert_allclose(g["phi_c"], o["phi_c"], rtol=rtol, atol=0.1 * rtol)
+
+
+SCAL = dict(r=0.05, rho=0.2, tau=0.02, dt_h0inv=8, dt_beta=4, dt_prob_expo=-1.5,
+            use_coarse_delta_cv=False, use_milstein=True)
+
+
+def _setup(nm, orc, d, cores, N0, beta, r_adapt, scal, seed, form=0, chol=None, rule=None):
+    nm.init_nestedmc_sampler(cores, N0, beta, r_adapt, True)
+    if rule:
+        nm.set_adaptive_rule(*rule)
+    nm.set_form(form)
+    pf, start, delta = to_library(nm, d, **scal)
+    nm.portfolio_init(pf, seed)
+    if chol is not None:
+        nm.portfolio_set_correlation(chol)
+    o = orc.new(cores, N0, beta, r_adapt, True,
+                **({"adapt_rule": rule[0], "cap_log2": rule[1]} if rule else {}))
+    o.portfolio_init(to_oracle(d, start, delta, **scal), seed, chol)
+    return pf, o
+
+
+@pytest.mark.parametrize("sub,cv,phi,ps,cores", [
+    (1, True, 0, 1, 1), (1, True, 1, 1, 4), (-1, False, 1, 1, 2), (0, True, 0, 1, 1),
+    (1, True, 2, 2, 3), (1, False, 3, 3, 1)])
+def test_plain_reference_portfolio(nm, oracle_f64, sub, cv, phi, ps, cores):
+    """The reference's generated portfolio (BS / Exact / Unbiased-Milstein)."""
+    print("case", sub, cv, phi, ps, cores, flush=True)
+    d = reference_generated()
+    scal = dict(SCAL, max_loss=0.02, subsample=sub, use_delta_cv=cv, use_antithetic_risk=cv,
+                phi_type=phi, phi_size=ps)
+    pf, o = _setup(nm, oracle_f64, d, cores, 8, 4, 1.5, scal, 3)
+    for ell, M in ((0, 9), (1, 6), (2, 3)):
+        g = nm.sample(ell, M, trace=True, phi_size=pf.phi_size)
+        _cmp(g, o.sample(ell, M, trace=True), rtol=RTOL_UNBIASED)
+    nm.cleanup_nestedmc_sampler()
+
+
+@pytest.mark.parametrize("counts,digitals,cores", [((10, 30, 0), True, 1), ((0, 40, 0), False, 8),
+                                                   ((20, 20, 0), True, 64)])
+def test_plain_fast_path_portfolio(nm, oracle_f64, counts, digitals, cores):
+    """Sub-sampled BS + Exact portfolios: the wave-parallel inner loop."""
+    d = synthetic(counts=counts, digitals=digitals, seed=len(counts) + cores)
+    scal = dict(SCAL, max_loss=0.01, subsample=1, use_delta_cv=True, use_antithetic_risk=True,
+                phi_type=1, phi_size=1)
+    pf, o = _setup(nm, oracle_f64, d, cores, 32, 4, 1.5, scal, 5)
+    for ell, M in ((0, 200), (1, 120), (3, 30), (2, 50)):
+        _cmp(nm.sample(ell, M, trace=True), o.sample(ell, M, trace=True))
+    nm.cleanup_nestedmc_sampler()
+
+
+def test_plain_cholesky_and_drop_in_entry(nm, oracle_f64):
+    """General correlation (Cholesky factor) and the portfolio_sample entry."""
+    d = synthetic(Q=5, counts=(5, 35, 0), seed=4)
+    C = 0.3 * np.ones((5, 5)) + 0.7 * np.eye(5)
+    L = np.linalg.cholesky(C)
+    scal = dict(SCAL, max_loss=0.01, subsample=1, use_delta_cv=True, use_antithetic_risk=True,
+                phi_type=0, phi_size=1)
+    pf, o = _setup(nm, oracle_f64, d, 4, 16, 4, 1.5, scal, 7, chol=L)
+    for ell, M in ((0, 50), (2, 20)):
+        g = nm.portfolio_sample(ell, M)
+        _cmp(g, o.sample(ell, M), trace=False)
+    nm.cleanup_nestedmc_sampler()
+
+
+@pytest.mark.parametrize("which", ["reference", "synthetic", "cholesky"])
+def test_batched_fp64_matches_oracle(nm, oracle_f64, which):
+    if which == "reference":
+        d = reference_generated()
+        chol = None
+    else:
+        d = synthetic(Q=6, counts=(10, 50, 0), seed=11)
+        chol = np.linalg.cholesky(0.25 * np.ones((6, 6)) + 0.75 * np.eye(6)) if which == "cholesky" else None
+    scal = dict(SCAL, max_loss=0.02, subsample=1, use_delta_cv=True, use_antithetic_risk=True,
+                phi_type=1, phi_size=1)
+    pf, o = _setup(nm, oracle_f64, d, 1, 8, 4, 1.5, scal, 9, form=1, chol=chol)
+    for ell, M, off in ((0, 300, 0), (1, 200, 0), (3, 40, 0), (1, 100, 200)):
+        g = nm.sample(ell, M, outer_offset=off, trace=True)
+        _cmp(g, o.sample(ell, M, keying=1, outer_offset=off, trace=True),
+             rtol=RTOL_UNBIASED if which == "reference" else RTOL)
+    nm.cleanup_nestedmc_sampler()
+    nm.set_form(nm.FORM_PLAIN)
+
+
+def test_mixed_portfolio_agrees_with_fp64(nm, oracle_f64):
+    """Mixed FP32+FP64 form on an Exact portfolio with digitals: wherever the
+    FP32 refinement decisions agree, every indicator equals the FP64 one."""
+    d = synthetic(Q=4, counts=(0, 200, 0), seed=21)
+    scal = dict(SCAL, max_loss=0.005, subsample=1, use_delta_cv=True, use_antithetic_risk=True,
+                phi_type=0, phi_size=1)
+    pf, o = _setup(nm, oracle_f64, d, 1, 32, 2, 0.0, scal, 13, form=2, rule=(1, 4))
+    for ell, M in ((0, 3000), (1, 1500), (3, 300)):
+        g = nm.sample(ell, M, outer_offset=0, trace=True)
+        r = o.sample(ell, M, keying=1, outer_offset=0, trace=True)
+        same = (g["nf"] == r["nf"]) & (g["nc"] == r["nc"])
+        assert same.mean() > 0.99, same.mean()
+        assert np.array_equal(g["phi_f"][same], r["phi_f"][same])
+        assert np.array_equal(g["phi_c"][same], r["phi_c"][same])
+        print("ell", ell, "count flips", int((~same).sum()), "mixed stats", nm.mixed_stats())
+    nm.cleanup_nestedmc_sampler()
+    nm.set_form(nm.FORM_PLAIN)
+
+
+def test_portfolio_errors(nm):
+    """Host-side validation: unsorted probabilities are rejected with a status."""
+    nm.init_nestedmc_sampler(1, 8, 4, 0.0, True)
+    d = synthetic(counts=(0, 5, 0), seed=1)
+    d["prob"] = d["prob"][::-1].copy()
+    pf, _, _ = to_library(nm, d, **dict(SCAL, max_loss=0.0))
+    with pytest.raises(nm.NmlmcError):
+        nm.portfolio_init(pf, 1)
+    nm.cleanup_nestedmc_sampler()

@@ -226,7 +226,7 @@ def test_batched_keying_layout(O):
         Y, _ = O.boxmuller(blk[0], blk[1])
         zs = []
         for u in range(2):
-            b = O.threefry([u, 65 << 40], key)
+            b = O.threefry([0, (65 << 40) | u], key)
             zs += list(O.boxmuller(b[0], b[1]))
         mean = sum(Y + z for z in zs) / 4
         assert o["phi_f"][m] == pytest.approx(50 + mean, rel=1e-15)
