@@ -19,6 +19,11 @@ x ops per outer sample, counts in nested-risk-mlmc_amd/opcount.json) / the
 kernel's duration from HIP events on the library's stream; peak = 256 CU x 64
 lanes x 2.4 GHz = 39.3 T lane-ops/s.
 
+traffic: HBM bytes per launch of the dominant kernel from two rocprofv3 --pmc
+passes (FETCH_SIZE, WRITE_SIZE), each scaled by what the same counter reads for
+a plain device copy of known size (python bench.py --copy-calibration); the
+scaled figures are kept in profiles/hbm_traffic.json and read from there.
+
 cpu_baseline: the oracle (C restatement, FP64, same keying and workload) on the
 host cores with OpenMP, over a bounded sample of the sweep (M_l scaled down).
 """
@@ -64,6 +69,32 @@ def ops_per_sample(form):
         oc = json.load(f)
     key = "mixed" if form == 2 else "fp64"
     return oc["shift_model"][key]["inner"], oc["shift_model"][key]["outer"]
+
+
+def traffic_bytes(kernel):
+    """HBM bytes per launch of ``kernel`` measured by rocprofv3 --pmc and scaled
+    by the copy calibration, or None when no measurement is recorded."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f)["kernels"][kernel]
+    except (OSError, KeyError, ValueError):
+        return None
+    return rec["bytes_per_launch"]
+
+
+def copy_calibration(nbytes=1 << 30, reps=4):
+    """A plain device-to-device copy of known size for scaling FETCH_SIZE and
+    WRITE_SIZE (torch is only device-memory plumbing here)."""
+    import torch
+    n = nbytes // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "reps": reps,
+                      "kernel": "copy of %d float32" % n}))
 
 
 def roofline(recs, form):
@@ -141,7 +172,12 @@ def main():
     ap.add_argument("--no-tte", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-M0", type=float, default=1e6)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device copy (for scaling rocprofv3 byte counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the outer samples "
               "would shard with one all-reduce of ~300 B per level call); run with --gpus 1.")

@@ -126,6 +126,27 @@ void portfolio_comp_params(portfolio_data* data);
  * that share one outer sample's inner loop (rounded to a power of two <= 64). */
 void portfolio_sample(uint32_t ell, uint64_t override_dyn_inner, uint64_t M, sample_output* out);
 
+/* portfolio_sampler.cpp:758-781.  Moments [0, moments) of *M delta samples of
+ * Unbiased option option_index (Milstein against exact paths, plain keying:
+ * uses and advances the inner streams of the virtual threads); *M is
+ * unchanged, *total_work is overwritten. */
+void portfolio_sample_delta_unbiased(uint32_t option_index, uint64_t* M, double* sums, uint32_t moments,
+                                     double* total_work);
+
+/* portfolio_sampler.cpp:914-934.  Adds to each Unbiased option's delta_est the
+ * mean of batches of 1000 delta samples drawn until 3 sqrt(V/M) < TOL. */
+void portfolio_estimate_delta_for_unbiased(portfolio_data* data, double TOL);
+
+/* portfolio_sampler.cpp:792-904.  A synthetic portfolio of P options on Q
+ * stocks (P >= 2Q), split over BS / Exact / Unbiased by floor(probs[m] * P),
+ * priced, delta-hedged, normalised and sorted; the arrays are allocated here
+ * and released by portfolio_free_data. */
+portfolio_data portfolio_gen_data(uint32_t P, uint32_t Q, uint64_t seed, double* probs, bool uniform_weights,
+                                  bool uniform_prob);
+
+/* portfolio_sampler.cpp:906-912. */
+void portfolio_free_data(portfolio_data* data);
+
 /* ---- extensions (no reference counterpart) -------------------------------- */
 
 enum {

@@ -212,8 +212,6 @@ __global__ __launch_bounds__(256) void k_b_final_count(typename P::Args A) {
     block_flush(acc, &c.ctl->acc);
 }
 
-// Writes each outer sample's final entries into its size class's region; block
-// ranks come from LDS counters, one global atomic per class and block.
 // Writes each outer sample's final entries: whole samples after the class
 // regions, split samples into their size class's region; block ranks come
 // from LDS counters, one global atomic per class and block.

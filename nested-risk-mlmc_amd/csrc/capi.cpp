@@ -276,12 +276,6 @@ int Engine::sample_model_parity(uint32_t ell, uint64_t M, uint32_t lanes_overrid
     return set_error(NMLMC_OK, "");
 }
 
-static uint32_t pow2_floor_u32(uint64_t x) {
-    uint32_t g = 1;
-    while ((uint64_t)g * 2 <= x && g < 64) g <<= 1;
-    return g;
-}
-
 int Engine::device_cus() {
     if (cus > 0) return cus;
     int n = 0;
@@ -376,7 +370,7 @@ int Engine::batched_common(uint32_t ell, uint64_t M, uint64_t outer_offset, Batc
     }
     c.base_min = bmin;
     c.fcap = 0;
-    return prepare_batched(M, c);
+    return prepare_batched(std::min(M, batched_chunk()), c);
 }
 
 int Engine::run_batched(uint32_t ell, uint64_t M, BatchedCommon& c, sample_output* out,
@@ -385,42 +379,78 @@ int Engine::run_batched(uint32_t ell, uint64_t M, BatchedCommon& c, sample_outpu
     TraceBufs tb;
     int rc = trace_alloc(M, tr_phi_f || tr_phi_c || tr_nf || tr_nc, tb);
     if (rc) return rc;
-    c.tr_phi_f = tb.phi_f;
-    c.tr_phi_c = tb.phi_c;
-    c.tr_nf = tb.nf;
-    c.tr_nc = tb.nc;
-    for (int attempt = 0; attempt < 4; attempt++) {
-        hipError_t e = hipEventRecord(ev0, stream);
-        if (e != hipSuccess) return hip_fail(e, "hipEventRecord");
-        e = launch();
-        if (e != hipSuccess) return hip_fail(e, "launch batched level");
-        if ((e = hipEventRecord(ev1, stream)) != hipSuccess) return hip_fail(e, "hipEventRecord");
-        e = hipMemcpyAsync(bat_ctl_h, bat_ctl, sizeof(BatchedCtl), hipMemcpyDeviceToHost, stream);
-        if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync ctl");
-        if ((e = hipStreamSynchronize(stream)) != hipSuccess) return hip_fail(e, "batched kernels");
-        if (!bat_ctl_h->overflow) break;
-        // More final groups than the pool holds: grow it and redo the call (the
-        // draws are keyed, so the redo is identical).
-        const uint64_t need = (uint64_t)bat_ctl_h->final_total + 4096;
-        if (bat_fpool) hipFree(bat_fpool);
-        bat_fpool = nullptr;
-        bat_fcap = 0;
-        if ((e = hipMalloc(&bat_fpool, sizeof(FinalEntry) * need)) != hipSuccess) return hip_fail(e, "hipMalloc finals");
-        bat_fcap = need;
-        c.fpool = bat_fpool;
-        c.fcap = (uint32_t)std::min<uint64_t>(need, 0xffffffffull);
+    // The level call runs in chunks of at most batched_chunk() outer samples:
+    // the queues and item states hold one chunk.  Outer samples are keyed by
+    // their global index, so only the order of the FP64 level sums depends on
+    // the chunk size.
+    const uint64_t chunk = batched_chunk();
+    const uint64_t offset0 = c.offset;
+    Partial total;
+    std::memset(&total, 0, sizeof(total));
+    double ms_total = 0;
+    for (uint64_t start = 0; start < M; start += chunk) {
+        const uint64_t n = std::min(chunk, M - start);
+        c.M = n;
+        c.offset = offset0 + start;
+        c.tr_phi_f = tb.phi_f ? tb.phi_f + start : nullptr;
+        c.tr_phi_c = tb.phi_c ? tb.phi_c + start : nullptr;
+        c.tr_nf = tb.nf ? tb.nf + start : nullptr;
+        c.tr_nc = tb.nc ? tb.nc + start : nullptr;
+        for (int attempt = 0; attempt < 4; attempt++) {
+            hipError_t e = hipEventRecord(ev0, stream);
+            if (e != hipSuccess) return hip_fail(e, "hipEventRecord");
+            e = launch();
+            if (e != hipSuccess) return hip_fail(e, "launch batched level");
+            if ((e = hipEventRecord(ev1, stream)) != hipSuccess) return hip_fail(e, "hipEventRecord");
+            e = hipMemcpyAsync(bat_ctl_h, bat_ctl, sizeof(BatchedCtl), hipMemcpyDeviceToHost, stream);
+            if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync ctl");
+            if ((e = hipStreamSynchronize(stream)) != hipSuccess) return hip_fail(e, "batched kernels");
+            if (!bat_ctl_h->overflow) break;
+            // More final groups than the pool holds: grow it and redo the chunk
+            // (the draws are keyed, so the redo is identical).
+            const uint64_t need = (uint64_t)bat_ctl_h->final_total + 4096;
+            if (bat_fpool) hipFree(bat_fpool);
+            bat_fpool = nullptr;
+            bat_fcap = 0;
+            if ((e = hipMalloc(&bat_fpool, sizeof(FinalEntry) * need)) != hipSuccess)
+                return hip_fail(e, "hipMalloc finals");
+            bat_fcap = need;
+            c.fpool = bat_fpool;
+            c.fcap = (uint32_t)std::min<uint64_t>(need, 0xffffffffull);
+        }
+        if (bat_ctl_h->overflow) return set_error(NMLMC_ERR_DEVICE, "final-group pool overflow");
+        float ms = 0;
+        hipEventElapsedTime(&ms, ev0, ev1);
+        ms_total += ms;
+        const Partial& a = bat_ctl_h->acc;
+        for (int i = 0; i < 16; i++) {
+            total.f[i] += a.f[i];
+            total.d[i] += a.d[i];
+        }
+        total.work += a.work;
+        for (int i = 0; i < kStatsExt; i++) total.stats[i] += a.stats[i];
     }
-    if (bat_ctl_h->overflow) return set_error(NMLMC_ERR_DEVICE, "final-group pool overflow");
-    float ms = 0;
-    hipEventElapsedTime(&ms, ev0, ev1);
-    last_ms = ms;
-    std::memcpy(h_out, &bat_ctl_h->acc, sizeof(Partial));
+    c.offset = offset0;
+    last_ms = ms_total;
+    std::memcpy(h_out, &total, sizeof(Partial));
     copy_out(ell, out);
     mixed_reeval = h_out->stats[6];
     mixed_flips = h_out->stats[7];
     rc = trace_finish(M, tb, tr_phi_f, tr_phi_c, tr_nf, tr_nc);
     if (rc) return rc;
     return set_error(NMLMC_OK, "");
+}
+
+// Outer samples per batched launch chain: 2^23 (about 2.8 GB of queues and item
+// states, plus 8 B x d per outer sample of stock states for the portfolio);
+// NMLMC_BATCH_CHUNK overrides it.
+uint64_t Engine::batched_chunk() {
+    uint64_t ch = 1ull << 23;
+    if (const char* s = getenv("NMLMC_BATCH_CHUNK")) {
+        const uint64_t v = strtoull(s, nullptr, 10);
+        if (v > 0) ch = v;
+    }
+    return ch;
 }
 
 uint32_t Engine::batched_blocks() { return (uint32_t)device_cus() * 8; }
@@ -438,10 +468,12 @@ int Engine::sample_batched(uint32_t ell, uint64_t M, uint64_t outer_offset, uint
     H.c.phi_type = mp.phi_type;
     H.c.nps = mp.variant == MODEL_REF ? 2 : 1;
     H.c.ups = 0;
-    // Segment: 16 Box-Muller pairs per lane (32 samples of the shift model).
-    H.c.S = lanes_override ? lanes_override : 16;
+    // Segment: 64 inner units per lane (128 samples of the shift model, one
+    // Threefry block + one Box-Muller pair per unit); the sweep of 32..256 over
+    // levels 0..12 of the adaptive shift model was fastest at 64.
+    H.c.S = lanes_override ? lanes_override : 64;
     if (const char* sS = getenv("NMLMC_SEGMENT")) H.c.S = (uint32_t)strtoul(sS, nullptr, 10);
-    if (H.c.S == 0) H.c.S = 16;
+    if (H.c.S == 0) H.c.S = 64;
     last_G = H.c.S;
     H.blocks = batched_blocks();
     H.mixed = form == NMLMC_FORM_MIXED;

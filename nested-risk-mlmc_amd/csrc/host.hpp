@@ -91,7 +91,9 @@ struct Engine {
                                 uint64_t* tr_nf, uint64_t* tr_nc);
     int cus = 0;
     int device_cus();
+    int sample_delta_unbiased(uint32_t option_index, uint64_t M, uint32_t moments, double* sums, double* work);
     uint32_t batched_blocks();
+    uint64_t batched_chunk();
     int prepare_batched(uint64_t M, BatchedCommon& c);
     int batched_common(uint32_t ell, uint64_t M, uint64_t outer_offset, BatchedCommon& c);
     int run_batched(uint32_t ell, uint64_t M, BatchedCommon& c, sample_output* out,

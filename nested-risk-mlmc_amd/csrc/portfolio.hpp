@@ -60,6 +60,17 @@ struct PortfolioParityHost {
 };
 hipError_t launch_portfolio_parity(const PortfolioParityHost& H, hipStream_t s);
 
+// kernel_sample_delta (portfolio_sampler.cpp:702-746): T virtual threads, one
+// lane each, delta samples of one Unbiased option from the inner streams.
+struct PortfolioDeltaHost {
+    PortfolioDevice pf;
+    uint64_t seed, M;
+    uint32_t T, option_index, moments;
+    StreamState* states;
+    Partial* partials;  // one per virtual thread
+};
+hipError_t launch_portfolio_delta(const PortfolioDeltaHost& H, hipStream_t s);
+
 struct PortfolioBatchedHost {
     BatchedCommon c;
     PortfolioDevice pf;

@@ -207,12 +207,12 @@ int Engine::sample_portfolio_batched(uint32_t ell, uint64_t M, uint64_t outer_of
     if (const char* sS = getenv("NMLMC_SEGMENT")) H.c.S = (uint32_t)strtoul(sS, nullptr, 10);
     if (H.c.S == 0) H.c.S = 32;
     last_G = H.c.S;
-    const size_t need = (size_t)M * pf.d;
+    const size_t need = (size_t)std::min(M, batched_chunk()) * pf.d;
     if (need > pf_R_cap) {
+        size_t cap = pf_R_cap ? pf_R_cap : 1024;
         if (pf_R) hipFree(pf_R);
         pf_R = nullptr;
         pf_R_cap = 0;
-        size_t cap = pf_R_cap ? pf_R_cap : 1024;
         while (cap < need) cap *= 2;
         hipError_t e = hipMalloc(&pf_R, sizeof(double) * cap);
         if (e != hipSuccess) return hip_fail(e, "hipMalloc outer states");
@@ -225,6 +225,46 @@ int Engine::sample_portfolio_batched(uint32_t ell, uint64_t M, uint64_t outer_of
     last_kernel = "k_b_final";
     return run_batched(ell, M, H.c, out, [&]() { return launch_portfolio_batched(H, stream); }, tr_phi_f,
                        tr_phi_c, tr_nf, tr_nc);
+}
+
+// portfolio_sample_delta_unbiased (portfolio_sampler.cpp:758-781): moments of
+// M delta samples of Unbiased option `option_index` from the plain-keyed inner
+// streams; reduced in thread order.
+int Engine::sample_delta_unbiased(uint32_t option_index, uint64_t M, uint32_t moments, double* sums,
+                                  double* work) {
+    if (problem != PROBLEM_PORTFOLIO || !d_states) return set_error(NMLMC_ERR_STATE, "no portfolio initialised");
+    if (option_index >= pf.nopt[2])
+        return set_error(NMLMC_ERR_ARG, "option_index %u: the portfolio has %u Unbiased options", option_index,
+                         pf.nopt[2]);
+    if (moments == 0 || moments > (uint32_t)kMaxMoments || !sums || !work)
+        return set_error(NMLMC_ERR_ARG, "moments must be 1..%d with output buffers", kMaxMoments);
+    int rc = ensure_partials(cores);
+    if (rc) return rc;
+    PortfolioDeltaHost H;
+    H.pf = pf;
+    H.seed = seed;
+    H.M = M;
+    H.T = cores;
+    H.option_index = option_index;
+    H.moments = moments;
+    H.states = d_states;
+    H.partials = d_partials;
+    hipError_t e = hipEventRecord(ev0, stream);
+    if (e != hipSuccess) return hip_fail(e, "hipEventRecord");
+    if ((e = launch_portfolio_delta(H, stream)) != hipSuccess) return hip_fail(e, "launch k_portfolio_delta");
+    if ((e = hipEventRecord(ev1, stream)) != hipSuccess) return hip_fail(e, "hipEventRecord");
+    if ((e = launch_reduce_partials(d_partials, cores, d_out, stream)) != hipSuccess)
+        return hip_fail(e, "launch k_reduce_partials");
+    last_kernel = "k_portfolio_delta";
+    if ((e = hipMemcpyAsync(h_out, d_out, sizeof(Partial), hipMemcpyDeviceToHost, stream)) != hipSuccess)
+        return hip_fail(e, "hipMemcpyAsync result");
+    if ((e = hipStreamSynchronize(stream)) != hipSuccess) return hip_fail(e, "kernel execution");
+    float ms = 0;
+    hipEventElapsedTime(&ms, ev0, ev1);
+    last_ms = ms;
+    for (uint32_t i = 0; i < moments; i++) sums[i] = h_out->f[i];
+    *work = h_out->work;
+    return set_error(NMLMC_OK, "");
 }
 
 }  // namespace nmlmc
@@ -280,6 +320,51 @@ void portfolio_sample(uint32_t ell, uint64_t override_dyn_inner, uint64_t M, sam
         return;
     }
     g_engine.sample(ell, M, UINT64_MAX, (uint32_t)override_dyn_inner, out, nullptr, nullptr, nullptr, nullptr);
+}
+
+// portfolio_sample_delta_unbiased (portfolio_sampler.cpp:758-781).  `sums`
+// receives `moments` values (the reference copies fine_moments values into it
+// whatever `moments` is); *M is unchanged; *total_work is overwritten.
+void portfolio_sample_delta_unbiased(uint32_t option_index, uint64_t* M, double* sums, uint32_t moments,
+                                     double* total_work) {
+    if (!M) {
+        set_error(NMLMC_ERR_ARG, "M is NULL");
+        return;
+    }
+    g_engine.sample_delta_unbiased(option_index, *M, moments, sums, total_work);
+}
+
+// portfolio_estimate_delta_for_unbiased (portfolio_sampler.cpp:914-934): for
+// each Unbiased option of the staged portfolio, batches of 1000 delta samples
+// until 3 sqrt(V / M) < TOL; the mean is added to the caller's delta_est.  The
+// loop also stops, with an error status, if the variance is not finite or 2^40
+// samples do not reach TOL (the reference would loop for ever).
+void portfolio_estimate_delta_for_unbiased(portfolio_data* pdata, double TOL) {
+    if (!pdata) {
+        set_error(NMLMC_ERR_ARG, "portfolio data is NULL");
+        return;
+    }
+    option_params* ops = (option_params*)pdata->options[2].data;
+    for (size_t j = 0; j < pdata->options[2].size; j++) {
+        uint64_t M = 1000, total_M = 0;
+        double total[2] = {0., 0.};
+        double sums[2], work = 0.;
+        while (true) {
+            if (g_engine.sample_delta_unbiased((uint32_t)j, M, 2, sums, &work)) return;
+            total[0] += sums[0];
+            total[1] += sums[1];
+            total_M += M;
+            const double mean = total[0] / total_M;
+            const double Vl = total[1] / total_M - mean * mean;
+            if (3. * std::sqrt(Vl / total_M) < TOL) break;
+            if (!std::isfinite(Vl) || total_M >= (1ull << 40)) {
+                set_error(NMLMC_ERR_ARG, "delta of Unbiased option %zu does not reach TOL %g", j, TOL);
+                return;
+            }
+        }
+        ops[j].delta_est += total[0] / total_M;
+    }
+    set_error(NMLMC_OK, "");
 }
 
 int nmlmc_portfolio_set_correlation(const double* chol, uint32_t d) {

@@ -310,4 +310,86 @@ hipError_t launch_portfolio_parity(const PortfolioParityHost& H, hipStream_t s) 
     return hipGetLastError();
 }
 
+// sampler::sample_delta_unbiased (portfolio_sampler.cpp:397-444): pathwise
+// delta of an Unbiased option from an antithetic pair, dt_h0inv Milstein steps
+// (volatility sigma and -sigma) against the exact GBM paths driven by the same
+// Brownian increment.
+__device__ double delta_unbiased_d(const PortfolioDevice& pf, const OptionRec& o, DStream& in, double& work) {
+    const StockRec st = pf.stocks[o.stock_ind];
+    const double r = pf.r, sigma = st.sigma;
+    const double T = o.maturity - pf.tau;
+    const uint64_t N = pf.dt_h0inv;
+    const double dW_0_tau = pf.stau * in.randn();
+    const double S1 = simulate_stock_d(st.S0, r, sigma, pf.tau, dW_0_tau);
+    const double S2 = simulate_stock_d(st.S0, r, sigma, pf.tau, -dW_0_tau);
+    double STf[2] = {S1, S2};
+    const double sig[2] = {sigma, -sigma};
+    const double dt = T / (double)N;
+    const double sqrt_dt = sqrt(dt);
+    double W = 0.;
+    for (uint64_t n = 0; n < N; n++) {
+        const double dW = sqrt_dt * in.randn();
+        W += dW;
+        for (int i = 0; i < 2; i++) {
+            const double m = pf.use_milstein ? (0.5 * (sig[i] * sig[i]) * (dW * dW - dt)) : 0.;
+            STf[i] += STf[i] * (r * dt + sig[i] * dW + m);
+        }
+    }
+    const double ST0 = simulate_stock_d(S1, r, sigma, T, W);
+    const double ST1 = simulate_stock_d(S2, r, sigma, T, -W);
+    work += (double)(2 * N) * o.W_est;
+    double val = 0.;
+    val += derv_from_stock_d(o, STf[0], st.S0, r);
+    val += derv_from_stock_d(o, STf[1], st.S0, r);
+    val -= derv_from_stock_d(o, ST0, st.S0, r);
+    val -= derv_from_stock_d(o, ST1, st.S0, r);
+    return val / (2.0 * o.w);
+}
+
+// One lane per virtual thread t: its libgomp static share of the M samples, in
+// order, from its inner stream {seed, t}; moments and work go to partials[t].
+// The work is tiny (dt_h0inv steps per sample), so lanes are independent and no
+// lane of a block waits on another.
+__global__ __launch_bounds__(64) void k_portfolio_delta(PortfolioDeltaHost A) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= A.T) return;
+    uint64_t q = A.M / A.T, rr = A.M % A.T;
+    if (t < rr) {
+        q++;
+        rr = 0;
+    }
+    const uint64_t s0 = q * t + rr, e0 = s0 + q;
+    StreamState si = A.states[2 * t + 1];
+    DStream in;
+    in.init(A.seed, t, 0, si.P, si.saved, si.is_saved);
+    const OptionRec o = A.pf.options[2][A.option_index];
+    double f[kMaxMoments] = {0., 0., 0., 0.};
+    double work = 0.;
+    for (uint64_t m = s0; m < e0; m++) {
+        const double dl = delta_unbiased_d(A.pf, o, in, work);
+        double p = 1.;
+        for (uint32_t j = 0; j < A.moments; j++) {
+            p *= dl;
+            f[j] += p;
+        }
+    }
+    si.P = in.P;
+    si.saved = in.saved;
+    si.is_saved = in.is_saved;
+    A.states[2 * t + 1] = si;
+    Partial& P = A.partials[t];
+    for (int i = 0; i < kMaxPhi * kMaxMoments; i++) {
+        P.f[i] = i < kMaxMoments ? f[i] : 0.;
+        P.d[i] = 0.;
+    }
+    P.work = work;
+    for (int i = 0; i < kStatsExt; i++) P.stats[i] = 0;
+}
+
+hipError_t launch_portfolio_delta(const PortfolioDeltaHost& H, hipStream_t s) {
+    const uint32_t blocks = (H.T + 63) / 64;
+    hipLaunchKernelGGL(k_portfolio_delta, dim3(blocks), dim3(64), 0, s, H);
+    return hipGetLastError();
+}
+
 }  // namespace nmlmc

@@ -81,7 +81,8 @@ EXPORTS = [
     "nmlmc_set_form", "nmlmc_get_form", "nmlmc_set_adaptive_rule", "nmlmc_model_init",
     "nmlmc_portfolio_set_correlation", "nmlmc_sample", "nmlmc_mixed_stats",
     "nmlmc_last_kernel_ms", "nmlmc_last_kernel_name", "nmlmc_last_group_lanes",
-    "nmlmc_mlmc_run", "nmlmc_mlmc_run_cb",
+    "nmlmc_mlmc_run", "nmlmc_mlmc_run_cb", "portfolio_sample_delta_unbiased",
+    "portfolio_estimate_delta_for_unbiased", "portfolio_gen_data", "portfolio_free_data",
 ]
 
 _lib = None
@@ -112,6 +113,10 @@ def load(path=LIB_PATH):
         "portfolio_init": (None, [C.POINTER(PortfolioData), u64]),
         "portfolio_comp_params": (None, [C.POINTER(PortfolioData)]),
         "portfolio_sample": (None, [u32, u64, u64, C.POINTER(SampleOutput)]),
+        "portfolio_sample_delta_unbiased": (None, [u32, u64p, dp, u32, dp]),
+        "portfolio_estimate_delta_for_unbiased": (None, [C.POINTER(PortfolioData), dbl]),
+        "portfolio_gen_data": (PortfolioData, [u32, u32, u64, dp, C.c_bool, C.c_bool]),
+        "portfolio_free_data": (None, [C.POINTER(PortfolioData)]),
         "nmlmc_last_error": (i32, []),
         "nmlmc_last_error_string": (C.c_char_p, []),
         "nmlmc_set_form": (i32, [i32]),
@@ -380,3 +385,59 @@ def portfolio_set_correlation(chol):
 
 def portfolio_cleanup():
     load().portfolio_cleanup()
+
+
+def portfolio_sample_delta_unbiased(option_index, M, moments=2):
+    """Moments of M delta samples of Unbiased option ``option_index`` and the work."""
+    sums = np.zeros(moments)
+    work = C.c_double(0)
+    m = C.c_uint64(M)
+    load().portfolio_sample_delta_unbiased(option_index, C.byref(m), sums.ctypes.data_as(
+        C.POINTER(C.c_double)), moments, C.byref(work))
+    check()
+    return sums, work.value
+
+
+def portfolio_estimate_delta_for_unbiased(pf, TOL):
+    """Adds the estimated pathwise delta to each Unbiased option of ``pf``."""
+    load().portfolio_estimate_delta_for_unbiased(C.byref(pf.c), TOL)
+    check()
+
+
+def _flatten_portfolio_data(pd):
+    """Arrays of a portfolio_data in staged order (method-major)."""
+    Q = pd.stocks.size
+    st = C.cast(pd.stocks.data, C.POINTER(StockParams)) if Q else None
+    out = {"S0": np.array([st[i].S0 for i in range(Q)]), "mu": np.array([st[i].mu for i in range(Q)]),
+           "sigma": np.array([st[i].sigma for i in range(Q)]),
+           "counts": np.array([pd.options[m].size for m in range(3)], np.uint32)}
+    cols = {k: [] for k in ("strike", "maturity", "start", "w", "delta_est", "stock_ind", "type", "prob",
+                            "W_est")}
+    for m in range(3):
+        n = pd.options[m].size
+        if not n:
+            continue
+        ops = C.cast(pd.options[m].data, C.POINTER(OptionParams))
+        for j in range(n):
+            for k in cols:
+                cols[k].append(getattr(ops[j], k))
+    for k, v in cols.items():
+        out[k] = np.array(v, dtype=np.uint32 if k == "stock_ind" else np.int32 if k == "type" else np.float64)
+    scal = {k: getattr(pd, k) for k in ("r", "rho", "s1_rho2", "max_loss", "tau", "dt_h0inv", "dt_beta",
+                                         "dt_prob_expo", "phi_type", "subsample", "use_delta_cv",
+                                         "use_coarse_delta_cv", "use_antithetic_risk", "use_milstein")}
+    scal["phi_size"] = pd._phi_size
+    return out, scal
+
+
+def portfolio_gen_data(P, Q, seed, probs, uniform_weights=False, uniform_prob=False):
+    """portfolio_gen_data: returns (arrays, scalars) and frees the C arrays."""
+    lib = load()
+    pr = np.ascontiguousarray(probs, np.float64)
+    pd = lib.portfolio_gen_data(P, Q, seed, pr.ctypes.data_as(C.POINTER(C.c_double)), uniform_weights,
+                                uniform_prob)
+    check()
+    try:
+        return _flatten_portfolio_data(pd)
+    finally:
+        lib.portfolio_free_data(C.byref(pd))

@@ -175,8 +175,10 @@ void orc_stream(uint64_t seed, uint64_t core, const int* kinds, int n, uint64_t*
 static inline uint64_t clog2_64(uint64_t x) { return 64 - __builtin_clzl(x - 1); }
 static inline uint32_t clog2_32(uint32_t x) { return 32 - __builtin_clz(x - 1); }
 
+/* uint_pow (common.hpp:146-157).  For x == 1 the reference's power-of-two branch
+ * evaluates clog2(1) = 64 - clz(0), which is undefined; 1^n = 1 is returned. */
 uint64_t orc_uint_pow(uint64_t x, uint64_t n) {
-    if (n == 0) return 1;
+    if (n == 0 || x == 1) return 1;
     if ((x & (x - 1)) == 0) return (uint64_t)1 << (n * clog2_64(x));
     uint64_t r;
     for (r = 1; n--; r *= x) {}
@@ -635,6 +637,43 @@ static real_t sample_inner_unbiased(orc_ctx* c, orc_sampler* s, const orc_option
     return (real_t)((prob_t)ret * L_prob_1);
 }
 
+/* sampler::sample_delta_unbiased (portfolio_sampler.cpp:397-444): pathwise
+ * delta of Unbiased option `o` from an antithetic pair of paths, the Milstein
+ * paths (dt_h0inv steps, sigma and -sigma) against the exact GBM paths driven by
+ * the same Brownian increment. */
+static real_t sample_delta_unbiased(orc_ctx* c, orc_sampler* s, const orc_option* o, work_t* tw) {
+    const orc_portfolio* pf = &c->pf;
+    const orc_stock* st = &pf->stocks[o->stock_ind];
+    const real_t S0 = st->S0, sigma = st->sigma, r = pf->r;
+    const real_t T = o->maturity - pf->tau;
+    const uint64_t N = pf->dt_h0inv;
+    real_t dW_0_tau = pf->stau * s_randn(s, s->inner);
+    real_t S1 = simulate_stock_r(S0, r, sigma, pf->tau, dW_0_tau);
+    real_t S2 = simulate_stock_r(S0, r, sigma, pf->tau, -dW_0_tau);
+    sum_t ST_f[2] = {S1, S2};
+    const real_t sig[2] = {sigma, -sigma};
+    real_t W = 0;
+    const real_t dt = T / N;
+    const real_t sqrt_dt = RSQRT(dt);
+    for (uint64_t n = 0; n < N; n++) {
+        real_t dW = sqrt_dt * s_randn(s, s->inner);
+        W += dW;
+        for (int i = 0; i < 2; i++) {
+            real_t m = pf->use_milstein ? (real_t)(0.5 * (sig[i] * sig[i]) * (dW * dW - dt)) : 0;
+            ST_f[i] += ST_f[i] * (r * dt + sig[i] * dW + m);
+        }
+    }
+    real_t ST0 = simulate_stock_r(S1, r, sigma, T, W);
+    real_t ST1 = simulate_stock_r(S2, r, sigma, T, -W);
+    *tw += (real_t)(2 * N) * o->W_est;
+    real_t val = 0;
+    val += option_derv_from_stock(o, (real_t)ST_f[0], S0, r);
+    val += option_derv_from_stock(o, (real_t)ST_f[1], S0, r);
+    val -= option_derv_from_stock(o, ST0, S0, r);
+    val -= option_derv_from_stock(o, ST1, S0, r);
+    return (real_t)(val / (2.0 * o->w));
+}
+
 /* sample_inner_option (382-396). */
 static real_t sample_inner_option(orc_ctx* c, orc_sampler* s, work_t* tw, int method,
                                   const orc_option* o) {
@@ -1054,6 +1093,42 @@ void orc_sample(orc_ctx* c, int keying, uint32_t ell, uint64_t M, uint64_t outer
             for (uint32_t i = 0; i < nd_sums; i++) diffsums[i] += acc[t].d[i];
         *work += acc[t].tw;
         for (int i = 0; i < NSTATS; i++) stats[i] += acc[t].st[i];
+    }
+    free(acc);
+}
+
+/* kernel_sample_delta + portfolio_sample_delta_unbiased (portfolio_sampler.cpp:
+ * 702-746, 758-781): T virtual threads, each its libgomp static share of M
+ * delta samples of Unbiased option `option_index` from its persistent inner
+ * stream; `moments` moments of the sample and the work are summed in thread order. */
+void orc_sample_delta_unbiased(orc_ctx* c, uint32_t option_index, uint64_t M, uint32_t moments,
+                               double* sums, double* work) {
+    const uint32_t T = c->cores;
+    orc_acc* acc = (orc_acc*)calloc(T ? T : 1, sizeof(orc_acc));
+    const orc_option* o = &c->pf.options[2][option_index];
+#pragma omp parallel for schedule(dynamic, 1)
+    for (uint32_t t = 0; t < T; t++) {
+        uint64_t q = M / T, rr = M % T;
+        if (t < rr) {
+            q++;
+            rr = 0;
+        }
+        const uint64_t s0 = q * t + rr, e0 = s0 + q;
+        orc_sampler* s = &c->samplers[t];
+        for (uint64_t m = s0; m < e0; m++) {
+            real_t d = sample_delta_unbiased(c, s, o, &acc[t].tw);
+            real_t p = 1.f; /* add_moments(&delta, 1, f_sums, moments) */
+            for (uint32_t j = 0; j < moments; j++) {
+                p *= d;
+                acc[t].f[j] += p;
+            }
+        }
+    }
+    for (uint32_t i = 0; i < moments; i++) sums[i] = 0;
+    *work = 0;
+    for (uint32_t t = 0; t < T; t++) {
+        for (uint32_t i = 0; i < moments; i++) sums[i] += acc[t].f[i];
+        *work += acc[t].tw;
     }
     free(acc);
 }

@@ -90,6 +90,12 @@ double orc_simulate_stock_dt(int with_coarse, const double* normals, double* S, 
                              uint32_t coarse_step);
 uint64_t orc_samples_count(orc_ctx* c, uint32_t ell, int which);  /* 0 base, 1 max */
 
+/* portfolio_sample_delta_unbiased (portfolio_sampler.cpp:758-781) with the
+ * plain keying: sums[0..moments) of the delta samples of Unbiased option
+ * option_index and their work. */
+void orc_sample_delta_unbiased(orc_ctx* c, uint32_t option_index, uint64_t M, uint32_t moments,
+                               double* sums, double* work);
+
 #ifdef __cplusplus
 }
 #endif

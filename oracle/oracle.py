@@ -157,6 +157,8 @@ class Oracle(_Common):
         lib.orc_sample.argtypes = [C.c_void_p, C.c_int, C.c_uint32, C.c_uint64, C.c_uint64, _dp,
                                    _dp, _dp, _u64p, _dp, _dp, _u64p, _u64p]
         lib.orc_samples_count.argtypes = [C.c_void_p, C.c_uint32, C.c_int]
+        lib.orc_sample_delta_unbiased.argtypes = [C.c_void_p, C.c_uint32, C.c_uint64, C.c_uint32,
+                                                  _dp, _dp]
         lib.orc_samples_count.restype = C.c_uint64
         lib.orc_real_bytes.restype = C.c_int
         self.real = real
@@ -214,6 +216,14 @@ class Oracle(_Common):
             out.update(phi_f=tr[0], phi_c=tr[1], nf=tr[2], nc=tr[3])
         return out
 
+    def sample_delta_unbiased(self, option_index, M, moments=2):
+        """portfolio_sample_delta_unbiased with the plain keying."""
+        sums = np.zeros(moments)
+        work = np.zeros(1)
+        self.lib.orc_sample_delta_unbiased(self.ctx, option_index, C.c_uint64(M), moments,
+                                           _ptr(sums, _dp), _ptr(work, _dp))
+        return sums, work[0]
+
     def samples_count(self, ell):
         return (self.lib.orc_samples_count(self.ctx, ell, 0),
                 self.lib.orc_samples_count(self.ctx, ell, 1))
@@ -248,6 +258,7 @@ class Ref(_Common):
              C.c_uint32, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int])
         lib.refp_portfolio_get_params.argtypes = [_dp, _dp]
         lib.refp_portfolio_init.argtypes = [C.c_uint64]
+        lib.refp_portfolio_sample_delta.argtypes = [C.c_uint32, C.c_uint64, C.c_uint32, _dp, _dp]
         lib.refp_portfolio_sample.argtypes = [C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32, _dp,
                                               _dp, _dp, _u64p]
         lib.refp_portfolio_gen.argtypes = [C.c_uint32, C.c_uint32, C.c_uint64, _dp, C.c_int,
@@ -320,6 +331,13 @@ class Ref(_Common):
         self.lib.refp_portfolio_sample(ell, M, self.moments, self.phi_size, _ptr(fs, _dp),
                                        _ptr(ds, _dp), _ptr(work, _dp), _ptr(stats, _u64p))
         return dict(fsums=fs, diffsums=ds, work=work[0], stats=stats)
+
+    def portfolio_sample_delta(self, option_index, M, moments=2):
+        sums = np.zeros(moments)
+        work = np.zeros(1)
+        self.lib.refp_portfolio_sample_delta(option_index, C.c_uint64(M), moments, _ptr(sums, _dp),
+                                             _ptr(work, _dp))
+        return sums, work[0]
 
     def portfolio_gen(self, P, Q, seed, probs, uniform_weights=False, uniform_prob=False):
         probs = np.ascontiguousarray(probs, np.float64)

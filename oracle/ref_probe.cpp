@@ -246,6 +246,18 @@ void refp_portfolio_sample(uint32_t ell, uint64_t M, uint32_t moments, uint32_t 
 
 void refp_portfolio_cleanup() { portfolio::portfolio_cleanup(); }
 
+// portfolio_sample_delta_unbiased: the reference copies fine_moments sums
+// whatever `moments` is, so it writes into a buffer of the maximum size and
+// `moments` values are returned.
+void refp_portfolio_sample_delta(uint32_t option_index, uint64_t M, uint32_t moments, double* sums,
+                                 double* work) {
+    double buf[16] = {0};
+    double w = 0;
+    portfolio::portfolio_sample_delta_unbiased(option_index, &M, buf, moments, &w);
+    for (uint32_t i = 0; i < moments; i++) sums[i] = buf[i];
+    *work = w;
+}
+
 // portfolio_gen_data, flattened: writes Q stocks and P options in staged order
 // (method-major, sorted) plus the scalar fields.
 void refp_portfolio_gen(uint32_t P, uint32_t Q, uint64_t seed, const double* probs,

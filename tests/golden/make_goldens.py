@@ -142,6 +142,32 @@ def portfolio_fixture(R):
         runs.append({"scalars": scal, "cores": cores, "N0": 8, "beta": 4, "r": 1.5,
                      "start": start.tolist(), "delta_est": delta.tolist(), "calls": calls})
     out["runs"] = runs
+    # portfolio_sample_delta_unbiased: consecutive calls (persistent streams).
+    deltas = []
+    for cores in (1, 3):
+        pf = dict(gen)
+        pf.update(r=0.05, rho=0.2, max_loss=0.02, tau=0.02, subsample=1, use_delta_cv=True,
+                  use_antithetic_risk=True, use_milstein=True, use_coarse_delta_cv=False,
+                  phi_type=0, phi_size=1, dt_h0inv=8, dt_beta=4, dt_prob_expo=-1.5)
+        R.init(cores, 8, 4, 1.5, True)
+        start, delta = R.portfolio_set(pf)
+        R.portfolio_init(5)
+        calls = []
+        for idx, M in ((0, 40), (3, 25), (0, 7)):
+            sums, work = R.portfolio_sample_delta(idx, M, 2)
+            calls.append({"option_index": idx, "M": M, "sums": sums.tolist(), "work": work})
+        deltas.append({"cores": cores, "seed": 5, "calls": calls})
+    out["delta_unbiased"] = deltas
+    # A second generated portfolio: other sizes and split, and all-equal
+    # probabilities (the order of equal keys is the sort's own).
+    extra = []
+    for P, Q, seed, probs, up in ((200, 7, 42, [0.5, 0.25, 0.25], False),
+                                  (64, 5, 3, [0.25, 0.5, 0.25], True)):
+        g = R.portfolio_gen(P, Q, seed, probs, uniform_prob=up)
+        extra.append({"P": P, "Q": Q, "seed": seed, "probs": probs, "uniform_prob": up,
+                      "data": {k: (v.tolist() if hasattr(v, "tolist") else v)
+                               for k, v in g.items()}})
+    out["gen_more"] = extra
     return out
 
 

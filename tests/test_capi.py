@@ -129,3 +129,27 @@ def test_mlmc_driver_callback_error_aborts(nm):
         raise RuntimeError("boom")
     with pytest.raises(nm.NmlmcError):
         nm.mlmc_run_cb(bad, 1e-2, 10)
+
+
+def test_portfolio_gen_data_matches_reference(nm):
+    """portfolio_gen_data (host only) against the reference build's output for
+    the same arguments: every field bit-identical, including the order of
+    equal sub-sampling probabilities, which std::sort decides."""
+    with open(os.path.join(ROOT, "tests", "golden", "reference_f64.json")) as f:
+        fx = json.load(f)["portfolio"]
+    g0 = fx["gen"]
+    cases = [dict(P=g0["P"], Q=g0["Q"], seed=g0["seed"], probs=g0["probs"], uniform_prob=False,
+                  data=g0["data"])] + fx["gen_more"]
+    for c in cases:
+        arr, scal = nm.portfolio_gen_data(c["P"], c["Q"], c["seed"], c["probs"], False,
+                                          c["uniform_prob"])
+        for k, v in c["data"].items():
+            assert np.array_equal(np.array(v, dtype=arr[k].dtype), arr[k]), (c["P"], k)
+        assert scal["max_loss"] == 0.7 and scal["subsample"] == 0 and scal["use_milstein"]
+
+
+def test_portfolio_gen_data_rejects_bad_split(nm):
+    with pytest.raises(nm.NmlmcError):
+        nm.portfolio_gen_data(10, 6, 1, [0.3, 0.5, 0.2])   # P < 2Q
+    with pytest.raises(nm.NmlmcError):
+        nm.portfolio_gen_data(10, 2, 1, [0.33, 0.33, 0.33])  # floors do not add up to P

@@ -114,6 +114,27 @@ def test_batched_fp64_matches_oracle(nm, oracle_f64, segment, monkeypatch):
     nm.set_form(nm.FORM_PLAIN)
 
 
+@pytest.mark.parametrize("form", [1, 2])
+def test_batched_chunked_level_calls(nm, oracle_f64, form, monkeypatch):
+    """A level call larger than the chunk runs as several launch chains (here
+    chunks of 97 outer samples, the last one ragged, and the continuing
+    offset): per-sample results equal the unchunked oracle's, sums agree."""
+    monkeypatch.setenv("NMLMC_BATCH_CHUNK", "97")
+    orc = _shift_setup(nm, oracle_f64, 1, form)
+    # Without an explicit offset each call continues after the level's last one.
+    for ell, M, continued_from in ((0, 500, 0), (2, 250, 0), (0, 3, 500)):
+        g = nm.sample(ell, M, trace=True)
+        o = orc.sample(ell, M, keying=1, outer_offset=continued_from, trace=True)
+        if form == 1:
+            _cmp(g, o)
+        else:
+            same = (g["nf"] == o["nf"]) & (g["nc"] == o["nc"])
+            assert same.mean() > 0.99
+            assert np.array_equal(g["phi_f"][same], o["phi_f"][same])
+    nm.cleanup_nestedmc_sampler()
+    nm.set_form(nm.FORM_PLAIN)
+
+
 def test_batched_reference_model_nonantithetic(nm, oracle_f64):
     """Batched keying with the reference model and the plain (non-antithetic)
     fine/coarse estimator."""

@@ -152,3 +152,51 @@ def test_portfolio_errors(nm):
     with pytest.raises(nm.NmlmcError):
         nm.portfolio_init(pf, 1)
     nm.cleanup_nestedmc_sampler()
+
+
+@pytest.mark.parametrize("cores", [1, 3, 64])
+def test_delta_unbiased_matches_oracle(nm, oracle_f64, cores):
+    """portfolio_sample_delta_unbiased on the reference's generated portfolio:
+    consecutive calls from the persistent inner streams, then a level call on
+    the same streams.  Work exact; moments to the Unbiased tolerance."""
+    d = reference_generated()
+    scal = dict(SCAL, max_loss=0.02, subsample=1, use_delta_cv=True, use_antithetic_risk=True,
+                phi_type=0, phi_size=1)
+    pf, o = _setup(nm, oracle_f64, d, cores, 8, 4, 1.5, scal, 5)
+    for idx, M in ((0, 40), (3, 25), (0, 7), (11, 300)):
+        g_sums, g_work = nm.portfolio_sample_delta_unbiased(idx, M, 2)
+        o_sums, o_work = o.sample_delta_unbiased(idx, M, 2)
+        assert g_work == o_work
+        np.testing.assert_allclose(g_sums, o_sums, rtol=RTOL_UNBIASED, atol=1e-12)
+    _cmp(nm.sample(1, 5, trace=True), o.sample(1, 5, trace=True), rtol=RTOL_UNBIASED)
+    with pytest.raises(nm.NmlmcError):
+        nm.portfolio_sample_delta_unbiased(12, 10)  # the portfolio has 12 Unbiased options
+    nm.cleanup_nestedmc_sampler()
+
+
+def test_estimate_delta_for_unbiased(nm, oracle_f64):
+    """portfolio_estimate_delta_for_unbiased: batches of 1000 delta samples per
+    Unbiased option until 3 sqrt(V/M) < TOL, the mean added to delta_est;
+    restated on the oracle's delta sampler."""
+    d = reference_generated()
+    scal = dict(SCAL, max_loss=0.02, subsample=1, use_delta_cv=True, use_antithetic_risk=True,
+                phi_type=0, phi_size=1)
+    pf, o = _setup(nm, oracle_f64, d, 4, 8, 4, 1.5, scal, 17)
+    n_unb = int(d["counts"][2])
+    before = pf.column("delta_est")[-n_unb:].copy()
+    TOL = 2e-3
+    nm.portfolio_estimate_delta_for_unbiased(pf, TOL)
+    after = pf.column("delta_est")[-n_unb:]
+    expect = []
+    for j in range(n_unb):
+        tot, tm = np.zeros(2), 0
+        while True:
+            s, _ = o.sample_delta_unbiased(j, 1000, 2)
+            tot += s
+            tm += 1000
+            V = tot[1] / tm - (tot[0] / tm) ** 2
+            if 3 * np.sqrt(V / tm) < TOL:
+                break
+        expect.append(tot[0] / tm)
+    np.testing.assert_allclose(after - before, expect, rtol=RTOL_UNBIASED, atol=1e-12)
+    nm.cleanup_nestedmc_sampler()

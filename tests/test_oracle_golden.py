@@ -135,16 +135,42 @@ def test_portfolio_level_calls(real):
             _cmp_call(O.sample(g["ell"], g["M"]), g, exact=run["cores"] == 1)
 
 
+@pytest.mark.parametrize("real", REALS)
+def test_portfolio_delta_unbiased(real):
+    """portfolio_sample_delta_unbiased: moments and work of consecutive calls
+    (streams persist) under 1 and 3 threads, Milstein paths of the generated
+    portfolio's Unbiased options."""
+    fx = fixture(real)["portfolio"]
+    O = Oracle(real)
+    gen = fx["gen"]["data"]
+    run0 = fx["runs"][0]
+    for case in fx["delta_unbiased"]:
+        O.new(case["cores"], 8, 4, 1.5, True)
+        scal = dict(run0["scalars"], subsample=1, use_delta_cv=True, use_antithetic_risk=True,
+                    phi_type=0, phi_size=1)
+        O.portfolio_init(_portfolio(gen, scal, run0["start"], run0["delta_est"]), case["seed"])
+        for g in case["calls"]:
+            sums, work = O.sample_delta_unbiased(g["option_index"], g["M"], 2)
+            assert work == g["work"]
+            if case["cores"] == 1:
+                assert sums.tolist() == g["sums"]
+            else:
+                np.testing.assert_allclose(sums, g["sums"], rtol=1e-14, atol=1e-300)
+
+
 @pytest.mark.skipif(not ref_available(), reason="reference build (oracle/_ref) not present")
 @pytest.mark.parametrize("real", REALS)
 def test_live_against_reference_build(real):
-    """Random parameter draws, live against the reference build."""
+    """Random parameter draws, live against the reference build.  beta is 4 or 8:
+    with beta = 2 and the adaptive rule the reference evaluates uint_pow(1, l),
+    whose clog2(1) = 64 - clz(0) is undefined (the oracle and the library return
+    1 there), so its output is not a reference value."""
     O, R = Oracle(real), Ref(real)
     rs = np.random.RandomState(5)
     for _ in range(6):
         cores = int(rs.choice([1, 2, 5]))
         N0 = int(rs.choice([4, 8, 32]))
-        beta = int(rs.choice([2, 4]))
+        beta = int(rs.choice([4, 8]))
         r = float(rs.choice([0.0, 1.3, 1.7]))
         anti = bool(rs.randint(2))
         phi = int(rs.randint(2))
