@@ -12,14 +12,17 @@ the device (counter-based RNG).  value = outer samples / s of the chosen form
 reference keying, 65536 virtual threads) is timed beside it, per level, and
 both forms are driven to RMSE eps by the MLMC driver (time to eps).
 
-roofline: the dominant kernel is VALU-bound (integer Threefry + Box-Muller +
-compensated sums; HBM traffic is a few KB per launch), so achieved = algorithmic
-VALU lane-ops per launch (inner samples x ops per inner sample + outer samples
-x ops per outer sample, counts in nested-risk-mlmc_amd/opcount.json) / the
-kernel's duration from HIP events on the library's stream; peak = 256 CU x 64
-lanes x 2.4 GHz = 39.3 T lane-ops/s.
+roofline: a launch is one level call's kernel chain (k_b_zero, k_b_init, one
+k_b_test per refinement round, k_b_final_count/alloc/fill, k_b_final,
+k_b_reeval), which is VALU-bound (integer Threefry + Box-Muller + compensated
+sums; no input data, about 1 KB of queue and item-state traffic per outer
+sample), so achieved = algorithmic VALU lane-ops per launch (inner samples x ops
+per inner sample + outer samples x ops per outer sample, counts in
+nested-risk-mlmc_amd/opcount.json) / the chain's duration from HIP events
+recorded on the library's stream around it; peak = 256 CU x 64 lanes x 2.4 GHz
+= 39.3 T lane-ops/s.
 
-traffic: HBM bytes per launch of the dominant kernel from two rocprofv3 --pmc
+traffic: HBM bytes per launch (level call) from two rocprofv3 --pmc
 passes (FETCH_SIZE, WRITE_SIZE), each scaled by what the same counter reads for
 a plain device copy of known size (python bench.py --copy-calibration); the
 scaled figures are kept in profiles/hbm_traffic.json and read from there.
@@ -102,9 +105,12 @@ def roofline(recs, form):
     ops = sum(r["inner"] * oi + r["M"] * oo for r in recs)
     secs = sum(r["kernel_ms"] for r in recs) / 1e3
     achieved = ops / secs / 1e12
+    chain = "batched_level_chain_mixed" if form == 2 else None
     return {"bound": "valu", "achieved": round(achieved, 3), "peak": round(PEAK_VALU_TOPS, 1),
-            "unit": "TOP/s", "frac": round(achieved / PEAK_VALU_TOPS, 4), "traffic": None,
-            "kernel": "k_model_batched" if form else "k_model_parity",
+            "unit": "TOP/s", "frac": round(achieved / PEAK_VALU_TOPS, 4),
+            "traffic": traffic_bytes(chain) if chain else None,
+            "kernel": ("level-call kernel chain k_b_zero..k_b_reeval (mixed)" if form == 2 else
+                       "batched level-call kernel chain" if form else "k_model_parity"),
             "launches": len(recs), "avg_launch_ms": round(1e3 * secs / len(recs), 4),
             "ops_per_inner_sample": oi}
 
@@ -144,8 +150,8 @@ def cpu_baseline(M0_cpu, levels):
                       % (levels, M0_cpu, total, dt)}
 
 
-def time_to_eps(nm, form, cores, eps, M0):
-    setup(nm, form, cores, seed=7)
+def time_to_eps(nm, form, cores, eps, M0, setup_fn=None):
+    (setup_fn or setup)(nm, form, cores, seed=7)
     t0 = time.perf_counter()
     r = nm.mlmc_run(eps, M0)
     dt = time.perf_counter() - t0
@@ -156,6 +162,69 @@ def time_to_eps(nm, form, cores, eps, M0):
     nm.cleanup_nestedmc_sampler()
     return {"seconds": round(dt, 3), "estimate": Eg, "rmse_est": rms, "levels": len(r["levels"]),
             "stat_error": r["stat_error"]}
+
+
+def portfolio_configs(nm):
+    import importlib
+    return importlib.import_module(nm.__name__ + ".configs")
+
+
+def portfolio_bench(nm, which, levels, M0, rel_eps, tte, plain_cores, tte_M0):
+    """Portfolio books: per-level outer samples/s of the mixed and plain forms
+    (N_base = 64 2^l, 3-sigma doubling capped at 2^4, M_l = ceil(M0 2^-1.5l))
+    and time to RMSE eps = rel_eps * (1 - q) (q the threshold's quantile)."""
+    cfg = portfolio_configs(nm)
+    t0 = time.perf_counter()
+    a, L, sc = cfg.large_portfolio() if which == "large" else cfg.small_portfolio()
+    q = 0.995 if which == "large" else 0.99
+    gen_s = time.perf_counter() - t0
+    keep = {}
+
+    def pf_setup(nm_, form, cores, seed=1):
+        nm_.init_nestedmc_sampler(cores, 64, 2, 0.0, True)
+        nm_.set_adaptive_rule(1, 4)
+        nm_.set_form(form)
+        pf = nm_.Portfolio(a["S0"], a["mu"], a["sigma"], a["counts"], a["strike"], a["maturity"],
+                           a["w"], a["stock_ind"], a["type"], a["prob"], a["W_est"], **sc)
+        pf.comp_params()
+        nm_.portfolio_init(pf, seed)
+        nm_.portfolio_set_correlation(L)
+        keep["pf"] = pf
+
+    out = {"workload": "%s book: d=%d assets, P=%d Exact options, threshold = pilot %.1f%% "
+                       "quantile (%.4g)" % (which, len(a["S0"]), len(a["w"]), 100 * q,
+                                            sc["max_loss"]),
+           "levels": levels, "M0": M0, "generate_s": round(gen_s, 2)}
+    eps = rel_eps * (1 - q)
+    for form, cores, label in ((nm.FORM_MIXED, 1, "mixed"), (nm.FORM_PLAIN, plain_cores, "plain_fp64")):
+        pf_setup(nm, form, cores)
+        sweep(nm, M0, levels)
+        recs = []
+        t1 = time.perf_counter()
+        sweep(nm, M0, levels, recs)
+        el = time.perf_counter() - t1
+        total = sum(level_M(M0, l) for l in range(levels + 1))
+        r = {"outer_samples_per_s": round(total / el, 1), "per_level": per_level(recs, levels),
+             "kernel_ms": [round(x["kernel_ms"], 3) for x in recs],
+             "inner_samples": sum(x["inner"] for x in recs)}
+        if form == nm.FORM_MIXED:
+            r["reevaluated"] = sum(x["reev"] for x in recs)
+            r["flips"] = sum(x["flips"] for x in recs)
+        nm.cleanup_nestedmc_sampler()
+        if tte:
+            r["time_to_eps"] = time_to_eps(nm, form, cores, eps, tte_M0, setup_fn=pf_setup)
+        out[label] = r
+        print("# portfolio %s %s: %.3e outer samples/s%s" % (
+            which, label, r["outer_samples_per_s"],
+            (", time to eps %.2f s" % r["time_to_eps"]["seconds"]) if tte else ""),
+            file=sys.stderr, flush=True)
+    if tte:
+        m, p_ = out["mixed"]["time_to_eps"], out["plain_fp64"]["time_to_eps"]
+        out["eps"] = eps
+        out["speedup_time_to_eps"] = round(p_["seconds"] / m["seconds"], 2)
+        sig = math.hypot(m["stat_error"], p_["stat_error"])
+        out["estimates_within_3_sigma"] = bool(abs(m["estimate"] - p_["estimate"]) <= 3 * sig)
+    return out
 
 
 def main():
@@ -172,6 +241,11 @@ def main():
     ap.add_argument("--no-tte", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-M0", type=float, default=1e6)
+    ap.add_argument("--portfolio", choices=["large", "small", "none"], default="large")
+    ap.add_argument("--pf-levels", type=int, default=6)
+    ap.add_argument("--pf-M0", type=float, default=65536)
+    ap.add_argument("--pf-rel-eps", type=float, default=0.05)
+    ap.add_argument("--pf-tte-M0", type=int, default=1024)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (for scaling rocprofv3 byte counters)")
     args = ap.parse_args()
@@ -239,6 +313,10 @@ def main():
         extra["time_to_eps"] = {"eps": args.eps, "mixed": tm, "plain_fp64": tp,
                                 "speedup": round(tp["seconds"] / tm["seconds"], 2),
                                 "exact": exact}
+    if args.portfolio != "none":
+        extra["portfolio"] = portfolio_bench(nm, args.portfolio, args.pf_levels, args.pf_M0,
+                                             args.pf_rel_eps, not args.no_tte, args.plain_cores,
+                                             args.pf_tte_M0)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args.cpu_M0, args.levels)

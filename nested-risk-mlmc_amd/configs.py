@@ -1,0 +1,125 @@
+"""Portfolio workloads: correlated multi-asset option books and their loss
+thresholds (generators the reference does not have; its own generator is
+portfolio_gen_data).
+
+Each generator returns ``(arrays, chol, scalars)``:
+
+- ``arrays``: S0, mu, sigma (per stock), counts (per method BS / Exact /
+  Unbiased), and strike, maturity, w, stock_ind, type, prob, W_est (per option,
+  method-major, each method sorted by descending prob = |w|);
+- ``chol``: the lower Cholesky factor of the asset correlation;
+- ``scalars``: the portfolio_data scalars, including ``max_loss``, the pilot
+  quantile of the loss.
+
+The loss of an outer scenario is sum_i w_i (V_i(0) - e^{-r tau} V_i(tau)),
+the value lost over the horizon tau (portfolio_sampler.cpp:186-219), where
+V_i(tau) is the option's value at the scenario's stock price. For Exact-GBM
+inner sampling, the inner mean given the scenario is exactly that sum with
+Black-Scholes values. The pilot threshold is therefore the q-quantile of this
+closed form over ``n_pilot`` scenarios drawn as the sampler draws them:
+W = sqrt(tau) L z, R = S0 exp(sigma W + (mu - sigma^2/2) tau).
+"""
+import numpy as np
+from scipy.special import erfc
+
+SQRT2 = np.sqrt(2.0)
+
+
+def _ncdf(x):
+    return 0.5 * erfc(-x / SQRT2)
+
+
+def bs_value(typ, S, K, r, t, sigma):
+    """Black-Scholes value (calc_option_blackscholes, portfolio_sampler.cpp:140-150)
+    of calls (1), puts (-1) and cash-or-nothing digitals paying 1 (2, -2)."""
+    typ = np.asarray(typ)
+    d1 = (np.log(S / K) + (r + 0.5 * sigma * sigma) * t) / (sigma * np.sqrt(t))
+    d2 = d1 - sigma * np.sqrt(t)
+    mod = np.where(np.abs(typ) == 2, typ // 2, typ).astype(np.float64)
+    vanilla = mod * S * _ncdf(mod * d1) - mod * K * np.exp(-r * t) * _ncdf(mod * d2)
+    digital = np.exp(-r * t) * _ncdf(mod * d2)
+    return np.where(np.abs(typ) == 2, digital, vanilla)
+
+
+def _book(rs, P, d, kinds, weights):
+    """P options on d stocks: kinds = list of (type, fraction); weights 'normal'
+    or 't3'.  All options are priced by Exact-GBM inner sampling (method 1)."""
+    types = np.concatenate([np.full(int(round(f * P)), t, np.int32) for t, f in kinds])
+    assert len(types) == P
+    rs.shuffle(types)
+    strike = 100.0 * rs.uniform(0.8, 1.2, P)
+    maturity = rs.uniform(0.25, 2.0, P)
+    stock_ind = rs.randint(0, d, P).astype(np.uint32)
+    w = rs.standard_normal(P) if weights == "normal" else rs.standard_t(3, P)
+    prob = np.abs(w)
+    order = np.argsort(-prob, kind="stable")
+    return dict(strike=strike[order], maturity=maturity[order], w=w[order],
+                stock_ind=stock_ind[order], type=types[order], prob=prob[order],
+                W_est=np.ones(P), counts=np.array([0, P, 0], np.uint32))
+
+
+def conditional_loss(arrays, chol, scalars, z):
+    """Closed-form E[loss | scenario] for the scenarios z [n x d] (standard normals)."""
+    tau, r = scalars["tau"], scalars["r"]
+    S0, mu, sig = arrays["S0"], arrays["mu"], arrays["sigma"]
+    W = np.sqrt(tau) * (z @ chol.T)
+    R = S0 * np.exp(sig * W + (mu - 0.5 * sig * sig) * tau)
+    si = arrays["stock_ind"]
+    typ, K, T, w = arrays["type"], arrays["strike"], arrays["maturity"], arrays["w"]
+    v0 = bs_value(typ, S0[si], K, r, T, sig[si])
+    out = np.empty(len(z))
+    for k in range(len(z)):
+        vt = bs_value(typ, R[k, si], K, r, T - tau, sig[si])
+        out[k] = np.dot(w, v0 - np.exp(-r * tau) * vt)
+    return out
+
+
+def pilot_threshold(arrays, chol, scalars, q, n_pilot, seed):
+    z = np.random.RandomState(seed).standard_normal((n_pilot, len(arrays["S0"])))
+    return float(np.quantile(conditional_loss(arrays, chol, scalars, z), q))
+
+
+def _scalars(tau, rho):
+    return dict(r=0.05, rho=rho, tau=tau, dt_h0inv=8, dt_beta=4, dt_prob_expo=-1.5, phi_type=0,
+                phi_size=1, subsample=1, use_delta_cv=True, use_coarse_delta_cv=False,
+                use_antithetic_risk=True, use_milstein=True, max_loss=0.0)
+
+
+def small_portfolio(seed=3, n_pilot=4000, P=10_000, d=10):
+    """d = 10 correlated GBM assets (S0 = 100, sigma ~ U(0.1, 0.4), drift 0.05,
+    pairwise correlation 0.3 via its Cholesky factor), tau = 1/52; P = 10^4
+    European calls and puts (50/50), strikes 100 U(0.8, 1.2), maturities
+    U(0.25, 2), assets uniform, weights N(0, 1); Exact-GBM inner loss;
+    sub-sampling with probability proportional to |w|; threshold = pilot 99%
+    quantile of the loss."""
+    rs = np.random.RandomState(seed)
+    arrays = dict(S0=np.full(d, 100.0), mu=np.full(d, 0.05), sigma=rs.uniform(0.1, 0.4, d))
+    arrays.update(_book(rs, P, d, [(1, 0.5), (-1, 0.5)], "normal"))
+    C = 0.3 * np.ones((d, d)) + 0.7 * np.eye(d)
+    chol = np.linalg.cholesky(C)
+    sc = _scalars(1.0 / 52, 0.3)
+    sc["max_loss"] = pilot_threshold(arrays, chol, sc, 0.99, n_pilot, seed + 1)
+    return arrays, chol, sc
+
+
+def large_portfolio(seed=4, n_pilot=1000, P=100_000, d=100, factors=5):
+    """d = 100 assets with a factor-model correlation (5 factors, loadings
+    U(0, 0.5), rows capped at systematic variance 0.95, idiosyncratic part to
+    unit variance), S0 = 100, sigma ~
+    U(0.1, 0.4), drift 0.05, tau = 1/52; P = 10^5 options: 40% calls, 40% puts,
+    20% cash-or-nothing digitals, weights Student-t (nu = 3); Exact-GBM inner
+    loss; threshold = pilot 99.5% quantile of the loss."""
+    rs = np.random.RandomState(seed)
+    B = rs.uniform(0.0, 0.5, (d, factors))
+    # A row's systematic variance sum_k B_ik^2 can exceed 1 (up to 1.25): such
+    # rows are scaled to 0.95 so that the idiosyncratic variance stays positive.
+    ss = np.sum(B * B, axis=1)
+    B *= np.sqrt(np.minimum(1.0, 0.95 / ss))[:, None]
+    C = B @ B.T
+    C[np.diag_indices(d)] = 1.0
+    chol = np.linalg.cholesky(C)
+    arrays = dict(S0=np.full(d, 100.0), mu=np.full(d, 0.05), sigma=rs.uniform(0.1, 0.4, d))
+    arrays.update(_book(rs, P, d, [(1, 0.4), (-1, 0.4), (2, 0.1), (-2, 0.1)], "t3"))
+    sc = _scalars(1.0 / 52, 0.0)
+    sc["max_loss"] = pilot_threshold(arrays, chol, sc, 0.995, n_pilot, seed + 1)
+    return arrays, chol, sc

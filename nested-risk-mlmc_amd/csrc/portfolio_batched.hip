@@ -91,6 +91,8 @@ struct PortfolioPolicy {
                     b1.add(d);
                     b2.add(d * d);
                     bab.add(mag * ip);
+                    s.nu += st.nu;
+                    s.nn += st.nn;
                     continue;
                 }
                 double v = method == 0 ? inner_bs_d(pf, o, R, work) : inner_unbiased_d(pf, o, R, st, st, work);

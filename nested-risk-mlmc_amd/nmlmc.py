@@ -296,8 +296,15 @@ def mlmc_run_cb(fn, targetTOL, M0, theta=0.5, Ca=3.0, starting_level_coeff=1.5, 
 
 # ---- portfolio ----------------------------------------------------------------
 
+OPTION_DTYPE = np.dtype([("strike", "<f8"), ("maturity", "<f8"), ("start", "<f8"), ("w", "<f8"),
+                         ("delta_est", "<f8"), ("stock_ind", "<u4"), ("type", "<i4"),
+                         ("prob", "<f8"), ("W_est", "<f8")])
+STOCK_DTYPE = np.dtype([("S0", "<f8"), ("mu", "<f8"), ("sigma", "<f8")])
+
+
 class Portfolio:
-    """Owns the C arrays behind a portfolio_data (portfolio_sampler.cpp:62-92).
+    """Owns the arrays behind a portfolio_data (portfolio_sampler.cpp:62-92):
+    option records in a numpy array with the option_params layout (64 B).
 
     Options are given flat, method-major (BS, Exact, Unbiased) with ``counts``
     per method, each method's options sorted by descending ``prob``."""
@@ -308,29 +315,28 @@ class Portfolio:
 
     def __init__(self, S0, mu, sigma, counts, strike, maturity, w, stock_ind, type, prob,
                  W_est=None, start=None, delta_est=None, **scalars):
+        assert OPTION_DTYPE.itemsize == C.sizeof(OptionParams)
         P = int(np.sum(counts))
         self.counts = [int(c) for c in counts]
-        self.stocks = (StockParams * len(S0))()
-        for i in range(len(S0)):
-            self.stocks[i] = StockParams(float(S0[i]), float(mu[i]), float(sigma[i]))
-        self.ops = (OptionParams * max(P, 1))()
-        W_est = np.ones(P) if W_est is None else W_est
-        start = np.zeros(P) if start is None else start
-        delta_est = np.zeros(P) if delta_est is None else delta_est
-        for j in range(P):
-            self.ops[j] = OptionParams(float(strike[j]), float(maturity[j]), float(start[j]),
-                                       float(w[j]), float(delta_est[j]), int(stock_ind[j]),
-                                       int(type[j]), float(prob[j]), float(W_est[j]))
+        self.stocks = np.zeros(len(S0), STOCK_DTYPE)
+        self.stocks["S0"], self.stocks["mu"], self.stocks["sigma"] = S0, mu, sigma
+        self.ops = np.zeros(max(P, 1), OPTION_DTYPE)
+        o = self.ops[:P]
+        o["strike"], o["maturity"], o["w"] = strike, maturity, w
+        o["stock_ind"], o["type"], o["prob"] = stock_ind, type, prob
+        o["W_est"] = 1.0 if W_est is None else W_est
+        o["start"] = 0.0 if start is None else start
+        o["delta_est"] = 0.0 if delta_est is None else delta_est
         sc = dict(self.SCALARS)
         sc.update(scalars)
         self.scalars = sc
         self.c = PortfolioData()
-        base = C.addressof(self.ops)
+        base = self.ops.ctypes.data
         off = 0
         for m in range(3):
-            self.c.options[m] = Span(base + off * C.sizeof(OptionParams), self.counts[m])
+            self.c.options[m] = Span(base + off * OPTION_DTYPE.itemsize, self.counts[m])
             off += self.counts[m]
-        self.c.stocks = Span(C.addressof(self.stocks), len(S0))
+        self.c.stocks = Span(self.stocks.ctypes.data, len(S0))
         self.c.r, self.c.rho = sc["r"], sc["rho"]
         self.c.s1_rho2 = float(np.sqrt(1 - sc["rho"] ** 2))
         self.c.max_loss, self.c.tau = sc["max_loss"], sc["tau"]
@@ -347,8 +353,16 @@ class Portfolio:
     def phi_size(self):
         return 1 if self.scalars["phi_type"] in (0, 1) else self.scalars["phi_size"]
 
+    @property
+    def P(self):
+        return sum(self.counts)
+
     def column(self, name):
-        return np.array([getattr(self.ops[j], name) for j in range(sum(self.counts))])
+        return self.ops[name][:self.P].copy()
+
+    def set_max_loss(self, max_loss):
+        self.scalars["max_loss"] = max_loss
+        self.c.max_loss = max_loss
 
     def comp_params(self):
         """portfolio_comp_params: start = BS value, delta_est = BS delta."""

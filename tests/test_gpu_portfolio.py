@@ -200,3 +200,36 @@ def test_estimate_delta_for_unbiased(nm, oracle_f64):
         expect.append(tot[0] / tm)
     np.testing.assert_allclose(after - before, expect, rtol=RTOL_UNBIASED, atol=1e-12)
     nm.cleanup_nestedmc_sampler()
+
+
+def _configs(nm):
+    import importlib
+    return importlib.import_module(nm.__name__.rsplit(".", 1)[0] + ".configs")
+
+
+@pytest.mark.parametrize("form,which", [(0, "small"), (1, "small"), (0, "large"), (1, "large")])
+def test_config_shaped_portfolios(nm, oracle_f64, form, which):
+    """Reduced copies of the benchmark books (10 assets with pairwise
+    correlation 0.3, or 100 assets with a 5-factor correlation; Exact calls,
+    puts and digitals; t3 weights for the large one) under the capped 3-sigma
+    rule: plain form against the reference keying, batched against the batched
+    keying; per-sample results exact."""
+    cfg = _configs(nm)
+    if which == "small":
+        a, L, sc = cfg.small_portfolio(P=2000, n_pilot=300)
+        calls = ((0, 400), (1, 200), (3, 60))
+    else:
+        a, L, sc = cfg.large_portfolio(P=5000, n_pilot=200)
+        calls = ((0, 200), (2, 60))
+    pf, o = _setup(nm, oracle_f64, a, 1 if form else 64, 64, 2, 0.0, sc, 23, form=form, chol=L,
+                   rule=(1, 4))
+    for ell, M in calls:
+        if form:
+            g = nm.sample(ell, M, outer_offset=0, trace=True)
+            r = o.sample(ell, M, keying=1, outer_offset=0, trace=True)
+        else:
+            g = nm.sample(ell, M, trace=True)
+            r = o.sample(ell, M, trace=True)
+        _cmp(g, r)
+    nm.cleanup_nestedmc_sampler()
+    nm.set_form(nm.FORM_PLAIN)
