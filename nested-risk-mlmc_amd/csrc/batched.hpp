@@ -35,7 +35,7 @@ constexpr int kMaxClasses = 32;
 // Relative bound (times the mean magnitude of the inner terms) on the FP32
 // inner mean's error; 16x above the worst case of 2^-22-accurate normals and
 // compensated sums.
-constexpr double kMixedBound = 0x1p-16;
+constexpr double kMixedBound = 0x1p-16;  // model problem (see ModelPolicy::mixed_bound)
 
 struct ItemState {
     double Y;        // model outer draw
@@ -49,7 +49,7 @@ struct ItemState {
     uint32_t known;  // bit 0: fine count known, bit 1: coarse count known (atomic or)
     uint32_t left;   // unfinished final groups (atomic)
     uint32_t near;   // mixed: some final mean within the FP32 bound (atomic or)
-    uint32_t pad;
+    uint32_t redo;   // mixed: finished by the FP64 redo of the final phase
 };
 
 struct TestEntry {
@@ -92,7 +92,9 @@ struct Kahan {
     __device__ __forceinline__ double value() const { return (double)s - (double)c; }
 };
 
-// Per-lane level accumulators, flushed once per block.
+// Per-lane level accumulators, flushed once per block.  They are touched once
+// per finished segment or outer sample; the runtime-indexed arrays stay in
+// scratch memory rather than taking registers from the inner loops.
 struct LaneAcc {
     double f[kMaxPhi * kMaxMoments];
     double d[kMaxPhi * kMaxMoments];

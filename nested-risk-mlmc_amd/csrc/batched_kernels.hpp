@@ -2,8 +2,9 @@
 // policy P and on the precision of the inner arithmetic.
 //
 // Launch order of one level call (host: Engine::run_batched_level):
-//   k_b_zero, k_b_init, k_b_test(r) for r = 0..R-1, k_b_final_count,
-//   k_b_final_alloc, k_b_final_fill, k_b_final, [k_b_reeval when MIXED]
+//   k_b_zero, [P's outer-draw kernel], k_b_init, k_b_test(r) for r = 0..R-1,
+//   k_b_final_count, k_b_final_alloc, k_b_final_fill, k_b_final, and when MIXED
+//   k_b_redo_reset and k_b_final in its FP64 redo mode
 #pragma once
 #include "batched.hpp"
 
@@ -17,12 +18,15 @@ __device__ __forceinline__ uint32_t class_of(uint64_t n, uint64_t base_min) {
 
 // sample_inner_delta2 + the refinement test (nested_sampler.hpp:120-129, 252-263):
 // true when the outer sample needs more inner samples than `cur`.
+// RULE 0 is the reference's rule (a power of delta^2), RULE 1 the capped
+// 3-sigma doubling; a compile-time choice keeps pow out of the rule-1 kernels.
+template <int RULE>
 __device__ __forceinline__ bool refine(const BatchedCommon& c, const LevelCounts& lc, double start,
                                        uint64_t cur, double s1, double s2) {
     const double out_var = s2 / (double)(cur - 1) - (s1 * s1) / (double)(cur * (cur - 1));
     const double dist = start + s1 / (double)cur;
     const double delta2 = dist * dist / out_var;
-    const double new_count = (lc.rule == 1) ? lc.tmp / delta2 : pow(delta2, -c.ep.adaptive_r / 2) * lc.tmp;
+    const double new_count = (RULE == 1) ? lc.tmp / delta2 : pow(delta2, -c.ep.adaptive_r / 2) * lc.tmp;
     return !(new_count <= (double)cur);
 }
 
@@ -46,6 +50,7 @@ __global__ __launch_bounds__(256) void k_b_init(typename P::Args A) {
     for (uint64_t m = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; m < c.M; m += stride) {
         ItemState& it = c.items[m];
         P::init_item(A, m, it, acc);
+        it.redo = 0;
         uint64_t nf = 0, nc = 0;
         if (!test_f) {
             nf = c.cf.max;
@@ -68,7 +73,7 @@ __global__ __launch_bounds__(256) void k_b_init(typename P::Args A) {
 }
 
 // Test round r of every outer sample still refining its fine or coarse count.
-template <class P, bool MIXED>
+template <class P, bool MIXED, int RULE>
 __global__ __launch_bounds__(256) void k_b_test(typename P::Args A, uint32_t r) {
     const BatchedCommon& c = A.c;
     LaneAcc acc;
@@ -113,7 +118,7 @@ __global__ __launch_bounds__(256) void k_b_test(typename P::Args A, uint32_t r) 
             acc.work += P::group_work(A);
             const LevelCounts& lc = fine ? c.cf : c.cc;
             uint64_t n = cur;
-            bool more = refine(c, lc, it.start, cur, s1, s2);
+            bool more = refine<RULE>(c, lc, it.start, cur, s1, s2);
             if (more) {
                 n = cur * 2;
                 if (lc.max <= 2 * n) {
@@ -315,8 +320,51 @@ __device__ __forceinline__ SegAcc lane_group(const typename P::Args& A, uint32_t
     return s;
 }
 
-template <class P, bool MIXED>
+// The FP64 redo of an outer sample marked in the FINAL_MIXED pass: counted in
+// inner_stats[6], and in inner_stats[7] when its FP32 phi differed.
+__device__ __forceinline__ void redo_stats(const ItemState& it, double phi_f, double phi_c, LaneAcc& acc) {
+    acc.st[6] += 1;
+    acc.st[7] += (phi_f != it.phi32_f) || (phi_c != it.phi32_c);
+}
+
+// Before the FINAL_REDO pass: the marked outer samples' final entries and
+// group accumulators start again from zero (as k_b_final_fill left them).
+template <class P>
+__global__ __launch_bounds__(256) void k_b_redo_reset(typename P::Args A) {
+    const BatchedCommon& c = A.c;
+    const uint32_t n = aload_u32(&c.ctl->nreeval);
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t t0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (uint64_t w = t0; w < n; w += stride) {
+        ItemState& it = c.items[c.reeval[w]];
+        const FinalShape s = final_shape(c, it);
+        it.left = (c.ell > 0 && !c.ep.antithetic) ? 2u : s.groups;
+        it.near = 0;
+        it.phi_acc = it.s1_big = it.sab_big = 0.;
+    }
+    const uint32_t n_split = c.ctl->class_off[kMaxClasses];
+    for (uint64_t e = t0; e < n_split; e += stride) {
+        FinalEntry& fe = c.fpool[e];
+        if (c.items[fe.item].redo) {
+            fe.s1 = fe.sab = 0.;
+            fe.done = 0;
+        }
+    }
+}
+
+// Final phase.  MODE FINAL_FP64 / FINAL_MIXED: every outer sample's final draw
+// groups in FP64 / FP32 arithmetic; in FINAL_MIXED an outer sample whose mean
+// came within the FP32 bound of the Heaviside threshold (or met a payoff
+// discontinuity) is not finished but marked `redo`.  FINAL_REDO: only the
+// marked outer samples, again in FP64 from the same draws (their work and draw
+// counts were taken in the FINAL_MIXED pass), finished with the FP64 phi.
+enum { FINAL_FP64 = 0, FINAL_MIXED = 1, FINAL_REDO = 2 };
+
+template <class P, int MODE>
 __global__ __launch_bounds__(256) void k_b_final(typename P::Args A) {
+    constexpr bool MIXED = MODE == FINAL_MIXED;
+    constexpr bool REDO = MODE == FINAL_REDO;
+    constexpr bool COUNT = !REDO;  // take work and draw counts
     const BatchedCommon& c = A.c;
     LaneAcc acc;
     acc.zero();
@@ -344,18 +392,21 @@ __global__ __launch_bounds__(256) void k_b_final(typename P::Args A) {
             const FinalEntry& fe = c.fpool[whole_off + (k - total_split)];
             const uint32_t m = fe.item;
             ItemState& it = c.items[m];
+            if (REDO && !it.redo) goto next;
             const FinalShape sh = final_shape(c, it);
             double phi_f, phi_c, s1_big = 0., sab_big = 0., phi_small = 0.;
             bool near = false;
             if (plain2) {
                 const SegAcc a = lane_group<P, MIXED>(A, m, it, TAG_FINAL, 0, it.nf);
                 const SegAcc b = lane_group<P, MIXED>(A, m, it, TAG_FINAL_COARSE, 0, it.nc);
-                acc.work += a.work + b.work + 2 * P::group_work(A);
-                acc.st[4] += a.nu + b.nu;
-                acc.st[5] += a.nn + b.nn;
+                if (COUNT) {
+                    acc.work += a.work + b.work + 2 * P::group_work(A);
+                    acc.st[4] += a.nu + b.nu;
+                    acc.st[5] += a.nn + b.nn;
+                }
                 const double mf = it.start + a.s1 / (double)it.nf, mc = it.start + b.s1 / (double)it.nc;
-                near = MIXED && (fabs(mf) <= kMixedBound * (a.sab / (double)it.nf + fabs(it.start)) ||
-                                 fabs(mc) <= kMixedBound * (b.sab / (double)it.nc + fabs(it.start)));
+                near = MIXED && (fabs(mf) <= P::mixed_bound(a.sab / (double)it.nf, it.start) ||
+                                 fabs(mc) <= P::mixed_bound(b.sab / (double)it.nc, it.start));
                 outer_phi(c, it, 0., 0., phi_scalar(c.phi_type, mf, 1., 0.), phi_scalar(c.phi_type, mc, 1., 0.),
                           phi_f, phi_c);
             } else {
@@ -363,26 +414,30 @@ __global__ __launch_bounds__(256) void k_b_final(typename P::Args A) {
                 for (uint32_t g = 0; g < sh.groups; g++) {
                     const SegAcc a = lane_group<P, MIXED>(A, m, it, TAG_FINAL, (uint64_t)g * sh.n,
                                                           (uint64_t)(g + 1) * sh.n);
-                    acc.work += a.work + P::group_work(A);
-                    acc.st[4] += a.nu;
-                    acc.st[5] += a.nn;
+                    if (COUNT) {
+                        acc.work += a.work + P::group_work(A);
+                        acc.st[4] += a.nu;
+                        acc.st[5] += a.nn;
+                    }
                     const double mean = it.start + a.s1 / (double)sh.n;
-                    if (MIXED) near |= fabs(mean) <= kMixedBound * (a.sab / (double)sh.n + fabs(it.start));
+                    if (MIXED) near |= fabs(mean) <= P::mixed_bound(a.sab / (double)sh.n, it.start);
                     phi_small = phi_scalar(c.phi_type, mean, coeff, phi_small);
                     s1_big += a.s1;
                     sab_big += a.sab;
                 }
                 if (MIXED)
                     near |= fabs(it.start + s1_big / (double)sh.big) <=
-                            kMixedBound * (sab_big / (double)sh.big + fabs(it.start));
+                            P::mixed_bound(sab_big / (double)sh.big, it.start);
                 outer_phi(c, it, s1_big, phi_small, 0., 0., phi_f, phi_c);
             }
             if (MIXED && near && c.phi_type == PHI_INDICATOR) {
                 it.phi32_f = phi_f;
                 it.phi32_c = phi_c;
+                it.redo = 1;
                 reeval = true;
                 m_done = m;
             } else {
+                if (REDO) redo_stats(it, phi_f, phi_c, acc);
                 finish_outer(c, m, it, phi_f, phi_c, acc);
             }
         } else {
@@ -402,15 +457,18 @@ __global__ __launch_bounds__(256) void k_b_final(typename P::Args A) {
             FinalEntry& fe = c.fpool[c.ctl->class_off[cls] + e];
             const uint32_t m = fe.item;
             ItemState& it = c.items[m];
+            if (REDO && !it.redo) goto next;
             const uint64_t i0 = (fe.kind == 0) ? (uint64_t)fe.group * n_cls : 0;
             const uint64_t i1 = i0 + n_cls;
             const uint64_t ub = i0 / P::spu, ue = (i1 + P::spu - 1) / P::spu;
             const uint64_t u0 = ub + j * c.S, u1 = (u0 + c.S < ue) ? u0 + c.S : ue;
             SegAcc s;
             P::template units<MIXED>(A, m, it, fe.kind == 2 ? TAG_FINAL_COARSE : TAG_FINAL, u0, u1, i0, i1, s);
-            acc.work += s.work;
-            acc.st[4] += s.nu;
-            acc.st[5] += s.nn;
+            if (COUNT) {
+                acc.work += s.work;
+                acc.st[4] += s.nu;
+                acc.st[5] += s.nn;
+            }
             bool complete = spe == 1;
             double s1 = s.s1, sab = s.sab;
             if (!complete) {
@@ -424,9 +482,9 @@ __global__ __launch_bounds__(256) void k_b_final(typename P::Args A) {
                 }
             }
             if (complete) {
-                acc.work += P::group_work(A);
+                if (COUNT) acc.work += P::group_work(A);
                 const double mean = it.start + s1 / (double)n_cls;
-                const bool near = MIXED && fabs(mean) <= kMixedBound * (sab / (double)n_cls + fabs(it.start));
+                const bool near = MIXED && fabs(mean) <= P::mixed_bound(sab / (double)n_cls, it.start);
                 if (near) __hip_atomic_fetch_or(&it.near, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 if (fe.kind == 0) {
                     const FinalShape sh = final_shape(c, it);
@@ -448,63 +506,25 @@ __global__ __launch_bounds__(256) void k_b_final(typename P::Args A) {
                     if (MIXED && fe.kind == 0) {
                         const FinalShape sh = final_shape(c, it);
                         const double mb = it.start + s1b / (double)sh.big;
-                        is_near |= fabs(mb) <= kMixedBound * (aread(&it.sab_big) / (double)sh.big + fabs(it.start));
+                        is_near |= fabs(mb) <= P::mixed_bound(aread(&it.sab_big) / (double)sh.big, it.start);
                     }
                     if (MIXED && is_near && c.phi_type == PHI_INDICATOR) {
                         it.phi32_f = phi_f;
                         it.phi32_c = phi_c;
+                        it.redo = 1;
                         reeval = true;
                         m_done = m;
                     } else {
+                        if (REDO) redo_stats(it, phi_f, phi_c, acc);
                         finish_outer(c, m, it, phi_f, phi_c, acc);
                     }
                 }
             }
         }
-        const uint32_t slot = wave_append(&c.ctl->nreeval, reeval);
-        if (reeval) c.reeval[slot] = m_done;
-    }
-    block_flush(acc, &c.ctl->acc);
-}
-
-// FP64 re-evaluation of the final draws of outer samples whose FP32 mean came
-// within the bound of the Heaviside threshold: one wave per outer sample.
-template <class P>
-__global__ __launch_bounds__(256) void k_b_reeval(typename P::Args A) {
-    const BatchedCommon& c = A.c;
-    LaneAcc acc;
-    acc.zero();
-    const uint32_t lane = threadIdx.x & 63;
-    const uint32_t n = aload_u32(&c.ctl->nreeval);
-    const uint32_t waves = gridDim.x * (blockDim.x / 64);
-    for (uint32_t w = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6); w < n; w += waves) {
-        const uint32_t m = c.reeval[w];
-        const ItemState& it = c.items[m];
-        const FinalShape sh = final_shape(c, it);
-        double phi_f = 0., phi_c = 0.;
-        auto group_sum64 = [&](uint32_t tag, uint64_t i0, uint64_t i1) {
-            SegAcc s;
-            const uint64_t ub = i0 / P::spu, ue = (i1 + P::spu - 1) / P::spu;
-            for (uint64_t u = ub + lane; u < ue; u += 64) P::template units<false>(A, m, it, tag, u, u + 1, i0, i1, s);
-            return wave_sum(s.s1);
-        };
-        if (c.ell > 0 && !c.ep.antithetic) {
-            phi_f = phi_scalar(c.phi_type, it.start + group_sum64(TAG_FINAL, 0, it.nf) / (double)it.nf, 1., 0.);
-            phi_c = phi_scalar(c.phi_type, it.start + group_sum64(TAG_FINAL_COARSE, 0, it.nc) / (double)it.nc, 1., 0.);
-        } else {
-            const double coeff = (double)(1.f / (float)sh.groups);
-            double s1_big = 0., phi_small = 0.;
-            for (uint32_t g = 0; g < sh.groups; g++) {
-                const double s1 = group_sum64(TAG_FINAL, (uint64_t)g * sh.n, (uint64_t)(g + 1) * sh.n);
-                phi_small = phi_scalar(c.phi_type, it.start + s1 / (double)sh.n, coeff, phi_small);
-                s1_big += s1;
-            }
-            outer_phi(c, it, s1_big, phi_small, 0., 0., phi_f, phi_c);
-        }
-        if (lane == 0) {
-            acc.st[6] += 1;
-            acc.st[7] += (phi_f != it.phi32_f) || (phi_c != it.phi32_c);
-            finish_outer(c, m, it, phi_f, phi_c, acc);
+    next:
+        if (MIXED) {
+            const uint32_t slot = wave_append(&c.ctl->nreeval, reeval);
+            if (reeval) c.reeval[slot] = m_done;
         }
     }
     block_flush(acc, &c.ctl->acc);
@@ -514,14 +534,22 @@ __global__ __launch_bounds__(256) void k_b_reeval(typename P::Args A) {
 template <class P, bool MIXED>
 hipError_t launch_batched_level(const typename P::Args& A, uint32_t blocks, hipStream_t s) {
     hipLaunchKernelGGL(k_b_zero, dim3(1), dim3(256), 0, s, A.c.ctl);
+    hipError_t e = P::launch_outer(A, s);
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_b_init<P>), dim3(blocks), dim3(256), 0, s, A);
     for (uint32_t r = 0; r < A.c.rounds; r++)
-        hipLaunchKernelGGL((k_b_test<P, MIXED>), dim3(blocks), dim3(256), 0, s, A, r);
+        if (A.c.ep.adapt_rule == 1) hipLaunchKernelGGL((k_b_test<P, MIXED, 1>), dim3(blocks), dim3(256), 0, s, A, r);
+        else hipLaunchKernelGGL((k_b_test<P, MIXED, 0>), dim3(blocks), dim3(256), 0, s, A, r);
     hipLaunchKernelGGL((k_b_final_count<P>), dim3(blocks), dim3(256), 0, s, A);
     hipLaunchKernelGGL(k_b_final_alloc, dim3(1), dim3(64), 0, s, A.c.ctl, A.c.fcap);
     hipLaunchKernelGGL((k_b_final_fill<P>), dim3(blocks), dim3(256), 0, s, A);
-    hipLaunchKernelGGL((k_b_final<P, MIXED>), dim3(blocks), dim3(256), 0, s, A);
-    if (MIXED) hipLaunchKernelGGL((k_b_reeval<P>), dim3(blocks), dim3(256), 0, s, A);
+    if (MIXED) {
+        hipLaunchKernelGGL((k_b_final<P, FINAL_MIXED>), dim3(blocks), dim3(256), 0, s, A);
+        hipLaunchKernelGGL((k_b_redo_reset<P>), dim3(blocks), dim3(256), 0, s, A);
+        hipLaunchKernelGGL((k_b_final<P, FINAL_REDO>), dim3(blocks), dim3(256), 0, s, A);
+    } else {
+        hipLaunchKernelGGL((k_b_final<P, FINAL_FP64>), dim3(blocks), dim3(256), 0, s, A);
+    }
     return hipGetLastError();
 }
 

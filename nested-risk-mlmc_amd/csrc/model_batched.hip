@@ -23,9 +23,18 @@ struct ModelPolicy {
     // init_inner_sampler does no work for the model (model_sampler.cpp:64).
     __device__ static double group_work(const Args&) { return 0.; }
 
+    // Mixed form: an FP32 mean within this distance of the threshold is
+    // re-evaluated in FP64 (mean_mag: mean of the inner terms' magnitudes).
+    __device__ static double mixed_bound(double mean_mag, double start) {
+        return kMixedBound * (mean_mag + fabs(start));
+    }
+
     __device__ static uint64_t key1(const Args& A, uint64_t m) {
         return ((uint64_t)A.c.ell << 46) | (A.c.offset + m);
     }
+
+    // The outer draw is one normal, taken in k_b_init.
+    static hipError_t launch_outer(const Args&, hipStream_t) { return hipSuccess; }
 
     // model::sampler::sample_outer (model_sampler.cpp:57-59) and
     // init_inner_sampler (64).

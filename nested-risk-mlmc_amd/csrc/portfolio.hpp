@@ -27,6 +27,8 @@ struct StockRec {
 struct PortfolioDevice {
     const OptionRec* options[3];
     const double* cdfs[3];
+    const uint32_t* guides[3];  // guides[m][b] = max{i : cdf[i] <= b / guide_scale[m]}, b = 0..n
+    double guide_scale[3];      // n / total probability of the method
     uint32_t nopt[3];
     const StockRec* stocks;
     const double* delta_coalesce;
@@ -78,6 +80,7 @@ struct PortfolioBatchedHost {
     double bs_work;
     uint32_t blocks;
     bool mixed;
+    bool exact_only;  // sub-sampled portfolio whose options are all Exact (method 1)
 };
 hipError_t launch_portfolio_batched(const PortfolioBatchedHost& H, hipStream_t s);
 

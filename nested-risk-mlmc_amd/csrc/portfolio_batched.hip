@@ -13,69 +13,113 @@
 
 namespace nmlmc {
 
-struct PortfolioPolicy {
+// EXACT_ONLY: every option is priced by Exact-GBM inner sampling with
+// sub-sampling on (the benchmark books); the other methods' code is left out
+// of the kernels.  Draws and results are those of the general policy.
+template <bool EXACT_ONLY>
+struct PortfolioPolicyT {
     struct Args {
         BatchedCommon c;
         PortfolioDevice pf;
-        double* R_tau;  // [M x d] outer state per outer sample
+        double* R_tau;  // [d x M]: stock i of outer sample m at R_tau[i * M + m]
         double bs_work; // work of one init_inner_sampler call (sub-sampling off)
     };
     static constexpr uint64_t spu = 1;
 
     __device__ static double group_work(const Args& A) { return A.bs_work; }
 
+    // Mixed form: every FP32 inner term is continuous in its inputs (a term at
+    // a payoff discontinuity makes its segment's magnitude infinite, so that the
+    // outer sample is redone in FP64, see inner_exact_f) with an error of at
+    // most about 2^-20 of its magnitude, so the FP32 mean is within 2^-20 of the
+    // mean magnitude of the FP64 one; 2^-18 leaves a factor 4.  start is FP64.
+    __device__ static double mixed_bound(double mean_mag, double) { return 0x1p-18 * mean_mag; }
+
     __device__ static uint64_t key1(const Args& A, uint64_t m) {
         return ((uint64_t)A.c.ell << 46) | (A.c.offset + m);
     }
 
+    // Stock levels of outer sample m, column-wise over the chunk's outer samples.
+    __device__ static StridedR stocks_at_tau(const Args& A, uint64_t m) { return StridedR{A.R_tau + m, A.c.M}; }
+
+    // Outer draw (sample_outer, portfolio_sampler.cpp:293-311): the one-factor
+    // model here; with a Cholesky factor, k_pf_outer_chol has already written
+    // the stock levels.  Then the delta control variate's constant and, without
+    // sub-sampling, the pre-summed BS options (init_inner_sampler, 312-324).
     __device__ static void init_item(const Args& A, uint64_t m, ItemState& it, LaneAcc& acc) {
         const PortfolioDevice& pf = A.pf;
-        double* R = A.R_tau + m * pf.d;
-        DStream os;
-        os.init(A.c.seed, key1(A, m), 0, 0, 0., 0);
+        double* Rw = A.R_tau + m;
+        const uint64_t stride = A.c.M;
         if (pf.chol) {
-            // W = sqrt(tau) L z, one row at a time; z kept in R until replaced.
-            for (uint32_t i = 0; i < pf.d; i++) R[i] = os.randn();
-            for (int i = (int)pf.d - 1; i >= 0; i--) {
-                double a = 0.;
-                for (int j = 0; j <= i; j++) a += pf.chol[(size_t)i * pf.d + j] * R[j];
-                const StockRec st = pf.stocks[i];
-                R[i] = simulate_stock_d(st.S0, st.mu, st.sigma, pf.tau, pf.stau * a);
-            }
+            acc.st[5] += pf.d;
         } else {
+            DStream os;
+            os.init(A.c.seed, key1(A, m), 0, 0, 0., 0);
             const double dW_sys = pf.stau * os.randn();
             if (pf.d == 1) {
                 const StockRec st = pf.stocks[0];
-                R[0] = simulate_stock_d(st.S0, st.mu, st.sigma, pf.tau, dW_sys);
+                Rw[0] = simulate_stock_d(st.S0, st.mu, st.sigma, pf.tau, dW_sys);
             } else {
                 for (uint32_t i = 0; i < pf.d; i++) {
                     const double dW_ind = pf.stau * os.randn();
                     const double W = pf.rho * dW_sys + pf.s1_rho2 * dW_ind;
                     const StockRec st = pf.stocks[i];
-                    R[i] = simulate_stock_d(st.S0, st.mu, st.sigma, pf.tau, W);
+                    Rw[(uint64_t)i * stride] = simulate_stock_d(st.S0, st.mu, st.sigma, pf.tau, W);
                 }
             }
+            acc.st[5] += os.nn;
         }
+        const StridedR R = stocks_at_tau(A, m);
         double cv = 0.;
         if (pf.use_delta_cv)
             for (uint32_t i = 0; i < pf.d; i++) cv += (R[i] - pf.stocks[i].S0) * pf.delta_coalesce[i];
-        double w0 = 0.;
-        it.start = det_bs_sum(pf, R, w0) + (-pf.max_loss - cv);
+        double w0 = 0., det = 0.;
+        if (!EXACT_ONLY) det = det_bs_sum(pf, R, w0);  // sub-sampling off: pre-summed BS options
+        it.start = det + (-pf.max_loss - cv);
         it.Y = 0.;
-        acc.st[5] += os.nn;
     }
+
+    static hipError_t launch_outer(const Args& A, hipStream_t s);
 
     template <bool F32>
     __device__ static void units(const Args& A, uint32_t m, const ItemState& it, uint32_t tag, uint64_t u0,
                                  uint64_t u1, uint64_t, uint64_t, SegAcc& s) {
         const PortfolioDevice& pf = A.pf;
-        const double* R = A.R_tau + (uint64_t)m * pf.d;
+        const StridedR R = stocks_at_tau(A, m);
         const uint64_t k1 = key1(A, m);
         double a1 = 0., a2 = 0., ab = 0., work = 0.;
         Kahan b1, b2, bab;
+        bool redo64 = false;  // a term at a payoff discontinuity (mixed form)
         for (uint64_t u = u0; u < u1; u++) {
             DStream st;
             st.init(A.c.seed, k1, ((uint64_t)tag << 40) | u, 0, 0., 0);
+            if (EXACT_ONLY) {
+                uint32_t method, idx;
+                const double prob = pick_option(pf, st, st, method, idx);
+                const OptionRec o = pf.options[1][idx];
+                const uint64_t first = st.rand(), second = st.rand();
+                s.nu += st.nu;
+                s.nn += 2;
+                work += o.W_est;
+                if (F32) {
+                    float x, y, mag;
+                    bool exact64;
+                    boxmuller_f32(second, first, x, y);
+                    const float ip = (float)(1. / prob);
+                    const float d = inner_exact_f(pf, o, (float)R[o.stock_ind], x, y, mag, exact64) * ip;
+                    redo64 |= exact64;
+                    b1.add(d);
+                    b2.add(d * d);
+                    bab.add(mag * ip);
+                } else {
+                    double z0, z1, w64 = 0.;
+                    boxmuller_f64(second, first, z0, z1);
+                    const double v = inner_exact_d(pf, o, R, z0, z1, w64) / prob;
+                    a1 += v;
+                    a2 += v * v;
+                }
+                continue;
+            }
             if (F32 && pf.subsample != 0) {
                 uint32_t method, idx;
                 const double prob = pick_option(pf, st, st, method, idx);
@@ -84,15 +128,17 @@ struct PortfolioPolicy {
                     const uint64_t first = st.rand(), second = st.rand();
                     st.nn += 2;
                     float x, y, mag;
+                    bool exact64;
                     boxmuller_f32(second, first, x, y);
                     work += o.W_est;
                     const float ip = (float)(1. / prob);
-                    const float d = inner_exact_f(pf, o, (float)R[o.stock_ind], x, y, mag) * ip;
+                    const float d = inner_exact_f(pf, o, (float)R[o.stock_ind], x, y, mag, exact64) * ip;
+                    s.nu += st.nu;
+                    s.nn += st.nn;
+                    redo64 |= exact64;
                     b1.add(d);
                     b2.add(d * d);
                     bab.add(mag * ip);
-                    s.nu += st.nu;
-                    s.nn += st.nn;
                     continue;
                 }
                 double v = method == 0 ? inner_bs_d(pf, o, R, work) : inner_unbiased_d(pf, o, R, st, st, work);
@@ -111,15 +157,64 @@ struct PortfolioPolicy {
         }
         s.s1 += a1 + (F32 ? b1.value() : 0.);
         s.s2 += a2 + (F32 ? b2.value() : 0.);
-        s.sab += F32 ? ab + bab.value() : 0.;
+        s.sab += F32 ? (redo64 ? (double)INFINITY : ab + bab.value()) : 0.;
         s.work += work;
     }
 };
 
+// Correlated outer draw with a Cholesky factor L: W = sqrt(tau) L z,
+// R_i = S0_i exp(sigma_i W_i + (mu_i - sigma_i^2 / 2) tau).  A block takes 64
+// outer samples, one per lane: their d normals (Box-Muller pairs of the outer
+// stream {seed, key}, counter {b, 0} for pair b) go to LDS transposed, z[j][lane],
+// then each wave forms every fourth row of L z, summing over j in order (as the
+// one-lane restatement does) from conflict-free LDS reads and wave-uniform
+// reads of L, and writes the stock levels column-wise (coalesced).
+template <class P>
+__global__ __launch_bounds__(256) void k_pf_outer_chol(typename P::Args A) {
+    extern __shared__ double zt[];  // [d][64]
+    const PortfolioDevice& pf = A.pf;
+    const uint32_t d = pf.d;
+    const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const uint64_t m = (uint64_t)blockIdx.x * 64 + lane;
+    const bool valid = m < A.c.M;
+    const uint64_t k1 = P::key1(A, valid ? m : 0);
+    for (uint32_t b = wv; 2 * b < d; b += 4) {
+        const u64x2 blk = threefry2x64_20(b, 0, A.c.seed, k1);
+        double x, y;
+        boxmuller_f64(blk.a, blk.b, x, y);  // words 2b (element b) and 2b + 1 (element a)
+        zt[(2 * b) * 64 + lane] = x;
+        if (2 * b + 1 < d) zt[(2 * b + 1) * 64 + lane] = y;
+    }
+    __syncthreads();
+    for (uint32_t i = wv; i < d; i += 4) {
+        const double* Li = pf.chol + (size_t)i * d;
+        double a = 0.;
+        for (uint32_t j = 0; j <= i; j++) a += Li[j] * zt[j * 64 + lane];
+        if (valid) {
+            const StockRec st = pf.stocks[i];
+            A.R_tau[(uint64_t)i * A.c.M + m] = simulate_stock_d(st.S0, st.mu, st.sigma, pf.tau, pf.stau * a);
+        }
+    }
+}
+
+template <bool EXACT_ONLY>
+hipError_t PortfolioPolicyT<EXACT_ONLY>::launch_outer(const Args& A, hipStream_t s) {
+    if (!A.pf.chol || A.c.M == 0) return hipSuccess;
+    const uint32_t blocks = (uint32_t)((A.c.M + 63) / 64);
+    hipLaunchKernelGGL(k_pf_outer_chol<PortfolioPolicyT<EXACT_ONLY>>, dim3(blocks), dim3(256),
+                       sizeof(double) * 64 * A.pf.d, s, A);
+    return hipGetLastError();
+}
+
+template <bool EXACT_ONLY>
+static hipError_t launch_kind(const PortfolioBatchedHost& H, hipStream_t s) {
+    using P = PortfolioPolicyT<EXACT_ONLY>;
+    typename P::Args A{H.c, H.pf, H.R_tau, H.bs_work};
+    return H.mixed ? launch_batched_level<P, true>(A, H.blocks, s) : launch_batched_level<P, false>(A, H.blocks, s);
+}
+
 hipError_t launch_portfolio_batched(const PortfolioBatchedHost& H, hipStream_t s) {
-    PortfolioPolicy::Args A{H.c, H.pf, H.R_tau, H.bs_work};
-    return H.mixed ? launch_batched_level<PortfolioPolicy, true>(A, H.blocks, s)
-                   : launch_batched_level<PortfolioPolicy, false>(A, H.blocks, s);
+    return H.exact_only ? launch_kind<true>(H, s) : launch_kind<false>(H, s);
 }
 
 }  // namespace nmlmc

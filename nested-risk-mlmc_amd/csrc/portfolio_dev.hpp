@@ -121,7 +121,8 @@ __device__ __forceinline__ double derv_from_stock_d(const OptionRec& o, double S
 }
 
 // sample_inner_bs (453-468).
-__device__ __forceinline__ double inner_bs_d(const PortfolioDevice& pf, const OptionRec& o, const double* R_tau,
+template <class RT>
+__device__ __forceinline__ double inner_bs_d(const PortfolioDevice& pf, const OptionRec& o, RT R_tau,
                                              double& work) {
     work += o.W_est;
     const uint32_t si = o.stock_ind;
@@ -133,7 +134,8 @@ __device__ __forceinline__ double inner_bs_d(const PortfolioDevice& pf, const Op
 }
 
 // sample_inner_exact (470-519) given its two inner normals.
-__device__ __forceinline__ double inner_exact_d(const PortfolioDevice& pf, const OptionRec& o, const double* R_tau,
+template <class RT>
+__device__ __forceinline__ double inner_exact_d(const PortfolioDevice& pf, const OptionRec& o, RT R_tau,
                                                 double z0, double z1, double& work) {
     const bool anti = pf.use_antithetic_risk;
     work += o.W_est;
@@ -193,7 +195,8 @@ __device__ __forceinline__ uint64_t upow(uint64_t x, uint64_t n) {
 }
 
 // sample_inner_unbiased (521-537) + sample_inner_approx (539-604).
-__device__ __forceinline__ double inner_unbiased_d(const PortfolioDevice& pf, const OptionRec& o, const double* R_tau,
+template <class RT>
+__device__ __forceinline__ double inner_unbiased_d(const PortfolioDevice& pf, const OptionRec& o, RT R_tau,
                                                    DStream& sync, DStream& inner, double& work) {
     const double e = pf.dt_prob_expo, x = (double)pf.dt_beta;
     const uint32_t L = (uint32_t)floor(log(sync.randu()) / (log(x) * e));
@@ -275,14 +278,37 @@ __device__ __forceinline__ uint32_t pick_method(const PortfolioDevice& pf, doubl
     return ret;
 }
 
-// rand_int_cdf (common.hpp:160-176): fixed-step binary search.
-__device__ __forceinline__ uint32_t pick_cdf(const double* __restrict__ cdf, uint32_t n, double u) {
-    uint32_t lo = 0, hi = n;
-    if (hi == 1) return 0;
-    const uint32_t steps = 32 - __builtin_clz(hi - 1);
-    for (uint32_t i = 0; i < steps; i++) {
-        const uint32_t mid = (lo + hi) / 2;
-        if (mid > 0 && u < cdf[mid]) hi = mid;
+// Stock levels at tau of one outer sample stored column-wise: stock i of outer
+// sample m at p[i * stride] with p = base + m (the batched forms' layout).
+struct StridedR {
+    const double* p;
+    uint64_t stride;
+    __device__ __forceinline__ double operator[](uint32_t i) const { return p[(uint64_t)i * stride]; }
+};
+
+// rand_int_cdf (common.hpp:160-176).  The reference's fixed-step binary search
+// over the exclusive-prefix CDF returns max{i : cdf[i] <= u} (its predicate
+// u < cdf[mid] is monotone in mid, and cdf[0] = 0 is never probed).  The same
+// index is found here from a guide table: bucket b = floor(u * n / total)
+// bounds the answer by guide[b] and guide[b + 1]; both bounds are checked
+// against the CDF (falling back to the whole range if either fails), and a
+// binary search closes the usually one- or two-entry gap.  About 4 dependent
+// loads instead of ceil(log2 n).
+__device__ __forceinline__ uint32_t pick_cdf(const PortfolioDevice& pf, uint32_t m, double u) {
+    const uint32_t n = pf.nopt[m];
+    if (n == 1) return 0;
+    const double* __restrict__ cdf = pf.cdfs[m];
+    const uint32_t* __restrict__ guide = pf.guides[m];
+    const double fb = u * pf.guide_scale[m];
+    const uint32_t b = fb < (double)(n - 1) ? (uint32_t)fb : n - 1;
+    uint32_t lo = guide[b];
+    uint32_t hi = guide[b + 1] + 1;
+    if (hi > n) hi = n;
+    if (lo > 0 && !(cdf[lo] <= u)) lo = 0;
+    if (hi < n && !(u < cdf[hi])) hi = n;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (u < cdf[mid]) hi = mid;
         else lo = mid;
     }
     return lo;
@@ -300,13 +326,14 @@ __device__ __forceinline__ double pick_option(const PortfolioDevice& pf, DStream
         return 1. / pf.options_count;
     }
     const double u = inner.randu() * pf.compute_prob[method];
-    idx = pick_cdf(pf.cdfs[method], pf.nopt[method], u);
+    idx = pick_cdf(pf, method, u);
     return pf.options[method][idx].prob / pf.prob_norm;
 }
 
 // One inner sample of portfolio::sampler::sample_inner (330-334) with the
 // reference's draw order.
-__device__ __forceinline__ double inner_sample_d(const PortfolioDevice& pf, const double* R_tau, DStream& sync,
+template <class RT>
+__device__ __forceinline__ double inner_sample_d(const PortfolioDevice& pf, RT R_tau, DStream& sync,
                                                  DStream& inner, double& work) {
     if (pf.subsample == 0) {
         // sample_inner_all_options (336-344).
@@ -338,7 +365,8 @@ __device__ __forceinline__ double inner_sample_d(const PortfolioDevice& pf, cons
 }
 
 // init_inner_sampler (312-324): the pre-summed BS options when not sub-sampling.
-__device__ __forceinline__ double det_bs_sum(const PortfolioDevice& pf, const double* R_tau, double& work) {
+template <class RT>
+__device__ __forceinline__ double det_bs_sum(const PortfolioDevice& pf, RT R_tau, double& work) {
     double det = 0.;
     if (pf.subsample == 0)
         for (uint32_t j = 0; j < pf.nopt[0]; j++) det += inner_bs_d(pf, pf.options[0][j], R_tau, work);
@@ -360,7 +388,7 @@ __device__ __forceinline__ float payoff_f(int type, float S, float K) {
 // sample_inner_exact in single precision; mag receives the sum of the absolute
 // values of the terms (the scale of the rounding error).
 __device__ __forceinline__ float inner_exact_f(const PortfolioDevice& pf, const OptionRec& o, float R, float z0, float z1,
-                                               float& mag) {
+                                               float& mag, bool& exact64) {
     const bool anti = pf.use_antithetic_risk;
     const StockRec st = pf.stocks[o.stock_ind];
     const float r = (float)pf.r, tau = (float)pf.tau, sig = (float)st.sigma, S0 = (float)st.S0;
@@ -377,18 +405,25 @@ __device__ __forceinline__ float inner_exact_f(const PortfolioDevice& pf, const 
     const float val1 = disc * (payoff_f(o.type, S1, K) - pt);
     const float val2 = anti ? disc * (payoff_f(o.type, S2, K) - pt) : val1;
     float dv1 = 0.f, dv2 = 0.f;
-    if (pf.use_delta_cv && !is_digital(o.type)) {
+    const bool cv = pf.use_delta_cv && !is_digital(o.type);
+    if (cv) {
         const float diff = R - S0;
         const float c = (float)(o.type == 1);
         dv1 = diff * disc * (S1 / S0) * (c - (float)(S1 < K));
         dv2 = anti ? diff * disc * (S2 / S0) * (c - (float)(S2 < K)) : dv1;
     }
-    // Scale of the single-precision rounding: the terms themselves plus the
-    // stock levels the payoffs are taken at; a digital whose path ends within
-    // 1e-4 K of the strike could flip its payoff, so it contributes the jump.
+    // Away from the strike every quantity above is a continuous function of
+    // the draws and of R: the stock levels carry a relative error of about
+    // 2^-20.5 at most (the FP32 normals' error times sigma sqrt(T), plus a few
+    // units of 2^-24 from the exponentials of O(1) arguments), so the term's
+    // error is at most about 2^-20 times mag.  A digital payoff, or the control
+    // variate's indicator S < K, whose stock level lies within 2^-17 K of the
+    // strike could jump under that rounding: exact64 then asks for the outer
+    // sample to be evaluated in FP64.
     mag = 0.5f * (fabsf(val1) + fabsf(val2) + fabsf(dv1) + fabsf(dv2)) + fabsf(disc) * (S1 + S2 + St);
-    if (is_digital(o.type) && (fabsf(S1 - K) < 1e-4f * K || fabsf(S2 - K) < 1e-4f * K || fabsf(St - K) < 1e-4f * K))
-        mag += fabsf(disc) * 1e4f;
+    const float win = 0x1p-17f * K;
+    const bool near1 = fabsf(S1 - K) < win || fabsf(S2 - K) < win;
+    exact64 = (is_digital(o.type) && (near1 || fabsf(St - K) < win)) || (cv && near1);
     return 0.5f * (val1 + val2 + dv1 + dv2);
 }
 

@@ -115,6 +115,19 @@ int Engine::stage_portfolio(const portfolio_data* data) {
         int rc = upload(*this, (const OptionRec*)ops, n, &P.options[m]);
         if (rc) return rc;
         if ((rc = upload(*this, cdf.data(), n, &P.cdfs[m]))) return rc;
+        // Guide table of the option search: guide[b] = max{i : cdf[i] <= b c / n}.
+        if (n) {
+            std::vector<uint32_t> guide(n + 1);
+            const double scale = (double)n / c;
+            size_t i = 0;
+            for (size_t b = 0; b <= n; b++) {
+                const double ub = (double)b / scale;
+                while (i + 1 < n && cdf[i + 1] <= ub) i++;
+                guide[b] = (uint32_t)i;
+            }
+            if ((rc = upload(*this, guide.data(), n + 1, &P.guides[m]))) return rc;
+            P.guide_scale[m] = scale;
+        }
         P.nopt[m] = (uint32_t)n;
         P.options_count += (uint32_t)n;
     }
@@ -198,6 +211,8 @@ int Engine::sample_portfolio_batched(uint32_t ell, uint64_t M, uint64_t outer_of
                                      uint64_t* tr_nc) {
     if (pf.phi_type != PHI_INDICATOR && pf.phi_type != PHI_MAX)
         return set_error(NMLMC_ERR_ARG, "batched forms support the Indicator and Max phi types");
+    if (pf.chol && pf.d > 256)
+        return set_error(NMLMC_ERR_ARG, "batched forms take a correlation factor of at most 256 stocks");
     PortfolioBatchedHost H;
     int rc = batched_common(ell, M, outer_offset, H.c);
     if (rc) return rc;
@@ -222,6 +237,7 @@ int Engine::sample_portfolio_batched(uint32_t ell, uint64_t M, uint64_t outer_of
     H.bs_work = pf_bs_work;
     H.blocks = batched_blocks();
     H.mixed = form == NMLMC_FORM_MIXED;
+    H.exact_only = pf.subsample != 0 && pf.nopt[0] == 0 && pf.nopt[2] == 0;
     last_kernel = "k_b_final";
     return run_batched(ell, M, H.c, out, [&]() { return launch_portfolio_batched(H, stream); }, tr_phi_f,
                        tr_phi_c, tr_nf, tr_nc);

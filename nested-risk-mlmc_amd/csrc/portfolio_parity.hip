@@ -71,7 +71,7 @@ __device__ InnerSums parity_inner(const PortfolioParityArgs& A, const double* R,
                     idx = (uint32_t)(ceil(u * (double)pf.nopt[method]) - 1);
                     prob = 1. / pf.options_count;
                 } else {
-                    idx = pick_cdf(pf.cdfs[method], pf.nopt[method], u * pf.compute_prob[method]);
+                    idx = pick_cdf(pf, method, u * pf.compute_prob[method]);
                     prob = pf.options[method][idx].prob / pf.prob_norm;
                 }
                 const OptionRec o = pf.options[method][idx];

@@ -233,3 +233,28 @@ def test_config_shaped_portfolios(nm, oracle_f64, form, which):
         _cmp(g, r)
     nm.cleanup_nestedmc_sampler()
     nm.set_form(nm.FORM_PLAIN)
+
+
+@pytest.mark.parametrize("form", [0, 1])
+def test_option_search_with_ties_and_tiny_probabilities(nm, oracle_f64, form):
+    """Sub-sampling probabilities spanning 16 decades, so that many exclusive-
+    prefix CDF entries are equal in FP64 (ties), plus a block of equal
+    probabilities: the option drawn for every inner sample matches the
+    reference search (per-sample results exact)."""
+    d = synthetic(Q=4, counts=(0, 300, 0), digitals=True, seed=31)
+    rs = np.random.RandomState(5)
+    p = np.concatenate([10.0 ** rs.uniform(-16, 0, 200), np.full(100, 1e-3)])
+    order = np.argsort(-p, kind="stable")
+    d["prob"] = p[order]
+    scal = dict(SCAL, max_loss=0.01, subsample=1, use_delta_cv=True, use_antithetic_risk=True,
+                phi_type=1, phi_size=1)
+    pf, o = _setup(nm, oracle_f64, d, 1 if form else 16, 16, 2, 0.0, scal, 41, form=form,
+                   rule=(1, 3))
+    for ell, M in ((0, 300), (2, 80)):
+        if form:
+            _cmp(nm.sample(ell, M, outer_offset=0, trace=True),
+                 o.sample(ell, M, keying=1, outer_offset=0, trace=True))
+        else:
+            _cmp(nm.sample(ell, M, trace=True), o.sample(ell, M, trace=True))
+    nm.cleanup_nestedmc_sampler()
+    nm.set_form(nm.FORM_PLAIN)
