@@ -35,7 +35,10 @@ constexpr int kMaxClasses = 32;
 // Relative bound (times the mean magnitude of the inner terms) on the FP32
 // inner mean's error; 16x above the worst case of 2^-22-accurate normals and
 // compensated sums.
-constexpr double kMixedBound = 0x1p-16;  // model problem (see ModelPolicy::mixed_bound)
+constexpr double kMixedBound = 0x1p-16;
+// FP64 redo of the mixed form: the first kRedoSlots listed outer samples with at
+// most kRedoGroups final groups are split over kRedoParts blocks each.
+constexpr uint32_t kRedoSlots = 65536, kRedoGroups = 64, kRedoParts = 32;  // model problem (see ModelPolicy::mixed_bound)
 
 struct ItemState {
     double Y;        // model outer draw
@@ -49,7 +52,7 @@ struct ItemState {
     uint32_t known;  // bit 0: fine count known, bit 1: coarse count known (atomic or)
     uint32_t left;   // unfinished final groups (atomic)
     uint32_t near;   // mixed: some final mean within the FP32 bound (atomic or)
-    uint32_t redo;   // mixed: finished by the FP64 redo of the final phase
+    uint32_t pad;
 };
 
 struct TestEntry {
@@ -199,6 +202,8 @@ struct BatchedCommon {
     FinalEntry* fpool;
     uint32_t fcap;
     uint32_t* reeval;
+    double* redo_acc;    // [kRedoSlots x kRedoGroups] FP64 group sums of the redo
+    uint32_t* redo_cnt;  // [kRedoSlots] parts of the redo done
     BatchedCtl* ctl;
     double* tr_phi_f;
     double* tr_phi_c;

@@ -3,8 +3,8 @@
 //
 // Launch order of one level call (host: Engine::run_batched_level):
 //   k_b_zero, [P's outer-draw kernel], k_b_init, k_b_test(r) for r = 0..R-1,
-//   k_b_final_count, k_b_final_alloc, k_b_final_fill, k_b_final, and when MIXED
-//   k_b_redo_reset and k_b_final in its FP64 redo mode
+//   k_b_final_count, k_b_final_alloc, k_b_final_fill, k_b_final, [k_b_redo_split
+//   and k_b_redo_seq when MIXED]
 #pragma once
 #include "batched.hpp"
 
@@ -50,7 +50,6 @@ __global__ __launch_bounds__(256) void k_b_init(typename P::Args A) {
     for (uint64_t m = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; m < c.M; m += stride) {
         ItemState& it = c.items[m];
         P::init_item(A, m, it, acc);
-        it.redo = 0;
         uint64_t nf = 0, nc = 0;
         if (!test_f) {
             nf = c.cf.max;
@@ -320,51 +319,138 @@ __device__ __forceinline__ SegAcc lane_group(const typename P::Args& A, uint32_t
     return s;
 }
 
-// The FP64 redo of an outer sample marked in the FINAL_MIXED pass: counted in
-// inner_stats[6], and in inner_stats[7] when its FP32 phi differed.
-__device__ __forceinline__ void redo_stats(const ItemState& it, double phi_f, double phi_c, LaneAcc& acc) {
-    acc.st[6] += 1;
-    acc.st[7] += (phi_f != it.phi32_f) || (phi_c != it.phi32_c);
-}
-
-// Before the FINAL_REDO pass: the marked outer samples' final entries and
-// group accumulators start again from zero (as k_b_final_fill left them).
-template <class P>
-__global__ __launch_bounds__(256) void k_b_redo_reset(typename P::Args A) {
-    const BatchedCommon& c = A.c;
-    const uint32_t n = aload_u32(&c.ctl->nreeval);
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    const uint64_t t0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    for (uint64_t w = t0; w < n; w += stride) {
-        ItemState& it = c.items[c.reeval[w]];
-        const FinalShape s = final_shape(c, it);
-        it.left = (c.ell > 0 && !c.ep.antithetic) ? 2u : s.groups;
-        it.near = 0;
-        it.phi_acc = it.s1_big = it.sab_big = 0.;
+// Finishing an outer sample of the FP64 redo (rare): its moments, trace and the
+// redo counts (inner_stats[6], and [7] when the FP32 phi differed) go straight
+// to the level accumulator with atomics.
+__device__ __forceinline__ void redo_finish(const BatchedCommon& c, uint64_t m, const ItemState& it, double phi_f,
+                                         double phi_c) {
+    Partial* out = &c.ctl->acc;
+    double p = 1.;
+    for (uint32_t j = 0; j < c.ep.fine_moments; j++) {
+        p *= phi_f;
+        atomicAdd(&out->f[j], p);
     }
-    const uint32_t n_split = c.ctl->class_off[kMaxClasses];
-    for (uint64_t e = t0; e < n_split; e += stride) {
-        FinalEntry& fe = c.fpool[e];
-        if (c.items[fe.item].redo) {
-            fe.s1 = fe.sab = 0.;
-            fe.done = 0;
+    if (c.ell > 0) {
+        const double dphi = phi_f - phi_c;
+        p = 1.;
+        for (uint32_t j = 0; j < c.ep.diff_moments; j++) {
+            p *= dphi;
+            atomicAdd(&out->d[j], p);
         }
     }
+    atomicAdd((unsigned long long*)&out->stats[6], 1ull);
+    if (phi_f != it.phi32_f || phi_c != it.phi32_c) atomicAdd((unsigned long long*)&out->stats[7], 1ull);
+    if (c.tr_phi_f) c.tr_phi_f[m] = phi_f;
+    if (c.tr_phi_c) c.tr_phi_c[m] = c.ell == 0 ? 0. : phi_c;
+    if (c.tr_nf) c.tr_nf[m] = it.nf;
+    if (c.tr_nc) c.tr_nc[m] = c.ell == 0 ? 0 : it.nc;
 }
 
-// Final phase.  MODE FINAL_FP64 / FINAL_MIXED: every outer sample's final draw
-// groups in FP64 / FP32 arithmetic; in FINAL_MIXED an outer sample whose mean
-// came within the FP32 bound of the Heaviside threshold (or met a payoff
-// discontinuity) is not finished but marked `redo`.  FINAL_REDO: only the
-// marked outer samples, again in FP64 from the same draws (their work and draw
-// counts were taken in the FINAL_MIXED pass), finished with the FP64 phi.
-enum { FINAL_FP64 = 0, FINAL_MIXED = 1, FINAL_REDO = 2 };
+// FP64 redo of the outer samples listed by the mixed final phase, from the same
+// draws (their work and draw counts were taken in the FP32 pass).  An outer
+// sample with a redo slot (the first kRedoSlots listed, at most kRedoGroups
+// final groups, antithetic or level 0) is split by k_b_redo_split over the
+// kRedoParts blocks of its grid column: lanes add their groups' FP64 sums into
+// the slot, and the block completing the slot's count forms phi.  The others
+// are summed by one block each in k_b_redo_seq, group after group.
+__device__ __forceinline__ bool redo_split(const BatchedCommon& c, uint32_t w, const FinalShape& sh, uint64_t spu) {
+    return w < kRedoSlots && !(c.ell > 0 && !c.ep.antithetic) && sh.groups <= kRedoGroups && sh.n % spu == 0;
+}
 
-template <class P, int MODE>
+template <class P>
+__global__ __launch_bounds__(256) void k_b_redo_split(typename P::Args A) {
+    const BatchedCommon& c = A.c;
+    __shared__ uint32_t last;
+    const uint32_t n = aload_u32(&c.ctl->nreeval);
+    const uint32_t part = blockIdx.y, parts = gridDim.y;
+    for (uint32_t w = blockIdx.x; w < n; w += gridDim.x) {
+        const uint32_t m = c.reeval[w];
+        const ItemState& it = c.items[m];
+        const FinalShape sh = final_shape(c, it);
+        if (!redo_split(c, w, sh, P::spu)) continue;
+        const uint64_t gu = sh.n / P::spu;  // units per group
+        const uint64_t U = (uint64_t)sh.groups * gu;
+        const uint64_t p0 = U * part / parts, p1 = U * (part + 1) / parts;
+        double* slot = c.redo_acc + (uint64_t)w * kRedoGroups;
+        double cur = 0.;
+        uint32_t cur_g = ~0u;
+        for (uint64_t k = p0 + threadIdx.x; k < p1; k += blockDim.x) {
+            const uint32_t g = (uint32_t)(k / gu);
+            if (g != cur_g) {
+                if (cur_g != ~0u) aadd(&slot[cur_g], cur);
+                cur = 0.;
+                cur_g = g;
+            }
+            SegAcc s;
+            const uint64_t i0 = (uint64_t)g * sh.n;
+            P::template units<false>(A, m, it, TAG_FINAL, k, k + 1, i0, i0 + sh.n, s);
+            cur += s.s1;
+        }
+        if (cur_g != ~0u) aadd(&slot[cur_g], cur);
+        wait_acks();
+        __syncthreads();
+        if (threadIdx.x == 0) last = acount(&c.redo_cnt[w], 1u) == parts - 1;
+        __syncthreads();
+        if (!last || threadIdx.x != 0) continue;
+        const double coeff = (double)(1.f / (float)sh.groups);
+        double s1_big = 0., phi_small = 0.;
+        for (uint32_t g = 0; g < sh.groups; g++) {
+            const double s1 = aread(&slot[g]);
+            phi_small = phi_scalar(c.phi_type, it.start + s1 / (double)sh.n, coeff, phi_small);
+            s1_big += s1;
+        }
+        double phi_f, phi_c;
+        outer_phi(c, it, s1_big, phi_small, 0., 0., phi_f, phi_c);
+        redo_finish(c, m, it, phi_f, phi_c);
+    }
+}
+
+template <class P>
+__global__ __launch_bounds__(256) void k_b_redo_seq(typename P::Args A) {
+    const BatchedCommon& c = A.c;
+    __shared__ double red[4];
+    const uint32_t n = aload_u32(&c.ctl->nreeval);
+    for (uint32_t w = blockIdx.x; w < n; w += gridDim.x) {
+        const uint32_t m = c.reeval[w];
+        const ItemState& it = c.items[m];
+        const FinalShape sh = final_shape(c, it);
+        if (redo_split(c, w, sh, P::spu)) continue;
+        auto group_sum64 = [&](uint32_t tag, uint64_t i0, uint64_t i1) {
+            SegAcc s;
+            const uint64_t ub = i0 / P::spu, ue = (i1 + P::spu - 1) / P::spu;
+            for (uint64_t u = ub + threadIdx.x; u < ue; u += blockDim.x)
+                P::template units<false>(A, m, it, tag, u, u + 1, i0, i1, s);
+            const double v = wave_sum(s.s1);
+            if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+            __syncthreads();
+            const double t = ((red[0] + red[1]) + red[2]) + red[3];
+            __syncthreads();
+            return t;
+        };
+        double phi_f = 0., phi_c = 0.;
+        if (c.ell > 0 && !c.ep.antithetic) {
+            phi_f = phi_scalar(c.phi_type, it.start + group_sum64(TAG_FINAL, 0, it.nf) / (double)it.nf, 1., 0.);
+            phi_c = phi_scalar(c.phi_type, it.start + group_sum64(TAG_FINAL_COARSE, 0, it.nc) / (double)it.nc, 1., 0.);
+        } else {
+            const double coeff = (double)(1.f / (float)sh.groups);
+            double s1_big = 0., phi_small = 0.;
+            for (uint32_t g = 0; g < sh.groups; g++) {
+                const double s1 = group_sum64(TAG_FINAL, (uint64_t)g * sh.n, (uint64_t)(g + 1) * sh.n);
+                phi_small = phi_scalar(c.phi_type, it.start + s1 / (double)sh.n, coeff, phi_small);
+                s1_big += s1;
+            }
+            outer_phi(c, it, s1_big, phi_small, 0., 0., phi_f, phi_c);
+        }
+        if (threadIdx.x == 0) redo_finish(c, m, it, phi_f, phi_c);
+    }
+}
+
+// Final phase: every outer sample's final draw groups, in FP64 or (MIXED) FP32
+// arithmetic.  In the mixed form an outer sample whose mean came within the
+// FP32 bound of the Heaviside threshold (or met a payoff discontinuity) is not
+// finished here but listed for the FP64 redo.
+template <class P, bool MIXED>
 __global__ __launch_bounds__(256) void k_b_final(typename P::Args A) {
-    constexpr bool MIXED = MODE == FINAL_MIXED;
-    constexpr bool REDO = MODE == FINAL_REDO;
-    constexpr bool COUNT = !REDO;  // take work and draw counts
     const BatchedCommon& c = A.c;
     LaneAcc acc;
     acc.zero();
@@ -392,18 +478,15 @@ __global__ __launch_bounds__(256) void k_b_final(typename P::Args A) {
             const FinalEntry& fe = c.fpool[whole_off + (k - total_split)];
             const uint32_t m = fe.item;
             ItemState& it = c.items[m];
-            if (REDO && !it.redo) goto next;
             const FinalShape sh = final_shape(c, it);
             double phi_f, phi_c, s1_big = 0., sab_big = 0., phi_small = 0.;
             bool near = false;
             if (plain2) {
                 const SegAcc a = lane_group<P, MIXED>(A, m, it, TAG_FINAL, 0, it.nf);
                 const SegAcc b = lane_group<P, MIXED>(A, m, it, TAG_FINAL_COARSE, 0, it.nc);
-                if (COUNT) {
-                    acc.work += a.work + b.work + 2 * P::group_work(A);
-                    acc.st[4] += a.nu + b.nu;
-                    acc.st[5] += a.nn + b.nn;
-                }
+                acc.work += a.work + b.work + 2 * P::group_work(A);
+                acc.st[4] += a.nu + b.nu;
+                acc.st[5] += a.nn + b.nn;
                 const double mf = it.start + a.s1 / (double)it.nf, mc = it.start + b.s1 / (double)it.nc;
                 near = MIXED && (fabs(mf) <= P::mixed_bound(a.sab / (double)it.nf, it.start) ||
                                  fabs(mc) <= P::mixed_bound(b.sab / (double)it.nc, it.start));
@@ -414,11 +497,9 @@ __global__ __launch_bounds__(256) void k_b_final(typename P::Args A) {
                 for (uint32_t g = 0; g < sh.groups; g++) {
                     const SegAcc a = lane_group<P, MIXED>(A, m, it, TAG_FINAL, (uint64_t)g * sh.n,
                                                           (uint64_t)(g + 1) * sh.n);
-                    if (COUNT) {
-                        acc.work += a.work + P::group_work(A);
-                        acc.st[4] += a.nu;
-                        acc.st[5] += a.nn;
-                    }
+                    acc.work += a.work + P::group_work(A);
+                    acc.st[4] += a.nu;
+                    acc.st[5] += a.nn;
                     const double mean = it.start + a.s1 / (double)sh.n;
                     if (MIXED) near |= fabs(mean) <= P::mixed_bound(a.sab / (double)sh.n, it.start);
                     phi_small = phi_scalar(c.phi_type, mean, coeff, phi_small);
@@ -433,11 +514,9 @@ __global__ __launch_bounds__(256) void k_b_final(typename P::Args A) {
             if (MIXED && near && c.phi_type == PHI_INDICATOR) {
                 it.phi32_f = phi_f;
                 it.phi32_c = phi_c;
-                it.redo = 1;
                 reeval = true;
                 m_done = m;
             } else {
-                if (REDO) redo_stats(it, phi_f, phi_c, acc);
                 finish_outer(c, m, it, phi_f, phi_c, acc);
             }
         } else {
@@ -457,18 +536,15 @@ __global__ __launch_bounds__(256) void k_b_final(typename P::Args A) {
             FinalEntry& fe = c.fpool[c.ctl->class_off[cls] + e];
             const uint32_t m = fe.item;
             ItemState& it = c.items[m];
-            if (REDO && !it.redo) goto next;
             const uint64_t i0 = (fe.kind == 0) ? (uint64_t)fe.group * n_cls : 0;
             const uint64_t i1 = i0 + n_cls;
             const uint64_t ub = i0 / P::spu, ue = (i1 + P::spu - 1) / P::spu;
             const uint64_t u0 = ub + j * c.S, u1 = (u0 + c.S < ue) ? u0 + c.S : ue;
             SegAcc s;
             P::template units<MIXED>(A, m, it, fe.kind == 2 ? TAG_FINAL_COARSE : TAG_FINAL, u0, u1, i0, i1, s);
-            if (COUNT) {
-                acc.work += s.work;
-                acc.st[4] += s.nu;
-                acc.st[5] += s.nn;
-            }
+            acc.work += s.work;
+            acc.st[4] += s.nu;
+            acc.st[5] += s.nn;
             bool complete = spe == 1;
             double s1 = s.s1, sab = s.sab;
             if (!complete) {
@@ -482,7 +558,7 @@ __global__ __launch_bounds__(256) void k_b_final(typename P::Args A) {
                 }
             }
             if (complete) {
-                if (COUNT) acc.work += P::group_work(A);
+                acc.work += P::group_work(A);
                 const double mean = it.start + s1 / (double)n_cls;
                 const bool near = MIXED && fabs(mean) <= P::mixed_bound(sab / (double)n_cls, it.start);
                 if (near) __hip_atomic_fetch_or(&it.near, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -511,20 +587,23 @@ __global__ __launch_bounds__(256) void k_b_final(typename P::Args A) {
                     if (MIXED && is_near && c.phi_type == PHI_INDICATOR) {
                         it.phi32_f = phi_f;
                         it.phi32_c = phi_c;
-                        it.redo = 1;
                         reeval = true;
                         m_done = m;
                     } else {
-                        if (REDO) redo_stats(it, phi_f, phi_c, acc);
                         finish_outer(c, m, it, phi_f, phi_c, acc);
                     }
                 }
             }
         }
-    next:
         if (MIXED) {
             const uint32_t slot = wave_append(&c.ctl->nreeval, reeval);
-            if (reeval) c.reeval[slot] = m_done;
+            if (reeval) {
+                c.reeval[slot] = m_done;
+                if (slot < kRedoSlots) {
+                    for (uint32_t g = 0; g < kRedoGroups; g++) astore(&c.redo_acc[(uint64_t)slot * kRedoGroups + g], 0.);
+                    __hip_atomic_exchange(&c.redo_cnt[slot], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+            }
         }
     }
     block_flush(acc, &c.ctl->acc);
@@ -543,12 +622,10 @@ hipError_t launch_batched_level(const typename P::Args& A, uint32_t blocks, hipS
     hipLaunchKernelGGL((k_b_final_count<P>), dim3(blocks), dim3(256), 0, s, A);
     hipLaunchKernelGGL(k_b_final_alloc, dim3(1), dim3(64), 0, s, A.c.ctl, A.c.fcap);
     hipLaunchKernelGGL((k_b_final_fill<P>), dim3(blocks), dim3(256), 0, s, A);
+    hipLaunchKernelGGL((k_b_final<P, MIXED>), dim3(blocks), dim3(256), 0, s, A);
     if (MIXED) {
-        hipLaunchKernelGGL((k_b_final<P, FINAL_MIXED>), dim3(blocks), dim3(256), 0, s, A);
-        hipLaunchKernelGGL((k_b_redo_reset<P>), dim3(blocks), dim3(256), 0, s, A);
-        hipLaunchKernelGGL((k_b_final<P, FINAL_REDO>), dim3(blocks), dim3(256), 0, s, A);
-    } else {
-        hipLaunchKernelGGL((k_b_final<P, FINAL_FP64>), dim3(blocks), dim3(256), 0, s, A);
+        hipLaunchKernelGGL((k_b_redo_split<P>), dim3(256, kRedoParts), dim3(256), 0, s, A);
+        hipLaunchKernelGGL((k_b_redo_seq<P>), dim3(256), dim3(256), 0, s, A);
     }
     return hipGetLastError();
 }

@@ -108,6 +108,26 @@ void Engine::free_samplers() {
     ws_cap = 0;
 }
 
+// Buffers of the batched forms (kept between level calls, released here).
+void Engine::free_batched() {
+    for (void* p : {(void*)bat_items, (void*)bat_tf[0], (void*)bat_tf[1], (void*)bat_tc[0], (void*)bat_tc[1],
+                    (void*)bat_reeval, (void*)bat_fpool, (void*)bat_redo_acc, (void*)bat_redo_cnt, (void*)bat_ctl,
+                    (void*)pf_R})
+        if (p) hipFree(p);
+    if (bat_ctl_h) hipHostFree(bat_ctl_h);
+    bat_items = nullptr;
+    bat_tf[0] = bat_tf[1] = bat_tc[0] = bat_tc[1] = nullptr;
+    bat_reeval = nullptr;
+    bat_fpool = nullptr;
+    bat_redo_acc = nullptr;
+    bat_redo_cnt = nullptr;
+    bat_ctl = nullptr;
+    bat_ctl_h = nullptr;
+    pf_R = nullptr;
+    bat_cap = bat_fcap = 0;
+    pf_R_cap = 0;
+}
+
 int Engine::ensure_partials(size_t n) {
     if (n <= partial_cap) return NMLMC_OK;
     if (d_partials) hipFree(d_partials);
@@ -313,6 +333,12 @@ int Engine::prepare_batched(uint64_t M, BatchedCommon& c) {
         if ((e = hipMalloc(&bat_fpool, sizeof(FinalEntry) * want_f)) != hipSuccess) return hip_fail(e, "hipMalloc finals");
         bat_fcap = want_f;
     }
+    if (!bat_redo_acc) {
+        if ((e = hipMalloc(&bat_redo_acc, sizeof(double) * kRedoSlots * kRedoGroups)) != hipSuccess)
+            return hip_fail(e, "hipMalloc redo sums");
+        if ((e = hipMalloc(&bat_redo_cnt, sizeof(uint32_t) * kRedoSlots)) != hipSuccess)
+            return hip_fail(e, "hipMalloc redo counts");
+    }
     if (!bat_ctl) {
         if ((e = hipMalloc(&bat_ctl, sizeof(BatchedCtl))) != hipSuccess) return hip_fail(e, "hipMalloc ctl");
         if ((e = hipHostMalloc(&bat_ctl_h, sizeof(BatchedCtl))) != hipSuccess) return hip_fail(e, "hipHostMalloc ctl");
@@ -325,6 +351,8 @@ int Engine::prepare_batched(uint64_t M, BatchedCommon& c) {
     c.fpool = bat_fpool;
     c.fcap = (uint32_t)std::min<uint64_t>(bat_fcap, 0xffffffffull);
     c.reeval = bat_reeval;
+    c.redo_acc = bat_redo_acc;
+    c.redo_cnt = bat_redo_cnt;
     c.ctl = bat_ctl;
     return NMLMC_OK;
 }
@@ -524,6 +552,7 @@ void init_nestedmc_sampler(uint32_t cores, const nested_sampler_params* params) 
 void cleanup_nestedmc_sampler(void) {
     g_engine.free_samplers();
     g_engine.free_portfolio();
+    g_engine.free_batched();
     g_engine.inited = false;
     g_engine.problem = -1;
     set_error(NMLMC_OK, "");

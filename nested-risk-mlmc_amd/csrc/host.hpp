@@ -63,6 +63,8 @@ struct Engine {
     TestEntry* bat_tc[2] = {nullptr, nullptr};
     FinalEntry* bat_fpool = nullptr;
     uint32_t* bat_reeval = nullptr;
+    double* bat_redo_acc = nullptr;
+    uint32_t* bat_redo_cnt = nullptr;
     BatchedCtl* bat_ctl = nullptr;
     BatchedCtl* bat_ctl_h = nullptr;
     uint64_t bat_cap = 0, bat_fcap = 0;
@@ -73,6 +75,7 @@ struct Engine {
     int seed_samplers(uint64_t s);
     void free_samplers();
     void free_portfolio();
+    void free_batched();
     uint32_t phi_size() const;
     void copy_out(uint32_t ell, sample_output* out);
     int ensure_partials(size_t n);

@@ -24,11 +24,32 @@ struct StockRec {
 };
 
 // data_ex (portfolio_sampler.cpp:94-132) with device pointers.
+// One Exact option for the FP32 inner loop of the mixed form (32 bytes).
+struct MixRec {
+    float K;         // strike
+    float sigT;      // sigma sqrt(T - tau)
+    float driftT;    // (r - sigma^2 / 2) (T - tau)
+    float discp;     // w e^{-rT} / p, p = prob / prob_norm
+    float sigtau;    // sigma sqrt(tau)
+    float S0e;       // S0 e^{(r - sigma^2 / 2) tau}
+    float S0;        // S0
+    uint32_t stype;  // stock index << 8 | (type + 2)
+};
+static_assert(sizeof(MixRec) == 32, "mixed option record must stay 32 bytes");
+
 struct PortfolioDevice {
     const OptionRec* options[3];
     const double* cdfs[3];
-    const uint32_t* guides[3];  // guides[m][b] = max{i : cdf[i] <= b / guide_scale[m]}, b = 0..n
-    double guide_scale[3];      // n / total probability of the method
+    // Option-search guide per method: u of bucket b = (uint32)(u * guide_scale)
+    // (clamped to nb - 1) has its answer max{i : cdf[i] <= u} in
+    // [guides[m][b].x, guides[m][b].y]; nb = 4n buckets with exact FP64 edges.
+    const uint2* guides[3];
+    double guide_scale[3];
+    uint32_t guide_nb[3];
+    // Mixed form, Exact options (method 1) in staged order: FP32 constants of
+    // the inner loss with the stock's parameters and the weight 1/p folded in.
+    const MixRec* mix;
+    double w_est_const;  // the common W_est of all options, or -1 when they differ
     uint32_t nopt[3];
     const StockRec* stocks;
     const double* delta_coalesce;
@@ -80,7 +101,7 @@ struct PortfolioBatchedHost {
     double bs_work;
     uint32_t blocks;
     bool mixed;
-    bool exact_only;  // sub-sampled portfolio whose options are all Exact (method 1)
+    bool exact_only;  // sub-sampling mode 1 and every option Exact (method 1)
 };
 hipError_t launch_portfolio_batched(const PortfolioBatchedHost& H, hipStream_t s);
 

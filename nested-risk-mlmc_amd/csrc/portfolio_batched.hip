@@ -21,18 +21,18 @@ struct PortfolioPolicyT {
     struct Args {
         BatchedCommon c;
         PortfolioDevice pf;
-        double* R_tau;  // [d x M]: stock i of outer sample m at R_tau[i * M + m]
+        double* R_tau;  // [M x d]: stock i of outer sample m at R_tau[m * d + i]
         double bs_work; // work of one init_inner_sampler call (sub-sampling off)
     };
     static constexpr uint64_t spu = 1;
 
     __device__ static double group_work(const Args& A) { return A.bs_work; }
 
-    // Mixed form: every FP32 inner term is continuous in its inputs (a term at
-    // a payoff discontinuity makes its segment's magnitude infinite, so that the
-    // outer sample is redone in FP64, see inner_exact_f) with an error of at
-    // most about 2^-20 of its magnitude, so the FP32 mean is within 2^-20 of the
-    // mean magnitude of the FP64 one; 2^-18 leaves a factor 4.  start is FP64.
+    // Mixed form: each FP32 inner term differs from its FP64 counterpart by at
+    // most 2^-18 of its magnitude (inner_exact_mix; for the other methods' FP32
+    // terms, inner_exact_f, a term at a payoff discontinuity makes its segment's
+    // magnitude infinite), hence the mean by at most 2^-18 of the mean
+    // magnitude.  start is FP64.
     __device__ static double mixed_bound(double mean_mag, double) { return 0x1p-18 * mean_mag; }
 
     __device__ static uint64_t key1(const Args& A, uint64_t m) {
@@ -40,7 +40,11 @@ struct PortfolioPolicyT {
     }
 
     // Stock levels of outer sample m, column-wise over the chunk's outer samples.
-    __device__ static StridedR stocks_at_tau(const Args& A, uint64_t m) { return StridedR{A.R_tau + m, A.c.M}; }
+    // Stock levels of outer sample m: one row of d values (kept in cache while
+    // the outer sample's inner samples gather from it).
+    __device__ static StridedR stocks_at_tau(const Args& A, uint64_t m) {
+        return StridedR{A.R_tau + m * A.pf.d, 1};
+    }
 
     // Outer draw (sample_outer, portfolio_sampler.cpp:293-311): the one-factor
     // model here; with a Cholesky factor, k_pf_outer_chol has already written
@@ -48,8 +52,8 @@ struct PortfolioPolicyT {
     // sub-sampling, the pre-summed BS options (init_inner_sampler, 312-324).
     __device__ static void init_item(const Args& A, uint64_t m, ItemState& it, LaneAcc& acc) {
         const PortfolioDevice& pf = A.pf;
-        double* Rw = A.R_tau + m;
-        const uint64_t stride = A.c.M;
+        double* Rw = A.R_tau + m * pf.d;
+        const uint64_t stride = 1;
         if (pf.chol) {
             acc.st[5] += pf.d;
         } else {
@@ -90,28 +94,75 @@ struct PortfolioPolicyT {
         double a1 = 0., a2 = 0., ab = 0., work = 0.;
         Kahan b1, b2, bab;
         bool redo64 = false;  // a term at a payoff discontinuity (mixed form)
+        if (EXACT_ONLY && F32) {
+            // Two inner samples per iteration: both option searches and record
+            // loads are issued before either sample's Box-Muller block is
+            // computed, so one sample's gathers overlap the other's arithmetic.
+            const double cp = pf.compute_prob[1];
+            const double wc = pf.w_est_const;
+            auto term = [&](const MixRec& q, float Rq, const u64x2& blk1, uint32_t idx) {
+                float x, y, mag;
+                boxmuller_f32(blk1.a, blk1.b, x, y);  // second = word 3 (element a), first = word 2
+                const float d = inner_exact_mix(q, pf.use_antithetic_risk, pf.use_delta_cv, Rq, x, y, mag);
+                work += wc >= 0. ? wc : pf.options[1][idx].W_est;
+                b1.add(d);
+                b2.add(d * d);
+                bab.add(mag);
+            };
+            uint64_t u = u0;
+            for (; u + 1 < u1; u += 2) {
+                const uint64_t ca = ((uint64_t)tag << 40) | u, cb = ca + 1;
+                const u64x2 a0 = threefry2x64_20(0, ca, A.c.seed, k1);
+                const u64x2 b0 = threefry2x64_20(0, cb, A.c.seed, k1);
+                const uint32_t ia = pick_cdf(pf, 1, u01_f64(a0.a) * cp);
+                const uint32_t ib = pick_cdf(pf, 1, u01_f64(b0.a) * cp);
+                const MixRec qa = pf.mix[ia], qb = pf.mix[ib];
+                const float Ra = (float)R[qa.stype >> 8], Rb = (float)R[qb.stype >> 8];
+                const u64x2 a1 = threefry2x64_20(1, ca, A.c.seed, k1);
+                const u64x2 b1k = threefry2x64_20(1, cb, A.c.seed, k1);
+                term(qa, Ra, a1, ia);
+                term(qb, Rb, b1k, ib);
+            }
+            if (u < u1) {
+                const uint64_t ca = ((uint64_t)tag << 40) | u;
+                const u64x2 a0 = threefry2x64_20(0, ca, A.c.seed, k1);
+                const uint32_t ia = pick_cdf(pf, 1, u01_f64(a0.a) * cp);
+                const MixRec qa = pf.mix[ia];
+                term(qa, (float)R[qa.stype >> 8], threefry2x64_20(1, ca, A.c.seed, k1), ia);
+            }
+            s.nu += 2 * (u1 - u0);
+            s.nn += 2 * (u1 - u0);
+            s.s1 += b1.value();
+            s.s2 += b2.value();
+            s.sab += bab.value();
+            s.work += work;
+            return;
+        }
         for (uint64_t u = u0; u < u1; u++) {
-            DStream st;
-            st.init(A.c.seed, k1, ((uint64_t)tag << 40) | u, 0, 0., 0);
             if (EXACT_ONLY) {
-                uint32_t method, idx;
-                const double prob = pick_option(pf, st, st, method, idx);
-                const OptionRec o = pf.options[1][idx];
-                const uint64_t first = st.rand(), second = st.rand();
-                s.nu += st.nu;
+                // The sample's words: 0 method uniform (only method 1 has options),
+                // 1 option uniform (block 0), 2 and 3 the Box-Muller pair (block 1).
+                const uint64_t cu = ((uint64_t)tag << 40) | u;
+                const u64x2 blk0 = threefry2x64_20(0, cu, A.c.seed, k1);
+                const u64x2 blk1 = threefry2x64_20(1, cu, A.c.seed, k1);
+                const uint32_t idx = pick_cdf(pf, 1, u01_f64(blk0.a) * pf.compute_prob[1]);
+                const uint64_t first = blk1.b, second = blk1.a;
+                s.nu += 2;
                 s.nn += 2;
-                work += o.W_est;
                 if (F32) {
+                    const MixRec q = pf.mix[idx];
+                    work += pf.w_est_const >= 0. ? pf.w_est_const : pf.options[1][idx].W_est;
                     float x, y, mag;
-                    bool exact64;
                     boxmuller_f32(second, first, x, y);
-                    const float ip = (float)(1. / prob);
-                    const float d = inner_exact_f(pf, o, (float)R[o.stock_ind], x, y, mag, exact64) * ip;
-                    redo64 |= exact64;
+                    const float d = inner_exact_mix(q, pf.use_antithetic_risk, pf.use_delta_cv,
+                                                    (float)R[q.stype >> 8], x, y, mag);
                     b1.add(d);
                     b2.add(d * d);
-                    bab.add(mag * ip);
+                    bab.add(mag);
                 } else {
+                    const OptionRec o = pf.options[1][idx];
+                    const double prob = o.prob / pf.prob_norm;
+                    work += o.W_est;
                     double z0, z1, w64 = 0.;
                     boxmuller_f64(second, first, z0, z1);
                     const double v = inner_exact_d(pf, o, R, z0, z1, w64) / prob;
@@ -120,6 +171,8 @@ struct PortfolioPolicyT {
                 }
                 continue;
             }
+            DStream st;
+            st.init(A.c.seed, k1, ((uint64_t)tag << 40) | u, 0, 0., 0);
             if (F32 && pf.subsample != 0) {
                 uint32_t method, idx;
                 const double prob = pick_option(pf, st, st, method, idx);
@@ -168,14 +221,17 @@ struct PortfolioPolicyT {
 // stream {seed, key}, counter {b, 0} for pair b) go to LDS transposed, z[j][lane],
 // then each wave forms every fourth row of L z, summing over j in order (as the
 // one-lane restatement does) from conflict-free LDS reads and wave-uniform
-// reads of L, and writes the stock levels column-wise (coalesced).
-template <class P>
+// reads of L.  STAGED: the 64 x d stock levels are collected in LDS and written
+// as contiguous rows; otherwise each lane writes its own row.
+template <class P, bool STAGED>
 __global__ __launch_bounds__(256) void k_pf_outer_chol(typename P::Args A) {
-    extern __shared__ double zt[];  // [d][64]
+    extern __shared__ double zt[];  // [d][64], then (STAGED) [64][d + 1]
     const PortfolioDevice& pf = A.pf;
     const uint32_t d = pf.d;
+    double* rt = zt + 64 * d;
     const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const uint64_t m = (uint64_t)blockIdx.x * 64 + lane;
+    const uint64_t m0 = (uint64_t)blockIdx.x * 64;
+    const uint64_t m = m0 + lane;
     const bool valid = m < A.c.M;
     const uint64_t k1 = P::key1(A, valid ? m : 0);
     for (uint32_t b = wv; 2 * b < d; b += 4) {
@@ -190,9 +246,17 @@ __global__ __launch_bounds__(256) void k_pf_outer_chol(typename P::Args A) {
         const double* Li = pf.chol + (size_t)i * d;
         double a = 0.;
         for (uint32_t j = 0; j <= i; j++) a += Li[j] * zt[j * 64 + lane];
-        if (valid) {
-            const StockRec st = pf.stocks[i];
-            A.R_tau[(uint64_t)i * A.c.M + m] = simulate_stock_d(st.S0, st.mu, st.sigma, pf.tau, pf.stau * a);
+        const StockRec st = pf.stocks[i];
+        const double Ri = simulate_stock_d(st.S0, st.mu, st.sigma, pf.tau, pf.stau * a);
+        if (STAGED) rt[lane * (d + 1) + i] = Ri;
+        else if (valid) A.R_tau[m * d + i] = Ri;
+    }
+    if (STAGED) {
+        __syncthreads();
+        const uint64_t nv = A.c.M - m0 < 64 ? A.c.M - m0 : 64;
+        for (uint32_t k = threadIdx.x; k < nv * d; k += blockDim.x) {
+            const uint32_t r = k / d, i = k - r * d;
+            A.R_tau[(m0 + r) * d + i] = rt[r * (d + 1) + i];
         }
     }
 }
@@ -201,8 +265,12 @@ template <bool EXACT_ONLY>
 hipError_t PortfolioPolicyT<EXACT_ONLY>::launch_outer(const Args& A, hipStream_t s) {
     if (!A.pf.chol || A.c.M == 0) return hipSuccess;
     const uint32_t blocks = (uint32_t)((A.c.M + 63) / 64);
-    hipLaunchKernelGGL(k_pf_outer_chol<PortfolioPolicyT<EXACT_ONLY>>, dim3(blocks), dim3(256),
-                       sizeof(double) * 64 * A.pf.d, s, A);
+    const size_t lds_z = sizeof(double) * 64 * A.pf.d, lds_r = sizeof(double) * 64 * (A.pf.d + 1);
+    using P = PortfolioPolicyT<EXACT_ONLY>;
+    if (lds_z + lds_r <= 160 * 1024)
+        hipLaunchKernelGGL((k_pf_outer_chol<P, true>), dim3(blocks), dim3(256), lds_z + lds_r, s, A);
+    else
+        hipLaunchKernelGGL((k_pf_outer_chol<P, false>), dim3(blocks), dim3(256), lds_z, s, A);
     return hipGetLastError();
 }
 

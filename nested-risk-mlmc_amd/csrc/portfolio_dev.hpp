@@ -289,21 +289,20 @@ struct StridedR {
 // rand_int_cdf (common.hpp:160-176).  The reference's fixed-step binary search
 // over the exclusive-prefix CDF returns max{i : cdf[i] <= u} (its predicate
 // u < cdf[mid] is monotone in mid, and cdf[0] = 0 is never probed).  The same
-// index is found here from a guide table: bucket b = floor(u * n / total)
-// bounds the answer by guide[b] and guide[b + 1]; both bounds are checked
-// against the CDF (falling back to the whole range if either fails), and a
-// binary search closes the usually one- or two-entry gap.  About 4 dependent
-// loads instead of ceil(log2 n).
+// index is found here from the guide: the bucket of u bounds the answer by
+// [g.x, g.y], which is usually one option (no CDF load at all); otherwise both
+// bounds are checked against the CDF (falling back to the whole range if one
+// failed) and a binary search closes the gap.
 __device__ __forceinline__ uint32_t pick_cdf(const PortfolioDevice& pf, uint32_t m, double u) {
     const uint32_t n = pf.nopt[m];
     if (n == 1) return 0;
-    const double* __restrict__ cdf = pf.cdfs[m];
-    const uint32_t* __restrict__ guide = pf.guides[m];
+    const uint32_t nb = pf.guide_nb[m];
     const double fb = u * pf.guide_scale[m];
-    const uint32_t b = fb < (double)(n - 1) ? (uint32_t)fb : n - 1;
-    uint32_t lo = guide[b];
-    uint32_t hi = guide[b + 1] + 1;
-    if (hi > n) hi = n;
+    const uint32_t b = fb < (double)(nb - 1) ? (uint32_t)fb : nb - 1;
+    const uint2 g = pf.guides[m][b];
+    if (g.x == g.y) return g.x;
+    const double* __restrict__ cdf = pf.cdfs[m];
+    uint32_t lo = g.x, hi = g.y + 1;
     if (lo > 0 && !(cdf[lo] <= u)) lo = 0;
     if (hi < n && !(u < cdf[hi])) hi = n;
     while (hi - lo > 1) {
@@ -328,6 +327,15 @@ __device__ __forceinline__ double pick_option(const PortfolioDevice& pf, DStream
     const double u = inner.randu() * pf.compute_prob[method];
     idx = pick_cdf(pf, method, u);
     return pf.options[method][idx].prob / pf.prob_norm;
+}
+
+// The draws of pick_option for sub-sampling with probability prob (mode 1),
+// without reading the option's probability: method and option index only.
+__device__ __forceinline__ uint32_t pick_index(const PortfolioDevice& pf, DStream& st, uint32_t& method) {
+    const double uc = st.randu();
+    method = pick_method(pf, uc * pf.prob_norm);
+    const double u = st.randu() * pf.compute_prob[method];
+    return pick_cdf(pf, method, u);
 }
 
 // One inner sample of portfolio::sampler::sample_inner (330-334) with the
@@ -424,6 +432,49 @@ __device__ __forceinline__ float inner_exact_f(const PortfolioDevice& pf, const 
     const float win = 0x1p-17f * K;
     const bool near1 = fabsf(S1 - K) < win || fabsf(S2 - K) < win;
     exact64 = (is_digital(o.type) && (near1 || fabsf(St - K) < win)) || (cv && near1);
+    return 0.5f * (val1 + val2 + dv1 + dv2);
+}
+
+// inner_exact_f from an option's MixRec: the term already divided by p and its
+// error scale `mag`, such that 2^-18 mag bounds (with a factor 4 to spare) the
+// difference between this FP32 term and the FP64 term of the same draws.
+// Away from the strike every quantity is a continuous function of the draws and
+// of R: the stock levels carry a relative error of about 2^-20.5 at most (the
+// FP32 normals' error times sigma sqrt(T), plus a few units of 2^-24 from the
+// exponentials of O(1) arguments), so the term's error is at most about 2^-20
+// times its continuous scale.  A digital payoff, or the control variate's
+// indicator S < K, whose stock level lies within 2^-17 K of the strike may
+// jump under that rounding; the jump's size (|discp|, or the indicator's
+// factor) enters mag with the weight 2^18, so that the bound covers it whole.
+__device__ __forceinline__ float inner_exact_mix(const MixRec& q, bool anti, bool cv_on, float R, float z0,
+                                                 float z1, float& mag) {
+    const int type = (int)(q.stype & 0xffu) - 2;
+    const float gT = __expf(q.sigT * z1 + q.driftT);
+    const float S1 = q.S0e * __expf(q.sigtau * z0) * gT;
+    const float S2 = anti ? q.S0e * __expf(-q.sigtau * z0) * gT : S1;
+    const float St = R * gT;
+    const float pt = payoff_f(type, St, q.K);
+    const float val1 = q.discp * (payoff_f(type, S1, q.K) - pt);
+    const float val2 = anti ? q.discp * (payoff_f(type, S2, q.K) - pt) : val1;
+    float dv1 = 0.f, dv2 = 0.f;
+    const bool digital = is_digital(type);
+    const bool cv = cv_on && !digital;
+    const float diff = R - q.S0;
+    if (cv) {
+        const float c = (float)(type == 1);
+        dv1 = diff * q.discp * __fdividef(S1, q.S0) * (c - (float)(S1 < q.K));
+        dv2 = anti ? diff * q.discp * __fdividef(S2, q.S0) * (c - (float)(S2 < q.K)) : dv1;
+    }
+    mag = 0.5f * (fabsf(val1) + fabsf(val2) + fabsf(dv1) + fabsf(dv2)) + fabsf(q.discp) * (S1 + S2 + St);
+    const float win = 0x1p-17f * q.K;
+    const bool near1 = fabsf(S1 - q.K) < win, near2 = fabsf(S2 - q.K) < win, neart = fabsf(St - q.K) < win;
+    if (digital) {
+        const float at_risk = (float)near1 + (float)near2 + 2.f * (float)neart;
+        mag += 0x1p18f * 0.5f * fabsf(q.discp) * at_risk;
+    } else if (cv) {
+        const float jump = fabsf(diff * q.discp) * __fdividef(q.K, q.S0) * 1.001f;
+        mag += 0x1p18f * 0.5f * jump * ((float)near1 + (float)near2);
+    }
     return 0.5f * (val1 + val2 + dv1 + dv2);
 }
 

@@ -115,18 +115,33 @@ int Engine::stage_portfolio(const portfolio_data* data) {
         int rc = upload(*this, (const OptionRec*)ops, n, &P.options[m]);
         if (rc) return rc;
         if ((rc = upload(*this, cdf.data(), n, &P.cdfs[m]))) return rc;
-        // Guide table of the option search: guide[b] = max{i : cdf[i] <= b c / n}.
+        // Guide of the option search: nb = 4n buckets; bucket b holds the u with
+        // (uint32)(u * scale) == b, i.e. [edge(b), edge(b + 1)) with edge(b) the
+        // smallest double whose product with scale (one IEEE multiply, as on the
+        // device) reaches b; its entry is the answer at both edges.
         if (n) {
-            std::vector<uint32_t> guide(n + 1);
-            const double scale = (double)n / c;
+            const uint64_t nb = 4 * (uint64_t)n;
+            const double scale = (double)nb / c;
+            auto edge = [scale](uint64_t b) {
+                if (b == 0) return 0.;
+                double x = (double)b / scale;
+                while (x > 0 && std::nextafter(x, 0.) * scale >= (double)b) x = std::nextafter(x, 0.);
+                while (x * scale < (double)b) x = std::nextafter(x, 1e308);
+                return x;
+            };
+            std::vector<uint32_t> at(nb + 1);
             size_t i = 0;
-            for (size_t b = 0; b <= n; b++) {
-                const double ub = (double)b / scale;
-                while (i + 1 < n && cdf[i + 1] <= ub) i++;
-                guide[b] = (uint32_t)i;
+            for (uint64_t b = 0; b < nb; b++) {
+                const double x = edge(b);
+                while (i + 1 < n && cdf[i + 1] <= x) i++;
+                at[b] = (uint32_t)i;
             }
-            if ((rc = upload(*this, guide.data(), n + 1, &P.guides[m]))) return rc;
+            at[nb] = (uint32_t)(n - 1);  // the last bucket is open above
+            std::vector<uint2> guide(nb);
+            for (uint64_t b = 0; b < nb; b++) guide[b] = make_uint2(at[b], at[b + 1]);
+            if ((rc = upload(*this, guide.data(), nb, &P.guides[m]))) return rc;
             P.guide_scale[m] = scale;
+            P.guide_nb[m] = (uint32_t)nb;
         }
         P.nopt[m] = (uint32_t)n;
         P.options_count += (uint32_t)n;
@@ -156,6 +171,40 @@ int Engine::stage_portfolio(const portfolio_data* data) {
     P.use_milstein = data->use_milstein;
     if (data->options[2].size && (data->dt_beta < 2 || !(data->dt_prob_expo < 0) || data->dt_h0inv == 0))
         return set_error(NMLMC_ERR_ARG, "Unbiased options need dt_beta >= 2, dt_prob_expo < 0, dt_h0inv > 0");
+    // FP32 records of the Exact options for the mixed form.
+    {
+        const option_params* ex = (const option_params*)data->options[1].data;
+        const size_t n1 = data->options[1].size;
+        std::vector<MixRec> mix(n1);
+        for (size_t j = 0; j < n1; j++) {
+            const option_params& o = ex[j];
+            const stock_params& st = stocks[o.stock_ind];
+            const double drift = data->r - 0.5 * st.sigma * st.sigma;
+            MixRec& q = mix[j];
+            q.K = (float)o.strike;
+            q.sigT = (float)(st.sigma * std::sqrt(o.maturity - data->tau));
+            q.driftT = (float)(drift * (o.maturity - data->tau));
+            q.discp = (float)(o.w * std::exp(-data->r * o.maturity) * P.prob_norm / o.prob);
+            q.sigtau = (float)(st.sigma * std::sqrt(data->tau));
+            q.S0e = (float)(st.S0 * std::exp(drift * data->tau));
+            q.S0 = (float)st.S0;
+            q.stype = (o.stock_ind << 8) | (uint32_t)(o.type + 2);
+        }
+        if ((rc = upload(*this, mix.data(), n1, &P.mix))) return rc;
+    }
+    P.w_est_const = -1.;
+    {
+        bool first = true, same = true;
+        double w0 = 0.;
+        for (int m = 0; m < 3; m++) {
+            const option_params* ops = (const option_params*)data->options[m].data;
+            for (size_t j = 0; j < data->options[m].size; j++) {
+                if (first) w0 = ops[j].W_est, first = false;
+                else same = same && ops[j].W_est == w0;
+            }
+        }
+        if (!first && same) P.w_est_const = w0;
+    }
     // Work of one init_inner_sampler call without sub-sampling: the BS options.
     pf_bs_work = 0.;
     if (data->subsample == 0) {
@@ -237,7 +286,7 @@ int Engine::sample_portfolio_batched(uint32_t ell, uint64_t M, uint64_t outer_of
     H.bs_work = pf_bs_work;
     H.blocks = batched_blocks();
     H.mixed = form == NMLMC_FORM_MIXED;
-    H.exact_only = pf.subsample != 0 && pf.nopt[0] == 0 && pf.nopt[2] == 0;
+    H.exact_only = pf.subsample == 1 && pf.nopt[0] == 0 && pf.nopt[2] == 0;
     last_kernel = "k_b_final";
     return run_batched(ell, M, H.c, out, [&]() { return launch_portfolio_batched(H, stream); }, tr_phi_f,
                        tr_phi_c, tr_nf, tr_nc);
