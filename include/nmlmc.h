@@ -223,6 +223,12 @@ int nmlmc_mlmc_run(double targetTOL, uint64_t M0, double theta, double Ca,
 /* Same driver over a caller-supplied level sampler (mlmc_input_t::sample_level,
  * mlmc.hpp:224): fn fills *out for M fresh samples of level ell and returns 0, or
  * a non-zero code that aborts the run. */
+/* The level table of the last completed nmlmc_mlmc_run / nmlmc_mlmc_run_cb:
+ * mlmc_output_t::to_string (mlmc.hpp:106-183), the text the reference prints
+ * after each iteration in verbose mode (and that parse_mlmc_output.py reads).
+ * The string stays valid until the next run. */
+const char* nmlmc_mlmc_last_table(void);
+
 typedef int (*nmlmc_sample_level_fn)(void* user, uint32_t ell, uint64_t M, nmlmc_level_stats* out);
 int nmlmc_mlmc_run_cb(nmlmc_sample_level_fn fn, void* user, double targetTOL, uint64_t M0,
                       double theta, double Ca, double starting_level_coeff, int verbose,

@@ -3,6 +3,7 @@
 #include <chrono>
 #include <cstring>
 #include <stdexcept>
+#include <string>
 
 #include "../../include/nmlmc.h"
 #include "host.hpp"
@@ -15,6 +16,8 @@ struct abort_run {
     int code;
 };
 
+std::string g_last_table;  // mlmc_output_t::to_string of the last completed run
+
 int run_driver(mlmc_real::mlmc_input_t& in, uint32_t max_levels, nmlmc_level_stats* levels_out,
                uint32_t* n_levels, double* summary_out) {
     const auto t0 = std::chrono::steady_clock::now();
@@ -25,6 +28,7 @@ int run_driver(mlmc_real::mlmc_input_t& in, uint32_t max_levels, nmlmc_level_sta
         return a.code;
     }
     const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    g_last_table = out.to_string();
     const uint32_t n = (uint32_t)out.levels.size();
     if (n_levels) *n_levels = n;
     if (levels_out) {
@@ -65,6 +69,8 @@ void fill_input(mlmc_real::mlmc_input_t& in, double targetTOL, uint64_t M0, doub
 using namespace nmlmc;
 
 extern "C" {
+
+const char* nmlmc_mlmc_last_table(void) { return g_last_table.c_str(); }
 
 int nmlmc_mlmc_run(double targetTOL, uint64_t M0, double theta, double Ca, double starting_level_coeff,
                    int verbose, uint32_t max_levels, nmlmc_level_stats* levels_out, uint32_t* n_levels,

@@ -83,6 +83,7 @@ EXPORTS = [
     "nmlmc_last_kernel_ms", "nmlmc_last_kernel_name", "nmlmc_last_group_lanes",
     "nmlmc_mlmc_run", "nmlmc_mlmc_run_cb", "portfolio_sample_delta_unbiased",
     "portfolio_estimate_delta_for_unbiased", "portfolio_gen_data", "portfolio_free_data",
+    "nmlmc_mlmc_last_table",
 ]
 
 _lib = None
@@ -117,6 +118,7 @@ def load(path=LIB_PATH):
         "portfolio_estimate_delta_for_unbiased": (None, [C.POINTER(PortfolioData), dbl]),
         "portfolio_gen_data": (PortfolioData, [u32, u32, u64, dp, C.c_bool, C.c_bool]),
         "portfolio_free_data": (None, [C.POINTER(PortfolioData)]),
+        "nmlmc_mlmc_last_table": (C.c_char_p, []),
         "nmlmc_last_error": (i32, []),
         "nmlmc_last_error_string": (C.c_char_p, []),
         "nmlmc_set_form": (i32, [i32]),
@@ -455,3 +457,8 @@ def portfolio_gen_data(P, Q, seed, probs, uniform_weights=False, uniform_prob=Fa
         return _flatten_portfolio_data(pd)
     finally:
         lib.portfolio_free_data(C.byref(pd))
+
+
+def mlmc_last_table():
+    """mlmc_output_t::to_string of the last completed MLMC run (the level table)."""
+    return load().nmlmc_mlmc_last_table().decode()

@@ -259,6 +259,7 @@ class Ref(_Common):
         lib.refp_portfolio_get_params.argtypes = [_dp, _dp]
         lib.refp_portfolio_init.argtypes = [C.c_uint64]
         lib.refp_portfolio_sample_delta.argtypes = [C.c_uint32, C.c_uint64, C.c_uint32, _dp, _dp]
+        lib.refp_mlmc_last_table.restype = C.c_char_p
         lib.refp_portfolio_sample.argtypes = [C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32, _dp,
                                               _dp, _dp, _u64p]
         lib.refp_portfolio_gen.argtypes = [C.c_uint32, C.c_uint32, C.c_uint64, _dp, C.c_int,
@@ -338,6 +339,9 @@ class Ref(_Common):
         self.lib.refp_portfolio_sample_delta(option_index, C.c_uint64(M), moments, _ptr(sums, _dp),
                                              _ptr(work, _dp))
         return sums, work[0]
+
+    def mlmc_last_table(self):
+        return self.lib.refp_mlmc_last_table().decode()
 
     def portfolio_gen(self, P, Q, seed, probs, uniform_weights=False, uniform_prob=False):
         probs = np.ascontiguousarray(probs, np.float64)

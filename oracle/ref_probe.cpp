@@ -289,6 +289,11 @@ void refp_portfolio_gen(uint32_t P, uint32_t Q, uint64_t seed, const double* pro
 struct refp_call { uint32_t ell; uint64_t M; };
 static std::vector<refp_call> g_calls;
 
+static std::string g_last_table;
+
+// mlmc_output_t::to_string of the last refp_mlmc_fake_run (mlmc.hpp:106-183).
+const char* refp_mlmc_last_table() { return g_last_table.c_str(); }
+
 int refp_mlmc_fake_run(double targetTOL, uint64_t M0, double theta, double Ca,
                        double starting_level_coeff, uint32_t nlev_max,
                        const double* E, const double* V, const double* W,
@@ -318,6 +323,7 @@ int refp_mlmc_fake_run(double targetTOL, uint64_t M0, double theta, double Ca,
         return o;
     };
     auto out = mlmc_real::run(in);
+    g_last_table = out.to_string();
     int n = 0;
     for (auto& c : g_calls) { if (n >= max_calls) break; out_ell[n] = c.ell; out_M[n] = c.M; n++; }
     out_summary[0] = out.levels.size();

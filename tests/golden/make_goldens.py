@@ -185,7 +185,8 @@ def mlmc_fixture(R):
     for cs in cases:
         ell, M, summ = R.mlmc_fake_run(cs["tol"], cs["M0"], cs["E"], cs["V"], cs["W"], cs["Ef"],
                                        cs["Vf"])
-        out.append({"case": cs, "ell": ell.tolist(), "M": u64s(M), "summary": summ.tolist()})
+        out.append({"case": cs, "ell": ell.tolist(), "M": u64s(M), "summary": summ.tolist(),
+                    "table": R.mlmc_last_table()})
     return out
 
 

@@ -103,6 +103,9 @@ def test_mlmc_driver_reproduces_reference_call_sequence(nm):
         assert r["bias_est"] == pytest.approx(case["summary"][2], rel=1e-15)
         assert r["stat_error"] == pytest.approx(case["summary"][3], rel=1e-15)
         assert r["TOL"] == pytest.approx(case["summary"][4], rel=1e-15)
+        # The level table (mlmc_output_t::to_string, mlmc.hpp:106-183), the
+        # format parse_mlmc_output.py reads: identical text.
+        assert nm.mlmc_last_table() == case["table"], cs["name"]
 
 
 def test_mlmc_driver_closed_form_allocation(nm):
