@@ -14,7 +14,7 @@ both forms are driven to RMSE eps by the MLMC driver (time to eps).
 
 roofline: a launch is one level call's kernel chain (k_b_zero, k_b_init, one
 k_b_test per refinement round, k_b_final_count/alloc/fill, k_b_final,
-k_b_reeval), which is VALU-bound (integer Threefry + Box-Muller + compensated
+k_b_redo_split, k_b_redo_seq), which is VALU-bound (integer Threefry + Box-Muller + compensated
 sums; no input data, about 1 KB of queue and item-state traffic per outer
 sample), so achieved = algorithmic VALU lane-ops per launch (inner samples x ops
 per inner sample + outer samples x ops per outer sample, counts in
@@ -64,7 +64,8 @@ def sweep(nm, M0, levels, rec=None):
             reev, flips = nm.mixed_stats()
             rec.append(dict(ell=ell, M=M, wall=dt, kernel_ms=nm.last_kernel_ms(),
                             inner=int(o["stats"][0]), G=nm.last_group_lanes(), reev=reev,
-                            flips=flips))
+                            flips=flips, fmean=o["fsums"][0] / M,
+                            dmean=(o["diffsums"][0] / M) if ell else 0.))
 
 
 def ops_per_sample(form):
@@ -109,7 +110,7 @@ def roofline(recs, form):
     return {"bound": "valu", "achieved": round(achieved, 3), "peak": round(PEAK_VALU_TOPS, 1),
             "unit": "TOP/s", "frac": round(achieved / PEAK_VALU_TOPS, 4),
             "traffic": traffic_bytes(chain) if chain else None,
-            "kernel": ("level-call kernel chain k_b_zero..k_b_reeval (mixed)" if form == 2 else
+            "kernel": ("level-call kernel chain k_b_zero..k_b_redo_seq (mixed)" if form == 2 else
                        "batched level-call kernel chain" if form else "k_model_parity"),
             "launches": len(recs), "avg_launch_ms": round(1e3 * secs / len(recs), 4),
             "ops_per_inner_sample": oi}
@@ -303,6 +304,17 @@ def main():
         bel = time.perf_counter() - t0
         extra["batched_fp64"] = {"outer_samples_per_s": round(outer_per_step / bel, 1),
                                  "per_level": per_level(brecs, args.levels)}
+        # The mixed form's first timed sweep and this sweep draw the same outer
+        # samples (same keys and offsets after one warm-up sweep each): per-level
+        # difference of the level means (fine mean at l = 0, difference mean
+        # above) and the mixed form's FP64 re-evaluations and indicator flips.
+        if args.warmup == 1:
+            first = recs[:args.levels + 1]
+            extra["mixed_vs_fp64_per_level"] = [
+                {"ell": a["ell"], "mean_diff": abs((a["dmean"] if a["ell"] else a["fmean"]) -
+                                                   (b["dmean"] if b["ell"] else b["fmean"])),
+                 "reevaluated": a["reev"], "flips": a["flips"]}
+                for a, b in zip(first, brecs)]
         nm.cleanup_nestedmc_sampler()
     if not args.no_tte:
         tm = time_to_eps(nm, chosen, 1, args.eps, args.tte_M0)
@@ -310,9 +322,13 @@ def main():
         tp = time_to_eps(nm, nm.FORM_PLAIN, args.plain_cores, args.eps, args.tte_M0)
         print("# time to eps (plain): %s" % tp, file=sys.stderr, flush=True)
         exact = 0.5 * math.erfc(2 / math.sqrt(2))
+        sig = math.hypot(tm["stat_error"], tp["stat_error"])
         extra["time_to_eps"] = {"eps": args.eps, "mixed": tm, "plain_fp64": tp,
                                 "speedup": round(tp["seconds"] / tm["seconds"], 2),
-                                "exact": exact}
+                                "exact": exact,
+                                "estimates_within_3_sigma": bool(abs(tm["estimate"] - tp["estimate"]) <= 3 * sig),
+                                "mixed_error_over_rmse": abs(tm["estimate"] - exact) / tm["rmse_est"],
+                                "plain_error_over_rmse": abs(tp["estimate"] - exact) / tp["rmse_est"]}
     if args.portfolio != "none":
         extra["portfolio"] = portfolio_bench(nm, args.portfolio, args.pf_levels, args.pf_M0,
                                              args.pf_rel_eps, not args.no_tte, args.plain_cores,

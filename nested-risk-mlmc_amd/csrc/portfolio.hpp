@@ -24,31 +24,41 @@ struct StockRec {
 };
 
 // data_ex (portfolio_sampler.cpp:94-132) with device pointers.
-// One Exact option for the FP32 inner loop of the mixed form (32 bytes).
+// One Exact option for the FP32 inner loop of the mixed form (16 bytes).
 struct MixRec {
     float K;         // strike
-    float sigT;      // sigma sqrt(T - tau)
-    float driftT;    // (r - sigma^2 / 2) (T - tau)
+    float Tt;        // T - tau
     float discp;     // w e^{-rT} / p, p = prob / prob_norm
-    float sigtau;    // sigma sqrt(tau)
-    float S0e;       // S0 e^{(r - sigma^2 / 2) tau}
-    float S0;        // S0
     uint32_t stype;  // stock index << 8 | (type + 2)
 };
-static_assert(sizeof(MixRec) == 32, "mixed option record must stay 32 bytes");
+static_assert(sizeof(MixRec) == 16, "mixed option record must stay 16 bytes");
+
+// Per-stock constants of the mixed form's inner loop (32 bytes).
+struct MixStock {
+    float sig;     // sigma
+    float dcoef;   // r - sigma^2 / 2
+    float sigtau;  // sigma sqrt(tau)
+    float S0e;     // S0 e^{(r - sigma^2 / 2) tau}
+    float S0;      // S0
+    float pad[3];
+};
+
+constexpr uint32_t kGuideLo = 0x7fffffffu, kGuideMulti = 0x80000000u;
 
 struct PortfolioDevice {
     const OptionRec* options[3];
     const double* cdfs[3];
     // Option-search guide per method: u of bucket b = (uint32)(u * guide_scale)
-    // (clamped to nb - 1) has its answer max{i : cdf[i] <= u} in
-    // [guides[m][b].x, guides[m][b].y]; nb = 4n buckets with exact FP64 edges.
-    const uint2* guides[3];
+    // (clamped to nb - 1) has its answer max{i : cdf[i] <= u} in [lo(b), lo(b + 1)]
+    // with lo = guides[m][.] & kGuideLo; bit 31 marks a bucket holding more than
+    // one answer.  nb = 4n buckets with exact FP64 edges, nb + 1 entries.
+    const uint32_t* guides[3];
     double guide_scale[3];
     uint32_t guide_nb[3];
     // Mixed form, Exact options (method 1) in staged order: FP32 constants of
     // the inner loss with the stock's parameters and the weight 1/p folded in.
     const MixRec* mix;
+    const MixStock* mix_stocks;
     double w_est_const;  // the common W_est of all options, or -1 when they differ
     uint32_t nopt[3];
     const StockRec* stocks;

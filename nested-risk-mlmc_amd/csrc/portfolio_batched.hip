@@ -103,7 +103,8 @@ struct PortfolioPolicyT {
             auto term = [&](const MixRec& q, float Rq, const u64x2& blk1, uint32_t idx) {
                 float x, y, mag;
                 boxmuller_f32(blk1.a, blk1.b, x, y);  // second = word 3 (element a), first = word 2
-                const float d = inner_exact_mix(q, pf.use_antithetic_risk, pf.use_delta_cv, Rq, x, y, mag);
+                const MixStock st = pf.mix_stocks[q.stype >> 8];
+                const float d = inner_exact_mix(q, st, pf.use_antithetic_risk, pf.use_delta_cv, Rq, x, y, mag);
                 work += wc >= 0. ? wc : pf.options[1][idx].W_est;
                 b1.add(d);
                 b2.add(d * d);
@@ -140,35 +141,23 @@ struct PortfolioPolicyT {
         }
         for (uint64_t u = u0; u < u1; u++) {
             if (EXACT_ONLY) {
-                // The sample's words: 0 method uniform (only method 1 has options),
-                // 1 option uniform (block 0), 2 and 3 the Box-Muller pair (block 1).
+                // FP64 (the FP32 loop returned above).  The sample's words: 0 method
+                // uniform (only method 1 has options), 1 option uniform (block 0),
+                // 2 and 3 the Box-Muller pair (block 1).
                 const uint64_t cu = ((uint64_t)tag << 40) | u;
                 const u64x2 blk0 = threefry2x64_20(0, cu, A.c.seed, k1);
                 const u64x2 blk1 = threefry2x64_20(1, cu, A.c.seed, k1);
                 const uint32_t idx = pick_cdf(pf, 1, u01_f64(blk0.a) * pf.compute_prob[1]);
-                const uint64_t first = blk1.b, second = blk1.a;
                 s.nu += 2;
                 s.nn += 2;
-                if (F32) {
-                    const MixRec q = pf.mix[idx];
-                    work += pf.w_est_const >= 0. ? pf.w_est_const : pf.options[1][idx].W_est;
-                    float x, y, mag;
-                    boxmuller_f32(second, first, x, y);
-                    const float d = inner_exact_mix(q, pf.use_antithetic_risk, pf.use_delta_cv,
-                                                    (float)R[q.stype >> 8], x, y, mag);
-                    b1.add(d);
-                    b2.add(d * d);
-                    bab.add(mag);
-                } else {
-                    const OptionRec o = pf.options[1][idx];
-                    const double prob = o.prob / pf.prob_norm;
-                    work += o.W_est;
-                    double z0, z1, w64 = 0.;
-                    boxmuller_f64(second, first, z0, z1);
-                    const double v = inner_exact_d(pf, o, R, z0, z1, w64) / prob;
-                    a1 += v;
-                    a2 += v * v;
-                }
+                const OptionRec o = pf.options[1][idx];
+                const double prob = o.prob / pf.prob_norm;
+                work += o.W_est;
+                double z0, z1, w64 = 0.;
+                boxmuller_f64(blk1.a, blk1.b, z0, z1);
+                const double v = inner_exact_d(pf, o, R, z0, z1, w64) / prob;
+                a1 += v;
+                a2 += v * v;
                 continue;
             }
             DStream st;

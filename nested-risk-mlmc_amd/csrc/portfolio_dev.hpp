@@ -299,10 +299,11 @@ __device__ __forceinline__ uint32_t pick_cdf(const PortfolioDevice& pf, uint32_t
     const uint32_t nb = pf.guide_nb[m];
     const double fb = u * pf.guide_scale[m];
     const uint32_t b = fb < (double)(nb - 1) ? (uint32_t)fb : nb - 1;
-    const uint2 g = pf.guides[m][b];
-    if (g.x == g.y) return g.x;
+    const uint32_t* __restrict__ guide = pf.guides[m];
+    const uint32_t g = guide[b];
+    if (!(g & kGuideMulti)) return g;
     const double* __restrict__ cdf = pf.cdfs[m];
-    uint32_t lo = g.x, hi = g.y + 1;
+    uint32_t lo = g & kGuideLo, hi = (guide[b + 1] & kGuideLo) + 1;
     if (lo > 0 && !(cdf[lo] <= u)) lo = 0;
     if (hi < n && !(u < cdf[hi])) hi = n;
     while (hi - lo > 1) {
@@ -446,12 +447,12 @@ __device__ __forceinline__ float inner_exact_f(const PortfolioDevice& pf, const 
 // indicator S < K, whose stock level lies within 2^-17 K of the strike may
 // jump under that rounding; the jump's size (|discp|, or the indicator's
 // factor) enters mag with the weight 2^18, so that the bound covers it whole.
-__device__ __forceinline__ float inner_exact_mix(const MixRec& q, bool anti, bool cv_on, float R, float z0,
-                                                 float z1, float& mag) {
+__device__ __forceinline__ float inner_exact_mix(const MixRec& q, const MixStock& st, bool anti, bool cv_on,
+                                                 float R, float z0, float z1, float& mag) {
     const int type = (int)(q.stype & 0xffu) - 2;
-    const float gT = __expf(q.sigT * z1 + q.driftT);
-    const float S1 = q.S0e * __expf(q.sigtau * z0) * gT;
-    const float S2 = anti ? q.S0e * __expf(-q.sigtau * z0) * gT : S1;
+    const float gT = __expf(st.sig * __fsqrt_rn(q.Tt) * z1 + st.dcoef * q.Tt);
+    const float S1 = st.S0e * __expf(st.sigtau * z0) * gT;
+    const float S2 = anti ? st.S0e * __expf(-st.sigtau * z0) * gT : S1;
     const float St = R * gT;
     const float pt = payoff_f(type, St, q.K);
     const float val1 = q.discp * (payoff_f(type, S1, q.K) - pt);
@@ -459,11 +460,11 @@ __device__ __forceinline__ float inner_exact_mix(const MixRec& q, bool anti, boo
     float dv1 = 0.f, dv2 = 0.f;
     const bool digital = is_digital(type);
     const bool cv = cv_on && !digital;
-    const float diff = R - q.S0;
+    const float diff = R - st.S0;
     if (cv) {
         const float c = (float)(type == 1);
-        dv1 = diff * q.discp * __fdividef(S1, q.S0) * (c - (float)(S1 < q.K));
-        dv2 = anti ? diff * q.discp * __fdividef(S2, q.S0) * (c - (float)(S2 < q.K)) : dv1;
+        dv1 = diff * q.discp * __fdividef(S1, st.S0) * (c - (float)(S1 < q.K));
+        dv2 = anti ? diff * q.discp * __fdividef(S2, st.S0) * (c - (float)(S2 < q.K)) : dv1;
     }
     mag = 0.5f * (fabsf(val1) + fabsf(val2) + fabsf(dv1) + fabsf(dv2)) + fabsf(q.discp) * (S1 + S2 + St);
     const float win = 0x1p-17f * q.K;
@@ -472,7 +473,7 @@ __device__ __forceinline__ float inner_exact_mix(const MixRec& q, bool anti, boo
         const float at_risk = (float)near1 + (float)near2 + 2.f * (float)neart;
         mag += 0x1p18f * 0.5f * fabsf(q.discp) * at_risk;
     } else if (cv) {
-        const float jump = fabsf(diff * q.discp) * __fdividef(q.K, q.S0) * 1.001f;
+        const float jump = fabsf(diff * q.discp) * __fdividef(q.K, st.S0) * 1.001f;
         mag += 0x1p18f * 0.5f * jump * ((float)near1 + (float)near2);
     }
     return 0.5f * (val1 + val2 + dv1 + dv2);

@@ -137,9 +137,11 @@ int Engine::stage_portfolio(const portfolio_data* data) {
                 at[b] = (uint32_t)i;
             }
             at[nb] = (uint32_t)(n - 1);  // the last bucket is open above
-            std::vector<uint2> guide(nb);
-            for (uint64_t b = 0; b < nb; b++) guide[b] = make_uint2(at[b], at[b + 1]);
-            if ((rc = upload(*this, guide.data(), nb, &P.guides[m]))) return rc;
+            if (n > kGuideLo) return set_error(NMLMC_ERR_ARG, "options[%d]: too many options", m);
+            std::vector<uint32_t> guide(nb + 1);
+            for (uint64_t b = 0; b < nb; b++) guide[b] = at[b] | (at[b + 1] != at[b] ? kGuideMulti : 0u);
+            guide[nb] = at[nb];
+            if ((rc = upload(*this, guide.data(), nb + 1, &P.guides[m]))) return rc;
             P.guide_scale[m] = scale;
             P.guide_nb[m] = (uint32_t)nb;
         }
@@ -171,26 +173,33 @@ int Engine::stage_portfolio(const portfolio_data* data) {
     P.use_milstein = data->use_milstein;
     if (data->options[2].size && (data->dt_beta < 2 || !(data->dt_prob_expo < 0) || data->dt_h0inv == 0))
         return set_error(NMLMC_ERR_ARG, "Unbiased options need dt_beta >= 2, dt_prob_expo < 0, dt_h0inv > 0");
-    // FP32 records of the Exact options for the mixed form.
+    // FP32 records of the Exact options and of the stocks for the mixed form.
     {
         const option_params* ex = (const option_params*)data->options[1].data;
         const size_t n1 = data->options[1].size;
         std::vector<MixRec> mix(n1);
         for (size_t j = 0; j < n1; j++) {
             const option_params& o = ex[j];
-            const stock_params& st = stocks[o.stock_ind];
-            const double drift = data->r - 0.5 * st.sigma * st.sigma;
             MixRec& q = mix[j];
             q.K = (float)o.strike;
-            q.sigT = (float)(st.sigma * std::sqrt(o.maturity - data->tau));
-            q.driftT = (float)(drift * (o.maturity - data->tau));
+            q.Tt = (float)(o.maturity - data->tau);
             q.discp = (float)(o.w * std::exp(-data->r * o.maturity) * P.prob_norm / o.prob);
-            q.sigtau = (float)(st.sigma * std::sqrt(data->tau));
-            q.S0e = (float)(st.S0 * std::exp(drift * data->tau));
-            q.S0 = (float)st.S0;
             q.stype = (o.stock_ind << 8) | (uint32_t)(o.type + 2);
         }
         if ((rc = upload(*this, mix.data(), n1, &P.mix))) return rc;
+        std::vector<MixStock> ms(d);
+        for (uint32_t i = 0; i < d; i++) {
+            const stock_params& st = stocks[i];
+            const double dcoef = data->r - 0.5 * st.sigma * st.sigma;
+            MixStock& q = ms[i];
+            std::memset(&q, 0, sizeof(q));
+            q.sig = (float)st.sigma;
+            q.dcoef = (float)dcoef;
+            q.sigtau = (float)(st.sigma * std::sqrt(data->tau));
+            q.S0e = (float)(st.S0 * std::exp(dcoef * data->tau));
+            q.S0 = (float)st.S0;
+        }
+        if ((rc = upload(*this, ms.data(), d, &P.mix_stocks))) return rc;
     }
     P.w_est_const = -1.;
     {
