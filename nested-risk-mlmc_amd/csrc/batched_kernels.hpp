@@ -10,6 +10,12 @@
 
 namespace nmlmc {
 
+// Waves per SIMD the mixed-form test and final kernels are compiled for (their
+// inner loops wait on gathers, so occupancy matters more than a few spills).
+#ifndef NMLMC_MIXED_WAVES
+#define NMLMC_MIXED_WAVES 4
+#endif
+
 __device__ __forceinline__ uint32_t class_of(uint64_t n, uint64_t base_min) {
     uint32_t c = 0;
     while ((base_min << c) < n && c + 1 < (uint32_t)kMaxClasses) c++;
@@ -73,7 +79,7 @@ __global__ __launch_bounds__(256) void k_b_init(typename P::Args A) {
 
 // Test round r of every outer sample still refining its fine or coarse count.
 template <class P, bool MIXED, int RULE>
-__global__ __launch_bounds__(256) void k_b_test(typename P::Args A, uint32_t r) {
+__global__ __launch_bounds__(256, MIXED ? NMLMC_MIXED_WAVES : 1) void k_b_test(typename P::Args A, uint32_t r) {
     const BatchedCommon& c = A.c;
     LaneAcc acc;
     acc.zero();
@@ -445,12 +451,55 @@ __global__ __launch_bounds__(256) void k_b_redo_seq(typename P::Args A) {
     }
 }
 
+// The final draw groups of an outer sample small enough for one lane (at most
+// S units in all): phi_f, phi_c in FP64 or (MIXED) FP32 arithmetic, with the
+// work and draw counts.  Returns true, in the mixed form, when a mean came
+// within the FP32 bound of the Heaviside threshold (the sample is then redone
+// in FP64 instead of being finished).
+template <class P, bool MIXED>
+__device__ __forceinline__ bool whole_final(const typename P::Args& A, uint32_t m, const ItemState& it,
+                                            LaneAcc& acc, double& phi_f, double& phi_c) {
+    const BatchedCommon& c = A.c;
+    const FinalShape sh = final_shape(c, it);
+    bool near = false;
+    if (c.ell > 0 && !c.ep.antithetic) {
+        const SegAcc a = lane_group<P, MIXED>(A, m, it, TAG_FINAL, 0, it.nf);
+        const SegAcc b = lane_group<P, MIXED>(A, m, it, TAG_FINAL_COARSE, 0, it.nc);
+        acc.work += a.work + b.work + 2 * P::group_work(A);
+        acc.st[4] += a.nu + b.nu;
+        acc.st[5] += a.nn + b.nn;
+        const double mf = it.start + a.s1 / (double)it.nf, mc = it.start + b.s1 / (double)it.nc;
+        near = MIXED && (fabs(mf) <= P::mixed_bound(a.sab / (double)it.nf, it.start) ||
+                         fabs(mc) <= P::mixed_bound(b.sab / (double)it.nc, it.start));
+        outer_phi(c, it, 0., 0., phi_scalar(c.phi_type, mf, 1., 0.), phi_scalar(c.phi_type, mc, 1., 0.), phi_f,
+                  phi_c);
+    } else {
+        const double coeff = (double)(1.f / (float)sh.groups);
+        double s1_big = 0., sab_big = 0., phi_small = 0.;
+        for (uint32_t g = 0; g < sh.groups; g++) {
+            const SegAcc a = lane_group<P, MIXED>(A, m, it, TAG_FINAL, (uint64_t)g * sh.n, (uint64_t)(g + 1) * sh.n);
+            acc.work += a.work + P::group_work(A);
+            acc.st[4] += a.nu;
+            acc.st[5] += a.nn;
+            const double mean = it.start + a.s1 / (double)sh.n;
+            if (MIXED) near |= fabs(mean) <= P::mixed_bound(a.sab / (double)sh.n, it.start);
+            phi_small = phi_scalar(c.phi_type, mean, coeff, phi_small);
+            s1_big += a.s1;
+            sab_big += a.sab;
+        }
+        if (MIXED)
+            near |= fabs(it.start + s1_big / (double)sh.big) <= P::mixed_bound(sab_big / (double)sh.big, it.start);
+        outer_phi(c, it, s1_big, phi_small, 0., 0., phi_f, phi_c);
+    }
+    return MIXED && near && c.phi_type == PHI_INDICATOR;
+}
+
 // Final phase: every outer sample's final draw groups, in FP64 or (MIXED) FP32
 // arithmetic.  In the mixed form an outer sample whose mean came within the
 // FP32 bound of the Heaviside threshold (or met a payoff discontinuity) is not
 // finished here but listed for the FP64 redo.
 template <class P, bool MIXED>
-__global__ __launch_bounds__(256) void k_b_final(typename P::Args A) {
+__global__ __launch_bounds__(256, MIXED ? NMLMC_MIXED_WAVES : 1) void k_b_final(typename P::Args A) {
     const BatchedCommon& c = A.c;
     LaneAcc acc;
     acc.zero();
@@ -478,40 +527,8 @@ __global__ __launch_bounds__(256) void k_b_final(typename P::Args A) {
             const FinalEntry& fe = c.fpool[whole_off + (k - total_split)];
             const uint32_t m = fe.item;
             ItemState& it = c.items[m];
-            const FinalShape sh = final_shape(c, it);
-            double phi_f, phi_c, s1_big = 0., sab_big = 0., phi_small = 0.;
-            bool near = false;
-            if (plain2) {
-                const SegAcc a = lane_group<P, MIXED>(A, m, it, TAG_FINAL, 0, it.nf);
-                const SegAcc b = lane_group<P, MIXED>(A, m, it, TAG_FINAL_COARSE, 0, it.nc);
-                acc.work += a.work + b.work + 2 * P::group_work(A);
-                acc.st[4] += a.nu + b.nu;
-                acc.st[5] += a.nn + b.nn;
-                const double mf = it.start + a.s1 / (double)it.nf, mc = it.start + b.s1 / (double)it.nc;
-                near = MIXED && (fabs(mf) <= P::mixed_bound(a.sab / (double)it.nf, it.start) ||
-                                 fabs(mc) <= P::mixed_bound(b.sab / (double)it.nc, it.start));
-                outer_phi(c, it, 0., 0., phi_scalar(c.phi_type, mf, 1., 0.), phi_scalar(c.phi_type, mc, 1., 0.),
-                          phi_f, phi_c);
-            } else {
-                const double coeff = (double)(1.f / (float)sh.groups);
-                for (uint32_t g = 0; g < sh.groups; g++) {
-                    const SegAcc a = lane_group<P, MIXED>(A, m, it, TAG_FINAL, (uint64_t)g * sh.n,
-                                                          (uint64_t)(g + 1) * sh.n);
-                    acc.work += a.work + P::group_work(A);
-                    acc.st[4] += a.nu;
-                    acc.st[5] += a.nn;
-                    const double mean = it.start + a.s1 / (double)sh.n;
-                    if (MIXED) near |= fabs(mean) <= P::mixed_bound(a.sab / (double)sh.n, it.start);
-                    phi_small = phi_scalar(c.phi_type, mean, coeff, phi_small);
-                    s1_big += a.s1;
-                    sab_big += a.sab;
-                }
-                if (MIXED)
-                    near |= fabs(it.start + s1_big / (double)sh.big) <=
-                            P::mixed_bound(sab_big / (double)sh.big, it.start);
-                outer_phi(c, it, s1_big, phi_small, 0., 0., phi_f, phi_c);
-            }
-            if (MIXED && near && c.phi_type == PHI_INDICATOR) {
+            double phi_f, phi_c;
+            if (whole_final<P, MIXED>(A, m, it, acc, phi_f, phi_c)) {
                 it.phi32_f = phi_f;
                 it.phi32_c = phi_c;
                 reeval = true;

@@ -12,7 +12,9 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "lib", "libnmlmc.so")
+# NMLMC_LIBRARY selects another build of the library (e.g. one compiled with other
+# tuning macros); by default the in-tree build.
+LIB_PATH = os.environ.get("NMLMC_LIBRARY") or os.path.join(HERE, "lib", "libnmlmc.so")
 
 FORM_PLAIN, FORM_BATCHED, FORM_MIXED = 0, 1, 2
 PHI_INDICATOR, PHI_MAX, PHI_MOMENTS, PHI_FMOMENTS = 0, 1, 2, 3
