@@ -95,52 +95,66 @@ struct Kahan {
     __device__ __forceinline__ double value() const { return (double)s - (double)c; }
 };
 
-// Per-lane level accumulators, flushed once per block.  They are touched once
-// per finished segment or outer sample; the runtime-indexed arrays stay in
-// scratch memory rather than taking registers from the inner loops.
+// Per-lane level accumulators, flushed once per block.  The batched forms
+// evaluate one scalar phi per outer sample, so kMaxMoments fine and difference
+// moments suffice.  Every index into these arrays is a compile-time constant
+// (add_lane_moments and block_flush unroll their loops), so they live in
+// registers: the fields a kernel writes cost a register each and the others
+// stay constant zeros.  (Runtime-indexed arrays would go to scratch memory,
+// and zeroing and flushing ~330 B per lane over a grid of 2^19 lanes costs
+// ~70 us per launch whatever the work.)
 struct LaneAcc {
-    double f[kMaxPhi * kMaxMoments];
-    double d[kMaxPhi * kMaxMoments];
+    double f[kMaxMoments];
+    double d[kMaxMoments];
     double work;
     uint64_t st[kStatsExt];
-    __device__ void zero() {
-        for (int i = 0; i < kMaxPhi * kMaxMoments; i++) f[i] = d[i] = 0.;
+    __device__ __forceinline__ void zero() {
+#pragma unroll
+        for (int i = 0; i < kMaxMoments; i++) f[i] = d[i] = 0.;
         work = 0.;
+#pragma unroll
         for (int i = 0; i < kStatsExt; i++) st[i] = 0;
     }
 };
 
-// Adds every lane's accumulators into the level accumulator: a block-level
-// reduction, then one atomic per field.  All threads of the block must call it.
+// add_moments for one scalar phi (nested_sampler.hpp:87-100): out[j] += phi^(j+1)
+// for j < moments, the powers formed by the same successive products.
+__device__ __forceinline__ void add_lane_moments(double phi, uint32_t moments, double* out) {
+    double p = 1.;
+#pragma unroll
+    for (int j = 0; j < kMaxMoments; j++) {
+        p *= phi;
+        if ((uint32_t)j < moments) out[j] += p;
+    }
+}
+
+// Adds every lane's accumulators into the level accumulator: wave sums of all
+// fields into LDS, one barrier, then one thread per field sums the four waves
+// and issues one atomic.  All threads of the block must call it.
 __device__ __forceinline__ void block_flush(const LaneAcc& a, Partial* out) {
-    __shared__ double sd[4];
-    __shared__ uint64_t su[4];
+    constexpr int ND = 2 * kMaxMoments + 1;
+    __shared__ double sd[ND][4];
+    __shared__ uint64_t su[kStatsExt][4];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    constexpr int nsum = kMaxPhi * kMaxMoments;
-    for (int k = 0; k < kPartialFields; k++) {
-        const bool is_u = k >= 2 * nsum + 1;
-        if (is_u) {
-            uint64_t u = wave_sum_u64(a.st[k - (2 * nsum + 1)]);
-            if (lane == 0) su[wave] = u;
-        } else {
-            double v = k < nsum ? a.f[k] : (k < 2 * nsum ? a.d[k - nsum] : a.work);
-            v = wave_sum(v);
-            if (lane == 0) sd[wave] = v;
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            if (is_u) {
-                const uint64_t t = su[0] + su[1] + su[2] + su[3];
-                if (t) atomicAdd((unsigned long long*)&out->stats[k - (2 * nsum + 1)], (unsigned long long)t);
-            } else {
-                const double t = ((sd[0] + sd[1]) + sd[2]) + sd[3];
-                if (t != 0.) {
-                    double* dst = k < nsum ? &out->f[k] : (k < 2 * nsum ? &out->d[k - nsum] : &out->work);
-                    atomicAdd(dst, t);
-                }
-            }
-        }
-        __syncthreads();
+#pragma unroll
+    for (int k = 0; k < ND; k++) {
+        const double v = wave_sum(k < kMaxMoments ? a.f[k] : (k < 2 * kMaxMoments ? a.d[k - kMaxMoments] : a.work));
+        if (lane == 0) sd[k][wave] = v;
+    }
+#pragma unroll
+    for (int k = 0; k < kStatsExt; k++) {
+        const uint64_t v = wave_sum_u64(a.st[k]);
+        if (lane == 0) su[k][wave] = v;
+    }
+    __syncthreads();
+    const int t = threadIdx.x;
+    if (t < ND) {
+        const double s = ((sd[t][0] + sd[t][1]) + sd[t][2]) + sd[t][3];
+        if (s != 0.) atomicAdd(t < kMaxMoments ? &out->f[t] : (t < 2 * kMaxMoments ? &out->d[t - kMaxMoments] : &out->work), s);
+    } else if (t >= 64 && t < 64 + kStatsExt) {
+        const int k = t - 64;
+        const uint64_t s = su[k][0] + su[k][1] + su[k][2] + su[k][3];
+        if (s) atomicAdd((unsigned long long*)&out->stats[k], (unsigned long long)s);
     }
 }
 

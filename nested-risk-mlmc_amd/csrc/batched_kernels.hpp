@@ -286,11 +286,8 @@ __global__ __launch_bounds__(256) void k_b_final_fill(typename P::Args A) {
 // Records a finished outer sample: moments, trace (the reference's lines 446-451).
 __device__ __forceinline__ void finish_outer(const BatchedCommon& c, uint64_t m, const ItemState& it,
                                              double phi_f, double phi_c, LaneAcc& acc) {
-    add_moments(&phi_f, 1, acc.f, c.ep.fine_moments);
-    if (c.ell > 0) {
-        const double dphi = phi_f - phi_c;
-        add_moments(&dphi, 1, acc.d, c.ep.diff_moments);
-    }
+    add_lane_moments(phi_f, c.ep.fine_moments, acc.f);
+    if (c.ell > 0) add_lane_moments(phi_f - phi_c, c.ep.diff_moments, acc.d);
     if (c.tr_phi_f) c.tr_phi_f[m] = phi_f;
     if (c.tr_phi_c) c.tr_phi_c[m] = c.ell == 0 ? 0. : phi_c;
     if (c.tr_nf) c.tr_nf[m] = it.nf;
@@ -518,7 +515,6 @@ __global__ __launch_bounds__(256, MIXED ? NMLMC_MIXED_WAVES : 1) void k_b_final(
     const uint32_t whole_off = c.ctl->class_off[kMaxClasses];
     const uint64_t total = total_split + n_whole;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    const bool plain2 = c.ell > 0 && !c.ep.antithetic;
     for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < total; k += stride) {
         bool reeval = false;
         uint32_t m_done = 0;
