@@ -73,7 +73,7 @@ struct BatchedCtl {
     uint32_t final_total;
     uint32_t nreeval;
     uint32_t overflow;
-    uint32_t pad;
+    uint32_t n_pend;  // fused first pass: outer samples left for the final phase
     Partial acc;  // level sums, work and stats (atomics)
 };
 
@@ -95,51 +95,71 @@ struct Kahan {
     __device__ __forceinline__ double value() const { return (double)s - (double)c; }
 };
 
-// Per-lane level accumulators, flushed once per block.  The batched forms
-// evaluate one scalar phi per outer sample, so kMaxMoments fine and difference
-// moments suffice.  Every index into these arrays is a compile-time constant
-// (add_lane_moments and block_flush unroll their loops), so they live in
-// registers: the fields a kernel writes cost a register each and the others
-// stay constant zeros.  (Runtime-indexed arrays would go to scratch memory,
-// and zeroing and flushing ~330 B per lane over a grid of 2^19 lanes costs
-// ~70 us per launch whatever the work.)
+// Per-lane level accumulators, flushed once per block.  The work and the
+// draw statistics change once per segment; every index into st is a
+// compile-time constant, so they live in registers (runtime-indexed arrays
+// would go to scratch memory, and zeroing and flushing them over a grid of 2^19
+// lanes costs ~70 us per launch whatever the work).  The moment sums change
+// once per finished outer sample and live in LDS (lane_moments), so they take
+// no registers from the inner loops.  Every kernel using them runs blocks of
+// 256 threads.
 struct LaneAcc {
-    double f[kMaxMoments];
-    double d[kMaxMoments];
     double work;
     uint64_t st[kStatsExt];
     __device__ __forceinline__ void zero() {
-#pragma unroll
-        for (int i = 0; i < kMaxMoments; i++) f[i] = d[i] = 0.;
         work = 0.;
 #pragma unroll
         for (int i = 0; i < kStatsExt; i++) st[i] = 0;
     }
 };
 
-// add_moments for one scalar phi (nested_sampler.hpp:87-100): out[j] += phi^(j+1)
-// for j < moments, the powers formed by the same successive products.
-__device__ __forceinline__ void add_lane_moments(double phi, uint32_t moments, double* out) {
+// Moment sums of the block's lanes: the batched forms evaluate one scalar phi
+// per outer sample, so kMaxMoments fine and kMaxMoments difference sums; sum
+// j of lane t at [j * 256 + t] (fine j < kMaxMoments, then difference).
+__device__ __forceinline__ double* lane_moments() {
+    __shared__ double mom[2 * kMaxMoments * 256];
+    return mom;
+}
+__device__ __forceinline__ void zero_lane_moments() {
+    double* m = lane_moments() + threadIdx.x;
+#pragma unroll
+    for (int j = 0; j < 2 * kMaxMoments; j++) m[j * 256] = 0.;
+}
+
+// add_moments for one scalar phi (nested_sampler.hpp:87-100): sum j += phi^(j+1)
+// for j < moments, the powers formed by the same successive products; diff
+// selects the difference sums.
+__device__ __forceinline__ void add_lane_moments(double phi, uint32_t moments, bool diff) {
+    double* m = lane_moments() + (diff ? kMaxMoments * 256 : 0) + threadIdx.x;
     double p = 1.;
 #pragma unroll
     for (int j = 0; j < kMaxMoments; j++) {
         p *= phi;
-        if ((uint32_t)j < moments) out[j] += p;
+        if ((uint32_t)j < moments) m[j * 256] += p;
     }
 }
 
-// Adds every lane's accumulators into the level accumulator: wave sums of all
-// fields into LDS, one barrier, then one thread per field sums the four waves
-// and issues one atomic.  All threads of the block must call it.
+// Adds every lane's accumulators (and with MOM its moment sums) into the level
+// accumulator: wave sums of all fields into LDS, one barrier, then one thread
+// per field sums the four waves and issues one atomic.  All threads of the
+// block must call it.
+template <bool MOM>
 __device__ __forceinline__ void block_flush(const LaneAcc& a, Partial* out) {
     constexpr int ND = 2 * kMaxMoments + 1;
     __shared__ double sd[ND][4];
     __shared__ uint64_t su[kStatsExt][4];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (MOM) {
+        const double* m = lane_moments() + threadIdx.x;
 #pragma unroll
-    for (int k = 0; k < ND; k++) {
-        const double v = wave_sum(k < kMaxMoments ? a.f[k] : (k < 2 * kMaxMoments ? a.d[k - kMaxMoments] : a.work));
-        if (lane == 0) sd[k][wave] = v;
+        for (int k = 0; k < 2 * kMaxMoments; k++) {
+            const double v = wave_sum(m[k * 256]);
+            if (lane == 0) sd[k][wave] = v;
+        }
+    }
+    {
+        const double v = wave_sum(a.work);
+        if (lane == 0) sd[ND - 1][wave] = v;
     }
 #pragma unroll
     for (int k = 0; k < kStatsExt; k++) {
@@ -148,7 +168,7 @@ __device__ __forceinline__ void block_flush(const LaneAcc& a, Partial* out) {
     }
     __syncthreads();
     const int t = threadIdx.x;
-    if (t < ND) {
+    if (t < ND && (MOM || t == ND - 1)) {
         const double s = ((sd[t][0] + sd[t][1]) + sd[t][2]) + sd[t][3];
         if (s != 0.) atomicAdd(t < kMaxMoments ? &out->f[t] : (t < 2 * kMaxMoments ? &out->d[t - kMaxMoments] : &out->work), s);
     } else if (t >= 64 && t < 64 + kStatsExt) {
@@ -216,6 +236,8 @@ struct BatchedCommon {
     FinalEntry* fpool;
     uint32_t fcap;
     uint32_t* reeval;
+    uint32_t* pend;      // [M] outer samples the fused first pass leaves to the final phase
+    uint32_t fused;      // 1: k_b_fused ran (round-0 tests in-lane; final phase over pend)
     double* redo_acc;    // [kRedoSlots x kRedoGroups] FP64 group sums of the redo
     uint32_t* redo_cnt;  // [kRedoSlots] parts of the redo done
     BatchedCtl* ctl;

@@ -1,10 +1,11 @@
 // Kernels of the batched engine (see batched.hpp), templated on the problem
 // policy P and on the precision of the inner arithmetic.
 //
-// Launch order of one level call (host: Engine::run_batched_level):
-//   k_b_zero, [P's outer-draw kernel], k_b_init, k_b_test(r) for r = 0..R-1,
-//   k_b_final_count, k_b_final_alloc, k_b_final_fill, k_b_final, [k_b_redo_split
-//   and k_b_redo_seq when MIXED]
+// Launch order of one level call (launch_batched_level):
+//   k_b_zero, [P's outer-draw kernel], k_b_init and k_b_test(r) for r = 0..R-1
+//   (or k_b_fused and k_b_test(r) for r = 1..R-1), k_b_final_count,
+//   k_b_final_alloc, k_b_final_fill, k_b_final, [k_b_redo_split and
+//   k_b_redo_seq when MIXED]
 #pragma once
 #include "batched.hpp"
 
@@ -34,6 +35,25 @@ __device__ __forceinline__ bool refine(const BatchedCommon& c, const LevelCounts
     const double delta2 = dist * dist / out_var;
     const double new_count = (RULE == 1) ? lc.tmp / delta2 : pow(delta2, -c.ep.adaptive_r / 2) * lc.tmp;
     return !(new_count <= (double)cur);
+}
+
+// The count an outer sample moves to after a finished test group at `cur`
+// (get_samples_count's loop body, nested_sampler.hpp:252-270): 2 cur with
+// more = true while it refines, else the decided count (lc.max once the next
+// doubling would reach it).
+template <int RULE>
+__device__ __forceinline__ uint64_t next_count(const BatchedCommon& c, const LevelCounts& lc, double start,
+                                               uint64_t cur, double s1, double s2, bool& more) {
+    uint64_t n = cur;
+    more = refine<RULE>(c, lc, start, cur, s1, s2);
+    if (more) {
+        n = cur * 2;
+        if (lc.max <= 2 * n) {
+            n = lc.max;
+            more = false;
+        }
+    }
+    return n;
 }
 
 // Outer draws.  Round 0 tests every outer sample, so its queue is implicit
@@ -74,7 +94,7 @@ __global__ __launch_bounds__(256) void k_b_init(typename P::Args A) {
         if (acc_f) c.tf[0][m] = TestEntry{0., 0., 0., (uint32_t)m, 0};
         if (acc_c) c.tc[0][m] = TestEntry{0., 0., 0., (uint32_t)m, 0};
     }
-    block_flush(acc, &c.ctl->acc);
+    block_flush<false>(acc, &c.ctl->acc);
 }
 
 // Test round r of every outer sample still refining its fine or coarse count.
@@ -122,15 +142,8 @@ __global__ __launch_bounds__(256, MIXED ? NMLMC_MIXED_WAVES : 1) void k_b_test(t
         if (complete) {
             acc.work += P::group_work(A);
             const LevelCounts& lc = fine ? c.cf : c.cc;
-            uint64_t n = cur;
-            bool more = refine<RULE>(c, lc, it.start, cur, s1, s2);
-            if (more) {
-                n = cur * 2;
-                if (lc.max <= 2 * n) {
-                    n = lc.max;
-                    more = false;
-                }
-            }
+            bool more;
+            const uint64_t n = next_count<RULE>(c, lc, it.start, cur, s1, s2, more);
             if (more) {
                 append = true;
                 acc.st[0] += n;
@@ -154,7 +167,7 @@ __global__ __launch_bounds__(256, MIXED ? NMLMC_MIXED_WAVES : 1) void k_b_test(t
             else c.tc[(r + 1) & 1][sc] = TestEntry{0., 0., 0., m, 0};
         }
     }
-    block_flush(acc, &c.ctl->acc);
+    block_flush<false>(acc, &c.ctl->acc);
 }
 
 // Final draw groups of an outer sample: count and shape.
@@ -201,7 +214,9 @@ __global__ __launch_bounds__(256) void k_b_final_count(typename P::Args A) {
     LaneAcc acc;
     acc.zero();
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t m = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; m < c.M; m += stride) {
+    const uint64_t n_items = c.fused ? aload_u32(&c.ctl->n_pend) : c.M;
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n_items; k += stride) {
+        const uint64_t m = c.fused ? c.pend[k] : k;
         const ItemState& it = c.items[m];
         const FinalShape s = final_shape(c, it);
         const bool plain2 = c.ell > 0 && !c.ep.antithetic;
@@ -219,7 +234,7 @@ __global__ __launch_bounds__(256) void k_b_final_count(typename P::Args A) {
     for (int i = threadIdx.x; i < kMaxClasses; i += blockDim.x)
         if (hist[i]) atomicAdd(&c.ctl->class_cnt[i], hist[i]);
     if (threadIdx.x == 0 && hist[kMaxClasses]) atomicAdd(&c.ctl->n_whole, hist[kMaxClasses]);
-    block_flush(acc, &c.ctl->acc);
+    block_flush<false>(acc, &c.ctl->acc);
 }
 
 // Writes each outer sample's final entries: whole samples after the class
@@ -232,13 +247,15 @@ __global__ __launch_bounds__(256) void k_b_final_fill(typename P::Args A) {
     __shared__ uint32_t cnt[kMaxClasses + 1], base[kMaxClasses + 1];
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     const uint32_t whole_off = c.ctl->class_off[kMaxClasses];
-    for (uint64_t b0 = (uint64_t)blockIdx.x * blockDim.x; b0 < c.M; b0 += stride) {
+    const uint64_t n_items = c.fused ? aload_u32(&c.ctl->n_pend) : c.M;
+    for (uint64_t b0 = (uint64_t)blockIdx.x * blockDim.x; b0 < n_items; b0 += stride) {
         for (int i = threadIdx.x; i <= kMaxClasses; i += blockDim.x) cnt[i] = 0;
         __syncthreads();
-        const uint64_t m = b0 + threadIdx.x;
+        const uint64_t k = b0 + threadIdx.x;
         FinalShape s{};
         uint32_t cls_a = 0, cls_b = 0, rank_a = 0, rank_b = 0;
-        const bool valid = m < c.M;
+        const bool valid = k < n_items;
+        const uint64_t m = valid ? (c.fused ? c.pend[k] : k) : 0;
         const bool plain2 = c.ell > 0 && !c.ep.antithetic;
         bool whole = false;
         if (valid) {
@@ -286,8 +303,8 @@ __global__ __launch_bounds__(256) void k_b_final_fill(typename P::Args A) {
 // Records a finished outer sample: moments, trace (the reference's lines 446-451).
 __device__ __forceinline__ void finish_outer(const BatchedCommon& c, uint64_t m, const ItemState& it,
                                              double phi_f, double phi_c, LaneAcc& acc) {
-    add_lane_moments(phi_f, c.ep.fine_moments, acc.f);
-    if (c.ell > 0) add_lane_moments(phi_f - phi_c, c.ep.diff_moments, acc.d);
+    add_lane_moments(phi_f, c.ep.fine_moments, false);
+    if (c.ell > 0) add_lane_moments(phi_f - phi_c, c.ep.diff_moments, true);
     if (c.tr_phi_f) c.tr_phi_f[m] = phi_f;
     if (c.tr_phi_c) c.tr_phi_c[m] = c.ell == 0 ? 0. : phi_c;
     if (c.tr_nf) c.tr_nf[m] = it.nf;
@@ -491,6 +508,19 @@ __device__ __forceinline__ bool whole_final(const typename P::Args& A, uint32_t 
     return MIXED && near && c.phi_type == PHI_INDICATOR;
 }
 
+// Lists an outer sample for the FP64 redo (mixed form) and clears its redo
+// slot; every lane of the wave that reaches it must call it.
+__device__ __forceinline__ void list_reeval(const BatchedCommon& c, bool reeval, uint32_t m) {
+    const uint32_t slot = wave_append(&c.ctl->nreeval, reeval);
+    if (reeval) {
+        c.reeval[slot] = m;
+        if (slot < kRedoSlots) {
+            for (uint32_t g = 0; g < kRedoGroups; g++) astore(&c.redo_acc[(uint64_t)slot * kRedoGroups + g], 0.);
+            __hip_atomic_exchange(&c.redo_cnt[slot], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
 // Final phase: every outer sample's final draw groups, in FP64 or (MIXED) FP32
 // arithmetic.  In the mixed form an outer sample whose mean came within the
 // FP32 bound of the Heaviside threshold (or met a payoff discontinuity) is not
@@ -500,8 +530,9 @@ __global__ __launch_bounds__(256, MIXED ? NMLMC_MIXED_WAVES : 1) void k_b_final(
     const BatchedCommon& c = A.c;
     LaneAcc acc;
     acc.zero();
+    zero_lane_moments();
     if (c.ctl->overflow) {
-        block_flush(acc, &c.ctl->acc);
+        block_flush<true>(acc, &c.ctl->acc);
         return;
     }
     // Split entries (per size class, largest first in memory order of classes)
@@ -608,29 +639,139 @@ __global__ __launch_bounds__(256, MIXED ? NMLMC_MIXED_WAVES : 1) void k_b_final(
                 }
             }
         }
-        if (MIXED) {
-            const uint32_t slot = wave_append(&c.ctl->nreeval, reeval);
-            if (reeval) {
-                c.reeval[slot] = m_done;
-                if (slot < kRedoSlots) {
-                    for (uint32_t g = 0; g < kRedoGroups; g++) astore(&c.redo_acc[(uint64_t)slot * kRedoGroups + g], 0.);
-                    __hip_atomic_exchange(&c.redo_cnt[slot], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                }
+        if (MIXED) list_reeval(c, reeval, m_done);
+    }
+    block_flush<true>(acc, &c.ctl->acc);
+}
+
+// Fused first pass, for level calls whose round-0 test groups fit one segment:
+// the outer draw (k_b_init), the round-0 fine and coarse tests (k_b_test at
+// r = 0) and, for an outer sample whose counts are both decided there and
+// whose final fits one segment, its final draws and moments (k_b_final), all
+// in the lane that draws the outer sample, with the item state in registers.
+// Only outer samples that refine (queued for round 1), that have a larger
+// final (both listed in pend for the final phase) or that need the FP64 redo
+// (MIXED) are written out.  Draws, decisions and counts are those of the
+// unfused sequence.
+template <class P, bool MIXED, int RULE>
+__global__ __launch_bounds__(256, MIXED ? NMLMC_MIXED_WAVES : 1) void k_b_fused(typename P::Args A) {
+    const BatchedCommon& c = A.c;
+    LaneAcc acc;
+    acc.zero();
+    zero_lane_moments();
+    const bool test_f = c.cf.max > 2 * c.cf.base;
+    const bool test_c = c.ell > 0 && c.cc.max > 2 * c.cc.base;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t m = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; m < c.M; m += stride) {
+        ItemState it;
+        P::init_item(A, m, it, acc);
+        it.nf = it.nc = 0;
+        bool app_f = false, app_c = false;
+        if (!test_f) {
+            it.nf = c.cf.max;
+            acc.st[1] += 1;
+            acc.st[2] += 1;
+            acc.st[3] += (c.cf.max == c.cf.base);
+        } else {
+            acc.st[0] += c.cf.base;
+            const SegAcc t = lane_group<P, MIXED>(A, (uint32_t)m, it, TAG_FINE_TEST, 0, c.cf.base);
+            acc.work += t.work + P::group_work(A);
+            acc.st[4] += t.nu;
+            acc.st[5] += t.nn;
+            const uint64_t n = next_count<RULE>(c, c.cf, it.start, c.cf.base, t.s1, t.s2, app_f);
+            if (app_f) {
+                acc.st[0] += n;
+            } else {
+                it.nf = n;
+                acc.st[1] += 1;
+                acc.st[2] += (n == c.cf.max);
+                acc.st[3] += (n == c.cf.base);
             }
         }
+        if (c.ell > 0) {
+            if (!test_c) {
+                it.nc = c.cc.max;
+            } else {
+                acc.st[0] += c.cc.base;
+                const SegAcc t = lane_group<P, MIXED>(A, (uint32_t)m, it, TAG_COARSE_TEST, 0, c.cc.base);
+                acc.work += t.work + P::group_work(A);
+                acc.st[4] += t.nu;
+                acc.st[5] += t.nn;
+                const uint64_t n = next_count<RULE>(c, c.cc, it.start, c.cc.base, t.s1, t.s2, app_c);
+                if (app_c) acc.st[0] += n;
+                else it.nc = n;
+            }
+        }
+        bool pending = app_f || app_c, reeval = false;
+        if (!pending) {
+            const FinalShape sh = final_shape(c, it);
+            if (final_units<P>(c, it, sh) <= c.S) {
+                // The final's inner samples, as k_b_final_count counts them.
+                acc.st[0] += (c.ell > 0 && !c.ep.antithetic) ? it.nf + it.nc : sh.big;
+                double phi_f, phi_c;
+                if (whole_final<P, MIXED>(A, (uint32_t)m, it, acc, phi_f, phi_c)) {
+                    it.phi32_f = phi_f;
+                    it.phi32_c = phi_c;
+                    reeval = true;
+                } else {
+                    finish_outer(c, m, it, phi_f, phi_c, acc);
+                }
+            } else {
+                pending = true;
+            }
+        }
+        if (pending || reeval) {
+            ItemState& w = c.items[m];
+            w.Y = it.Y;
+            w.start = it.start;
+            w.nf = it.nf;
+            w.nc = it.nc;
+            if (reeval) {
+                w.phi32_f = it.phi32_f;
+                w.phi32_c = it.phi32_c;
+            }
+        }
+        const uint32_t sf = wave_append(&c.ctl->cnt_f[1], app_f);
+        const uint32_t sc = wave_append(&c.ctl->cnt_c[1], app_c);
+        const uint32_t sp = wave_append(&c.ctl->n_pend, pending);
+        if (app_f) c.tf[1][sf] = TestEntry{0., 0., 0., (uint32_t)m, 0};
+        if (app_c) c.tc[1][sc] = TestEntry{0., 0., 0., (uint32_t)m, 0};
+        if (pending) c.pend[sp] = (uint32_t)m;
+        if (MIXED) list_reeval(c, reeval, (uint32_t)m);
     }
-    block_flush(acc, &c.ctl->acc);
+    block_flush<true>(acc, &c.ctl->acc);
+}
+
+// True when the fused first pass applies: both round-0 test groups (those
+// that run) fit one segment.
+template <class P>
+bool fused_applies(const BatchedCommon& c) {
+    const bool test_f = c.cf.max > 2 * c.cf.base;
+    const bool test_c = c.ell > 0 && c.cc.max > 2 * c.cc.base;
+    const bool fit_f = !test_f || (c.cf.base + P::spu - 1) / P::spu <= c.S;
+    const bool fit_c = !test_c || (c.cc.base + P::spu - 1) / P::spu <= c.S;
+    return c.pend != nullptr && fit_f && fit_c;
 }
 
 // The kernel sequence of one level call on stream s.
 template <class P, bool MIXED>
-hipError_t launch_batched_level(const typename P::Args& A, uint32_t blocks, hipStream_t s) {
+hipError_t launch_batched_level(const typename P::Args& A0, uint32_t blocks, hipStream_t s) {
+    typename P::Args A = A0;
+    A.c.fused = fused_applies<P>(A.c) ? 1u : 0u;
+    const bool rule1 = A.c.ep.adapt_rule == 1;
     hipLaunchKernelGGL(k_b_zero, dim3(1), dim3(256), 0, s, A.c.ctl);
     hipError_t e = P::launch_outer(A, s);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_b_init<P>), dim3(blocks), dim3(256), 0, s, A);
-    for (uint32_t r = 0; r < A.c.rounds; r++)
-        if (A.c.ep.adapt_rule == 1) hipLaunchKernelGGL((k_b_test<P, MIXED, 1>), dim3(blocks), dim3(256), 0, s, A, r);
+    uint32_t r0 = 0;
+    if (A.c.fused) {
+        if (rule1) hipLaunchKernelGGL((k_b_fused<P, MIXED, 1>), dim3(blocks), dim3(256), 0, s, A);
+        else hipLaunchKernelGGL((k_b_fused<P, MIXED, 0>), dim3(blocks), dim3(256), 0, s, A);
+        r0 = 1;
+    } else {
+        hipLaunchKernelGGL((k_b_init<P>), dim3(blocks), dim3(256), 0, s, A);
+    }
+    for (uint32_t r = r0; r < A.c.rounds; r++)
+        if (rule1) hipLaunchKernelGGL((k_b_test<P, MIXED, 1>), dim3(blocks), dim3(256), 0, s, A, r);
         else hipLaunchKernelGGL((k_b_test<P, MIXED, 0>), dim3(blocks), dim3(256), 0, s, A, r);
     hipLaunchKernelGGL((k_b_final_count<P>), dim3(blocks), dim3(256), 0, s, A);
     hipLaunchKernelGGL(k_b_final_alloc, dim3(1), dim3(64), 0, s, A.c.ctl, A.c.fcap);

@@ -111,13 +111,14 @@ void Engine::free_samplers() {
 // Buffers of the batched forms (kept between level calls, released here).
 void Engine::free_batched() {
     for (void* p : {(void*)bat_items, (void*)bat_tf[0], (void*)bat_tf[1], (void*)bat_tc[0], (void*)bat_tc[1],
-                    (void*)bat_reeval, (void*)bat_fpool, (void*)bat_redo_acc, (void*)bat_redo_cnt, (void*)bat_ctl,
-                    (void*)pf_R})
+                    (void*)bat_reeval, (void*)bat_pend, (void*)bat_fpool, (void*)bat_redo_acc, (void*)bat_redo_cnt,
+                    (void*)bat_ctl, (void*)pf_R})
         if (p) hipFree(p);
     if (bat_ctl_h) hipHostFree(bat_ctl_h);
     bat_items = nullptr;
     bat_tf[0] = bat_tf[1] = bat_tc[0] = bat_tc[1] = nullptr;
     bat_reeval = nullptr;
+    bat_pend = nullptr;
     bat_fpool = nullptr;
     bat_redo_acc = nullptr;
     bat_redo_cnt = nullptr;
@@ -311,11 +312,12 @@ int Engine::prepare_batched(uint64_t M, BatchedCommon& c) {
         uint64_t cap = bat_cap ? bat_cap : 1024;
         while (cap < M) cap *= 2;
         for (void* p : {(void*)bat_items, (void*)bat_tf[0], (void*)bat_tf[1], (void*)bat_tc[0],
-                        (void*)bat_tc[1], (void*)bat_reeval})
+                        (void*)bat_tc[1], (void*)bat_reeval, (void*)bat_pend})
             if (p) hipFree(p);
         bat_items = nullptr;
         bat_tf[0] = bat_tf[1] = bat_tc[0] = bat_tc[1] = nullptr;
         bat_reeval = nullptr;
+        bat_pend = nullptr;
         bat_cap = 0;
         if ((e = hipMalloc(&bat_items, sizeof(ItemState) * cap)) != hipSuccess) return hip_fail(e, "hipMalloc items");
         for (int i = 0; i < 2; i++) {
@@ -323,6 +325,7 @@ int Engine::prepare_batched(uint64_t M, BatchedCommon& c) {
             if ((e = hipMalloc(&bat_tc[i], sizeof(TestEntry) * cap)) != hipSuccess) return hip_fail(e, "hipMalloc queue");
         }
         if ((e = hipMalloc(&bat_reeval, sizeof(uint32_t) * cap)) != hipSuccess) return hip_fail(e, "hipMalloc reeval");
+        if ((e = hipMalloc(&bat_pend, sizeof(uint32_t) * cap)) != hipSuccess) return hip_fail(e, "hipMalloc pending");
         bat_cap = cap;
     }
     const uint64_t want_f = 3 * M + 4096;
@@ -351,6 +354,8 @@ int Engine::prepare_batched(uint64_t M, BatchedCommon& c) {
     c.fpool = bat_fpool;
     c.fcap = (uint32_t)std::min<uint64_t>(bat_fcap, 0xffffffffull);
     c.reeval = bat_reeval;
+    // The fused first pass (on unless NMLMC_FUSED=0) lists its leftovers here.
+    c.pend = fused_enabled() ? bat_pend : nullptr;
     c.redo_acc = bat_redo_acc;
     c.redo_cnt = bat_redo_cnt;
     c.ctl = bat_ctl;
@@ -467,6 +472,13 @@ int Engine::run_batched(uint32_t ell, uint64_t M, BatchedCommon& c, sample_outpu
     rc = trace_finish(M, tb, tr_phi_f, tr_phi_c, tr_nf, tr_nc);
     if (rc) return rc;
     return set_error(NMLMC_OK, "");
+}
+
+// The batched forms' fused first pass; NMLMC_FUSED=0 turns it off (results
+// are the same either way).
+bool Engine::fused_enabled() {
+    const char* s = getenv("NMLMC_FUSED");
+    return !(s && s[0] == '0');
 }
 
 // Outer samples per batched launch chain: 2^23 (about 2.8 GB of queues and item

@@ -63,6 +63,7 @@ struct Engine {
     TestEntry* bat_tc[2] = {nullptr, nullptr};
     FinalEntry* bat_fpool = nullptr;
     uint32_t* bat_reeval = nullptr;
+    uint32_t* bat_pend = nullptr;
     double* bat_redo_acc = nullptr;
     uint32_t* bat_redo_cnt = nullptr;
     BatchedCtl* bat_ctl = nullptr;
@@ -97,6 +98,7 @@ struct Engine {
     int sample_delta_unbiased(uint32_t option_index, uint64_t M, uint32_t moments, double* sums, double* work);
     uint32_t batched_blocks();
     uint64_t batched_chunk();
+    static bool fused_enabled();
     int prepare_batched(uint64_t M, BatchedCommon& c);
     int batched_common(uint32_t ell, uint64_t M, uint64_t outer_offset, BatchedCommon& c);
     int run_batched(uint32_t ell, uint64_t M, BatchedCommon& c, sample_output* out,
