@@ -68,6 +68,13 @@ def sweep(nm, M0, levels, rec=None):
                             dmean=(o["diffsums"][0] / M) if ell else 0.))
 
 
+def autotune(nm, M0, levels):
+    """Launch-shape autotuning of every level call of the sweep, before the
+    timed region (the tuning calls draw outer samples set apart for tuning):
+    [segment, blocks per CU] per level."""
+    return [list(nm.autotune_level(l, level_M(M0, l))[:2]) for l in range(levels + 1)]
+
+
 def ops_per_sample(form):
     with open(os.path.join(ROOT, "nested-risk-mlmc_amd", "opcount.json")) as f:
         oc = json.load(f)
@@ -170,7 +177,7 @@ def portfolio_configs(nm):
     return importlib.import_module(nm.__name__ + ".configs")
 
 
-def portfolio_bench(nm, which, levels, M0, rel_eps, tte, plain_cores, tte_M0):
+def portfolio_bench(nm, which, levels, M0, rel_eps, tte, plain_cores, tte_M0, tune=True):
     """Portfolio books: per-level outer samples/s of the mixed and plain forms
     (N_base = 64 2^l, 3-sigma doubling capped at 2^4, M_l = ceil(M0 2^-1.5l))
     and time to RMSE eps = rel_eps * (1 - q) (q the threshold's quantile)."""
@@ -199,6 +206,7 @@ def portfolio_bench(nm, which, levels, M0, rel_eps, tte, plain_cores, tte_M0):
     eps = rel_eps * (1 - q)
     for form, cores, label in ((nm.FORM_MIXED, 1, "mixed"), (nm.FORM_PLAIN, plain_cores, "plain_fp64")):
         pf_setup(nm, form, cores)
+        shapes = autotune(nm, M0, levels) if (tune and form != nm.FORM_PLAIN) else None
         sweep(nm, M0, levels)
         recs = []
         t1 = time.perf_counter()
@@ -209,6 +217,7 @@ def portfolio_bench(nm, which, levels, M0, rel_eps, tte, plain_cores, tte_M0):
              "kernel_ms": [round(x["kernel_ms"], 3) for x in recs],
              "inner_samples": sum(x["inner"] for x in recs)}
         if form == nm.FORM_MIXED:
+            r["launch_shapes"] = shapes
             r["reevaluated"] = sum(x["reev"] for x in recs)
             r["flips"] = sum(x["flips"] for x in recs)
         nm.cleanup_nestedmc_sampler()
@@ -233,6 +242,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="use the default launch shapes instead of autotuning each level first")
     ap.add_argument("--M0", type=float, default=1e7)
     ap.add_argument("--levels", type=int, default=12)
     ap.add_argument("--plain-cores", type=int, default=65536)
@@ -263,8 +274,9 @@ def main():
     nm.load()
     chosen = nm.FORM_MIXED
 
-    # Chosen form: warmup, then K timed sweeps.
+    # Chosen form: launch-shape autotuning, warmup, then K timed sweeps.
     setup(nm, chosen, 1)
+    shapes = None if args.no_autotune else autotune(nm, args.M0, args.levels)
     for _ in range(args.warmup):
         sweep(nm, args.M0, args.levels)
     recs = []
@@ -297,13 +309,14 @@ def main():
         print("# plain: %.3e outer samples/s" % (outer_per_step / pel), file=sys.stderr, flush=True)
         nm.cleanup_nestedmc_sampler()
         setup(nm, nm.FORM_BATCHED, 1)
+        bshapes = None if args.no_autotune else autotune(nm, args.M0, args.levels)
         sweep(nm, args.M0, args.levels)
         brecs = []
         t0 = time.perf_counter()
         sweep(nm, args.M0, args.levels, brecs)
         bel = time.perf_counter() - t0
         extra["batched_fp64"] = {"outer_samples_per_s": round(outer_per_step / bel, 1),
-                                 "per_level": per_level(brecs, args.levels)}
+                                 "per_level": per_level(brecs, args.levels), "launch_shapes": bshapes}
         # The mixed form's first timed sweep and this sweep draw the same outer
         # samples (same keys and offsets after one warm-up sweep each): per-level
         # difference of the level means (fine mean at l = 0, difference mean
@@ -332,7 +345,7 @@ def main():
     if args.portfolio != "none":
         extra["portfolio"] = portfolio_bench(nm, args.portfolio, args.pf_levels, args.pf_M0,
                                              args.pf_rel_eps, not args.no_tte, args.plain_cores,
-                                             args.pf_tte_M0)
+                                             args.pf_tte_M0, tune=not args.no_autotune)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args.cpu_M0, args.levels)
@@ -353,7 +366,8 @@ def main():
         "config": {"workload": "adaptive model problem: X=Y+Z, threshold 2, levels 0..%d, "
                                "N_base=32*2^l, 3-sigma doubling capped at 2^6, "
                                "M_l=ceil(%g*2^-1.5l)" % (args.levels, args.M0),
-                   "form": "batched FP32+FP64 (mixed)", "outer_samples_per_step": outer_per_step},
+                   "form": "batched FP32+FP64 (mixed)", "outer_samples_per_step": outer_per_step,
+                   "launch_shapes": shapes},
         "per_level_outer_samples_per_s": levels_mixed,
         "mixed_reevaluated": reev,
         "mixed_flips": flips,

@@ -201,6 +201,16 @@ const char* nmlmc_last_kernel_name(void);
 /* Lanes per outer sample chosen for the last call (autotuned or overridden). */
 uint32_t nmlmc_last_group_lanes(void);
 
+/* Launch-shape autotuner of the batched forms (no counterpart in the reference,
+ * whose launch geometry is fixed): times the level call (ell, M) of the staged
+ * problem in the current form over segments of 16, 32, 64 and 128 inner units
+ * per lane and grids of 4, 8 and 16 blocks per CU, on outer samples set apart
+ * for tuning (ids from 2^45), and keeps the fastest for later calls at level
+ * ell.  Writes the choice and its kernel time (any pointer may be NULL).  The
+ * choices are dropped when a problem is staged or the adaptive rule changes;
+ * per-sample results never depend on them. */
+int nmlmc_autotune_level(uint32_t ell, uint64_t M, uint32_t* segment, uint32_t* blocks_per_cu, double* ms);
+
 /* ---- MLMC driver (mlmc.hpp:207-337) over the current problem -------------- */
 
 /* Output of nmlmc_mlmc_run: per level sums as in mlmc_t::mc_diff_output_t

@@ -204,6 +204,39 @@ __device__ __forceinline__ uint32_t aload_u32(const uint32_t* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Segmented sums over the lanes of a wave ahead of a cross-lane hand-off.  The
+// active lanes holding segments of the same draw group (equal key; such lanes
+// are adjacent, as consecutive lanes take consecutive segments) add their a
+// and b; the last lane of each run gets the run's totals and its length n
+// (tail = true).  One hand-off per run instead of one per segment: without it
+// the ~1000 segments of one large group queue on the same few addresses.  All
+// active lanes must call it together, and they must be a prefix of the wave
+// (as in the grid-stride loops).
+struct RunSums {
+    double a, b;
+    uint32_t n;
+    bool tail;
+};
+__device__ __forceinline__ RunSums run_sums(unsigned long long key, double a, double b) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t active = __ballot(1);
+    const unsigned long long kp = __shfl_up(key, 1u, 64);
+    const uint64_t starts = __ballot(lane == 0 || kp != key);
+#pragma unroll
+    for (uint32_t o = 1; o < 64; o <<= 1) {
+        const double pa = __shfl_up(a, o, 64), pb = __shfl_up(b, o, 64);
+        const unsigned long long pk = __shfl_up(key, o, 64);
+        if (lane >= o && pk == key) {
+            a += pa;
+            b += pb;
+        }
+    }
+    const bool tail = lane == 63 || !((active >> (lane + 1)) & 1) || ((starts >> (lane + 1)) & 1);
+    const uint64_t at_or_below = starts & ((2ull << lane) - 1);  // lane 63: all bits
+    const uint32_t first = 63 - (uint32_t)__clzll((long long)at_or_below);
+    return {a, b, lane - first + 1, tail};
+}
+
 // Append with one atomic per wave for the lanes appending together.
 __device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool want) {
     const uint64_t mask = __ballot(want);

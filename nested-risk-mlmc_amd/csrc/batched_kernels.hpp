@@ -108,6 +108,7 @@ __global__ __launch_bounds__(256, MIXED ? NMLMC_MIXED_WAVES : 1) void k_b_test(t
     const uint64_t uf = (cur_f + P::spu - 1) / P::spu, uc = (cur_c + P::spu - 1) / P::spu;
     const uint64_t spe_f = (uf + c.S - 1) / c.S, spe_c = (uc + c.S - 1) / c.S;
     const uint64_t segs_f = (uint64_t)nfe * spe_f, total = segs_f + (uint64_t)nce * spe_c;
+    if (total == 0) return;  // nothing refined into this round: nothing to flush
     TestEntry* const lf = c.tf[r & 1];
     TestEntry* const lc_ = c.tc[r & 1];
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -128,14 +129,17 @@ __global__ __launch_bounds__(256, MIXED ? NMLMC_MIXED_WAVES : 1) void k_b_test(t
         acc.st[5] += s.nn;
         bool complete = spe == 1;
         double s1 = s.s1, s2 = s.s2;
-        if (!complete) {
-            aadd(&te.s1, s.s1);
-            aadd(&te.s2, s.s2);
-            wait_acks();
-            if (acount(&te.done, 1u) == spe - 1) {
-                complete = true;
-                s1 = aread(&te.s1);
-                s2 = aread(&te.s2);
+        if (__ballot(spe > 1)) {  // wave-uniform
+            const RunSums rs = run_sums(fine ? e : (e | (1ull << 63)), s.s1, s.s2);
+            if (!complete && rs.tail) {
+                aadd(&te.s1, rs.a);
+                aadd(&te.s2, rs.b);
+                wait_acks();
+                if (acount(&te.done, rs.n) == spe - rs.n) {
+                    complete = true;
+                    s1 = aread(&te.s1);
+                    s2 = aread(&te.s2);
+                }
             }
         }
         bool append = false;
@@ -549,7 +553,10 @@ __global__ __launch_bounds__(256, MIXED ? NMLMC_MIXED_WAVES : 1) void k_b_final(
     for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < total; k += stride) {
         bool reeval = false;
         uint32_t m_done = 0;
-        if (k >= total_split) {
+        const bool split = k < total_split;
+        uint64_t spe = 1, n_cls = 0, fidx = 0;
+        SegAcc s;
+        if (!split) {
             // ---- a whole outer sample on this lane: no atomics ----
             const FinalEntry& fe = c.fpool[whole_off + (k - total_split)];
             const uint32_t m = fe.item;
@@ -567,7 +574,6 @@ __global__ __launch_bounds__(256, MIXED ? NMLMC_MIXED_WAVES : 1) void k_b_final(
             // ---- one segment of a split final group ----
             uint64_t kk = k;
             int cls = 0;
-            uint64_t spe = 1, n_cls = 0;
             for (; cls < kMaxClasses; cls++) {
                 n_cls = c.base_min << cls;
                 const uint64_t units = (n_cls + P::spu - 1) / P::spu;
@@ -577,65 +583,71 @@ __global__ __launch_bounds__(256, MIXED ? NMLMC_MIXED_WAVES : 1) void k_b_final(
                 kk -= segs;
             }
             const uint64_t e = kk / spe, j = kk % spe;
-            FinalEntry& fe = c.fpool[c.ctl->class_off[cls] + e];
+            fidx = c.ctl->class_off[cls] + e;
+            const FinalEntry& fe = c.fpool[fidx];
             const uint32_t m = fe.item;
-            ItemState& it = c.items[m];
             const uint64_t i0 = (fe.kind == 0) ? (uint64_t)fe.group * n_cls : 0;
             const uint64_t i1 = i0 + n_cls;
             const uint64_t ub = i0 / P::spu, ue = (i1 + P::spu - 1) / P::spu;
             const uint64_t u0 = ub + j * c.S, u1 = (u0 + c.S < ue) ? u0 + c.S : ue;
-            SegAcc s;
-            P::template units<MIXED>(A, m, it, fe.kind == 2 ? TAG_FINAL_COARSE : TAG_FINAL, u0, u1, i0, i1, s);
+            P::template units<MIXED>(A, m, c.items[m], fe.kind == 2 ? TAG_FINAL_COARSE : TAG_FINAL, u0, u1, i0, i1, s);
             acc.work += s.work;
             acc.st[4] += s.nu;
             acc.st[5] += s.nn;
-            bool complete = spe == 1;
-            double s1 = s.s1, sab = s.sab;
-            if (!complete) {
-                aadd(&fe.s1, s.s1);
-                if (MIXED) aadd(&fe.sab, s.sab);
+        }
+        bool complete = split && spe == 1;
+        double s1 = s.s1, sab = s.sab;
+        if (__ballot(split && spe > 1)) {  // wave-uniform
+            FinalEntry& fe = c.fpool[fidx];
+            const RunSums rs = run_sums(split ? fidx : ((1ull << 63) | k), s.s1, s.sab);
+            if (split && spe > 1 && rs.tail) {
+                aadd(&fe.s1, rs.a);
+                if (MIXED) aadd(&fe.sab, rs.b);
                 wait_acks();
-                if (acount(&fe.done, 1u) == spe - 1) {
+                if (acount(&fe.done, rs.n) == spe - rs.n) {
                     complete = true;
                     s1 = aread(&fe.s1);
                     sab = MIXED ? aread(&fe.sab) : 0.;
                 }
             }
-            if (complete) {
-                acc.work += P::group_work(A);
-                const double mean = it.start + s1 / (double)n_cls;
-                const bool near = MIXED && fabs(mean) <= P::mixed_bound(sab / (double)n_cls, it.start);
-                if (near) __hip_atomic_fetch_or(&it.near, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if (fe.kind == 0) {
+        }
+        if (complete) {
+            const FinalEntry& fe = c.fpool[fidx];
+            const uint32_t m = fe.item;
+            ItemState& it = c.items[m];
+            acc.work += P::group_work(A);
+            const double mean = it.start + s1 / (double)n_cls;
+            const bool near = MIXED && fabs(mean) <= P::mixed_bound(sab / (double)n_cls, it.start);
+            if (near) __hip_atomic_fetch_or(&it.near, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (fe.kind == 0) {
+                const FinalShape sh = final_shape(c, it);
+                const double coeff = (double)(1.f / (float)sh.groups);
+                if (c.ell > 0) aadd(&it.phi_acc, phi_scalar(c.phi_type, mean, coeff, 0.));
+                aadd(&it.s1_big, s1);
+                if (MIXED) aadd(&it.sab_big, sab);
+            } else if (fe.kind == 1) {
+                astore(&it.phi_f, phi_scalar(c.phi_type, mean, 1., 0.));
+            } else {
+                astore(&it.phi_c, phi_scalar(c.phi_type, mean, 1., 0.));
+            }
+            wait_acks();
+            if (__hip_atomic_fetch_sub(&it.left, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 1u) {
+                const double s1b = aread(&it.s1_big), pacc = aread(&it.phi_acc);
+                double phi_f, phi_c;
+                outer_phi(c, it, s1b, pacc, aread(&it.phi_f), aread(&it.phi_c), phi_f, phi_c);
+                bool is_near = MIXED && aread_u32(&it.near) != 0;
+                if (MIXED && fe.kind == 0) {
                     const FinalShape sh = final_shape(c, it);
-                    const double coeff = (double)(1.f / (float)sh.groups);
-                    if (c.ell > 0) aadd(&it.phi_acc, phi_scalar(c.phi_type, mean, coeff, 0.));
-                    aadd(&it.s1_big, s1);
-                    if (MIXED) aadd(&it.sab_big, sab);
-                } else if (fe.kind == 1) {
-                    astore(&it.phi_f, phi_scalar(c.phi_type, mean, 1., 0.));
-                } else {
-                    astore(&it.phi_c, phi_scalar(c.phi_type, mean, 1., 0.));
+                    const double mb = it.start + s1b / (double)sh.big;
+                    is_near |= fabs(mb) <= P::mixed_bound(aread(&it.sab_big) / (double)sh.big, it.start);
                 }
-                wait_acks();
-                if (__hip_atomic_fetch_sub(&it.left, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 1u) {
-                    const double s1b = aread(&it.s1_big), pacc = aread(&it.phi_acc);
-                    double phi_f, phi_c;
-                    outer_phi(c, it, s1b, pacc, aread(&it.phi_f), aread(&it.phi_c), phi_f, phi_c);
-                    bool is_near = MIXED && aread_u32(&it.near) != 0;
-                    if (MIXED && fe.kind == 0) {
-                        const FinalShape sh = final_shape(c, it);
-                        const double mb = it.start + s1b / (double)sh.big;
-                        is_near |= fabs(mb) <= P::mixed_bound(aread(&it.sab_big) / (double)sh.big, it.start);
-                    }
-                    if (MIXED && is_near && c.phi_type == PHI_INDICATOR) {
-                        it.phi32_f = phi_f;
-                        it.phi32_c = phi_c;
-                        reeval = true;
-                        m_done = m;
-                    } else {
-                        finish_outer(c, m, it, phi_f, phi_c, acc);
-                    }
+                if (MIXED && is_near && c.phi_type == PHI_INDICATOR) {
+                    it.phi32_f = phi_f;
+                    it.phi32_c = phi_c;
+                    reeval = true;
+                    m_done = m;
+                } else {
+                    finish_outer(c, m, it, phi_f, phi_c, acc);
                 }
             }
         }

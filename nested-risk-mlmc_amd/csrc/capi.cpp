@@ -163,6 +163,7 @@ int Engine::seed_samplers(uint64_t s) {
     e = hipMemsetAsync(d_states, 0, sizeof(StreamState) * 2 * (size_t)cores, stream);
     if (e != hipSuccess) return hip_fail(e, "hipMemset states");
     next_outer.assign(64, 0);
+    tuned.clear();
     e = hipStreamSynchronize(stream);
     if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
     return NMLMC_OK;
@@ -493,7 +494,94 @@ uint64_t Engine::batched_chunk() {
     return ch;
 }
 
-uint32_t Engine::batched_blocks() { return (uint32_t)device_cus() * 8; }
+// Launch shape of a batched level call.  An explicit segment size (the
+// override argument, else NMLMC_SEGMENT) wins; otherwise the shape
+// autotune_level chose for this form and level, else the default: the largest
+// segment of 64, 32 or 16 units that still cuts the outer samples' first draw
+// groups into at least 2^19 segments (two waves per SIMD over 256 CUs), with
+// 8 blocks per CU.  On the adaptive benchmarks this default is within ~3% of
+// the best fixed segment at every level (larger segments pay fewer hand-offs,
+// smaller ones spread the few large groups of the high levels).
+Engine::Shape Engine::batched_shape(uint32_t ell, uint64_t M, uint64_t spu, const BatchedCommon& c,
+                                    uint32_t lanes_override) {
+    if (force.S) return force;
+    Shape sh;
+    const auto t = tuned.find(((uint64_t)form << 32) | ell);
+    if (t != tuned.end()) {
+        sh = t->second;
+    } else {
+        sh.bpc = 8;
+        const uint64_t group = c.cf.max > 2 * c.cf.base ? c.cf.base : c.cf.max;
+        const uint64_t units = (group + spu - 1) / spu;
+        const uint64_t m = std::min(M, batched_chunk());
+        sh.S = 16;
+        for (uint32_t S : {64u, 32u})
+            if (m * ((units + S - 1) / S) >= (1ull << 19)) {
+                sh.S = S;
+                break;
+            }
+    }
+    if (lanes_override) {
+        sh.S = lanes_override;
+    } else if (const char* env = getenv("NMLMC_SEGMENT")) {
+        const uint32_t v = (uint32_t)strtoul(env, nullptr, 10);
+        if (v) sh.S = v;
+    }
+    return sh;
+}
+
+// Times the batched level call (ell, M) of the staged problem in the current
+// form over candidate launch shapes and keeps the fastest for later calls at
+// this level: segments of 16, 32, 64 and 128 units at 8 blocks per CU, then 4
+// and 16 blocks per CU at the best segment.  Each candidate runs twice (the
+// faster kernel time counts) on outer samples 2^45 onwards, far from those of
+// the level's calls, so later results do not depend on whether tuning ran
+// (and per-sample results never depend on the shape).
+int Engine::autotune_level(uint32_t ell, uint64_t M, Shape& best, double& best_ms) {
+    if (!inited || problem < 0) return set_error(NMLMC_ERR_STATE, "no problem staged");
+    if (form == NMLMC_FORM_PLAIN) return set_error(NMLMC_ERR_ARG, "autotuning applies to the batched forms");
+    if (M == 0 || M >= (1ull << 45) || ell > 40) return set_error(NMLMC_ERR_ARG, "autotune: bad level or M");
+    std::vector<double> f(kMaxPhi * kMaxMoments), d(kMaxPhi * kMaxMoments);
+    sample_output out{};
+    out.fsums = f.data();
+    out.diffsums = d.data();
+    auto time_shape = [&](Shape s, double& ms) -> int {
+        force = s;
+        ms = 1e300;
+        for (int rep = 0; rep < 2; rep++) {
+            out.M = M;
+            const int rc = sample_batched(ell, M, 1ull << 45, 0, &out, nullptr, nullptr, nullptr, nullptr);
+            if (rc) {
+                force = Shape{};
+                return rc;
+            }
+            ms = std::min(ms, (double)last_ms);
+        }
+        force = Shape{};
+        return NMLMC_OK;
+    };
+    best = Shape{};
+    best_ms = 1e300;
+    for (uint32_t S : {16u, 32u, 64u, 128u}) {
+        double ms;
+        if (int rc = time_shape(Shape{S, 8}, ms)) return rc;
+        if (ms < best_ms) {
+            best = Shape{S, 8};
+            best_ms = ms;
+        }
+    }
+    const uint32_t S = best.S;
+    for (uint32_t bpc : {4u, 16u}) {
+        double ms;
+        if (int rc = time_shape(Shape{S, bpc}, ms)) return rc;
+        if (ms < best_ms) {
+            best = Shape{S, bpc};
+            best_ms = ms;
+        }
+    }
+    tuned[((uint64_t)form << 32) | ell] = best;
+    return set_error(NMLMC_OK, "");
+}
 
 // Batched forms: per-outer-sample keys, segment queues, one launch per round.
 int Engine::sample_batched(uint32_t ell, uint64_t M, uint64_t outer_offset, uint32_t lanes_override,
@@ -508,14 +596,12 @@ int Engine::sample_batched(uint32_t ell, uint64_t M, uint64_t outer_offset, uint
     H.c.phi_type = mp.phi_type;
     H.c.nps = mp.variant == MODEL_REF ? 2 : 1;
     H.c.ups = 0;
-    // Segment: 64 inner units per lane (128 samples of the shift model, one
-    // Threefry block + one Box-Muller pair per unit); the sweep of 32..256 over
-    // levels 0..12 of the adaptive shift model was fastest at 64.
-    H.c.S = lanes_override ? lanes_override : 64;
-    if (const char* sS = getenv("NMLMC_SEGMENT")) H.c.S = (uint32_t)strtoul(sS, nullptr, 10);
-    if (H.c.S == 0) H.c.S = 64;
+    // One unit = one Threefry block + one Box-Muller pair (two samples of the
+    // shift model, one of the reference model).
+    const Shape sh = batched_shape(ell, M, mp.variant == MODEL_SHIFT ? 2 : 1, H.c, lanes_override);
+    H.c.S = sh.S;
     last_G = H.c.S;
-    H.blocks = batched_blocks();
+    H.blocks = (uint32_t)device_cus() * sh.bpc;
     H.mixed = form == NMLMC_FORM_MIXED;
     last_kernel = "k_b_final";
     return run_batched(ell, M, H.c, out, [&]() { return launch_model_batched(H, stream); }, tr_phi_f, tr_phi_c,
@@ -582,6 +668,7 @@ int nmlmc_set_adaptive_rule(uint32_t rule, uint32_t cap_log2) {
     if (rule > 1 || cap_log2 > 24) return set_error(NMLMC_ERR_ARG, "invalid adaptive rule");
     g_engine.ep.adapt_rule = rule;
     g_engine.ep.cap_log2 = cap_log2;
+    g_engine.tuned.clear();
     return set_error(NMLMC_OK, "");
 }
 
@@ -650,5 +737,16 @@ int nmlmc_mixed_stats(uint64_t* reevaluated, uint64_t* flips) {
 double nmlmc_last_kernel_ms(void) { return g_engine.last_ms; }
 const char* nmlmc_last_kernel_name(void) { return g_engine.last_kernel; }
 uint32_t nmlmc_last_group_lanes(void) { return g_engine.last_G; }
+
+int nmlmc_autotune_level(uint32_t ell, uint64_t M, uint32_t* segment, uint32_t* blocks_per_cu, double* ms) {
+    nmlmc::Engine::Shape best;
+    double best_ms = 0.;
+    const int rc = g_engine.autotune_level(ell, M, best, best_ms);
+    if (rc) return rc;
+    if (segment) *segment = best.S;
+    if (blocks_per_cu) *blocks_per_cu = best.bpc;
+    if (ms) *ms = best_ms;
+    return set_error(NMLMC_OK, "");
+}
 
 }  // extern "C"

@@ -6,6 +6,7 @@
 #include <stdint.h>
 
 #include <functional>
+#include <map>
 #include <vector>
 
 #include "../../include/nmlmc.h"
@@ -96,9 +97,17 @@ struct Engine {
     int cus = 0;
     int device_cus();
     int sample_delta_unbiased(uint32_t option_index, uint64_t M, uint32_t moments, double* sums, double* work);
-    uint32_t batched_blocks();
     uint64_t batched_chunk();
     static bool fused_enabled();
+    // Launch shape of a batched level call: inner units per segment (S) and
+    // blocks of 256 threads per CU.
+    struct Shape {
+        uint32_t S = 0, bpc = 0;
+    };
+    std::map<uint64_t, Shape> tuned;  // autotune_level's choices, key (form << 32) | level
+    Shape force{};                    // the candidate autotune_level is timing
+    Shape batched_shape(uint32_t ell, uint64_t M, uint64_t spu, const BatchedCommon& c, uint32_t lanes_override);
+    int autotune_level(uint32_t ell, uint64_t M, Shape& best, double& best_ms);
     int prepare_batched(uint64_t M, BatchedCommon& c);
     int batched_common(uint32_t ell, uint64_t M, uint64_t outer_offset, BatchedCommon& c);
     int run_batched(uint32_t ell, uint64_t M, BatchedCommon& c, sample_output* out,

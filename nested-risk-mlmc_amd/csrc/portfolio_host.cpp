@@ -72,6 +72,7 @@ void Engine::free_portfolio() {
 int Engine::stage_portfolio(const portfolio_data* data) {
     if (!inited) return set_error(NMLMC_ERR_STATE, "init_nestedmc_sampler has not been called");
     if (!data) return set_error(NMLMC_ERR_ARG, "portfolio data is NULL");
+    tuned.clear();
     const uint32_t d = (uint32_t)data->stocks.size;
     if (d == 0 || !data->stocks.data) return set_error(NMLMC_ERR_ARG, "portfolio has no stocks");
     if (data->subsample < -1 || data->subsample > 1) return set_error(NMLMC_ERR_ARG, "unknown sub-sampling mode");
@@ -276,9 +277,8 @@ int Engine::sample_portfolio_batched(uint32_t ell, uint64_t M, uint64_t outer_of
     if (rc) return rc;
     H.pf = pf;
     H.c.phi_type = pf.phi_type;
-    H.c.S = lanes_override ? lanes_override : 32;
-    if (const char* sS = getenv("NMLMC_SEGMENT")) H.c.S = (uint32_t)strtoul(sS, nullptr, 10);
-    if (H.c.S == 0) H.c.S = 32;
+    const Shape sh = batched_shape(ell, M, 1, H.c, lanes_override);
+    H.c.S = sh.S;
     last_G = H.c.S;
     const size_t need = (size_t)std::min(M, batched_chunk()) * pf.d;
     if (need > pf_R_cap) {
@@ -293,7 +293,7 @@ int Engine::sample_portfolio_batched(uint32_t ell, uint64_t M, uint64_t outer_of
     }
     H.R_tau = pf_R;
     H.bs_work = pf_bs_work;
-    H.blocks = batched_blocks();
+    H.blocks = (uint32_t)device_cus() * sh.bpc;
     H.mixed = form == NMLMC_FORM_MIXED;
     H.exact_only = pf.subsample == 1 && pf.nopt[0] == 0 && pf.nopt[2] == 0;
     last_kernel = "k_b_final";

@@ -85,7 +85,7 @@ EXPORTS = [
     "nmlmc_last_kernel_ms", "nmlmc_last_kernel_name", "nmlmc_last_group_lanes",
     "nmlmc_mlmc_run", "nmlmc_mlmc_run_cb", "portfolio_sample_delta_unbiased",
     "portfolio_estimate_delta_for_unbiased", "portfolio_gen_data", "portfolio_free_data",
-    "nmlmc_mlmc_last_table",
+    "nmlmc_mlmc_last_table", "nmlmc_autotune_level",
 ]
 
 _lib = None
@@ -133,6 +133,7 @@ def load(path=LIB_PATH):
         "nmlmc_last_kernel_ms": (dbl, []),
         "nmlmc_last_kernel_name": (C.c_char_p, []),
         "nmlmc_last_group_lanes": (u32, []),
+        "nmlmc_autotune_level": (i32, [u32, u64, C.POINTER(u32), C.POINTER(u32), dp]),
         "nmlmc_mlmc_run": (i32, [dbl, u64, dbl, dbl, dbl, i32, u32, C.POINTER(LevelStats),
                                  C.POINTER(C.c_uint32), dp]),
         "nmlmc_mlmc_run_cb": (i32, [SAMPLE_LEVEL_FN, C.c_void_p, dbl, u64, dbl, dbl, dbl, i32, u32,
@@ -247,6 +248,14 @@ def last_kernel_name():
 
 def last_group_lanes():
     return load().nmlmc_last_group_lanes()
+
+
+def autotune_level(ell, M):
+    """Time the batched level call (ell, M) over launch shapes and keep the
+    fastest for later calls at this level: returns (segment, blocks_per_cu, ms)."""
+    S, B, ms = C.c_uint32(), C.c_uint32(), C.c_double()
+    check(load().nmlmc_autotune_level(ell, M, C.byref(S), C.byref(B), C.byref(ms)))
+    return S.value, B.value, ms.value
 
 
 def _run_outputs(lib_call, max_levels):

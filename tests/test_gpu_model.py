@@ -114,6 +114,25 @@ def test_batched_fp64_matches_oracle(nm, oracle_f64, segment, monkeypatch):
     nm.set_form(nm.FORM_PLAIN)
 
 
+def test_autotuned_launch_shape_keeps_results(nm, oracle_f64):
+    """The launch-shape autotuner times its candidates on outer samples set
+    apart for tuning: the level calls after it draw the outer samples they would
+    have drawn without it, and per-sample results equal the oracle's whatever
+    shape it chose.  The plain form has nothing to tune."""
+    orc = _shift_setup(nm, oracle_f64, 1, nm.FORM_BATCHED)
+    for ell, M in ((0, 3000), (2, 400)):
+        S, B, ms = nm.autotune_level(ell, M)
+        assert S in (16, 32, 64, 128) and B in (4, 8, 16) and ms > 0
+        g = nm.sample(ell, M, trace=True)
+        assert nm.last_group_lanes() == S
+        o = orc.sample(ell, M, keying=1, outer_offset=0, trace=True)
+        _cmp(g, o)
+    nm.set_form(nm.FORM_PLAIN)
+    with pytest.raises(nm.NmlmcError):
+        nm.autotune_level(0, 100)
+    nm.cleanup_nestedmc_sampler()
+
+
 @pytest.mark.parametrize("form", [1, 2])
 def test_batched_chunked_level_calls(nm, oracle_f64, form, monkeypatch):
     """A level call larger than the chunk runs as several launch chains (here
