@@ -189,6 +189,16 @@ int nmlmc_portfolio_set_correlation(const double* chol, uint32_t d);
 int nmlmc_sample(uint32_t ell, uint64_t M, uint64_t outer_offset, sample_output* out,
                  double* tr_phi_f, double* tr_phi_c, uint64_t* tr_nf, uint64_t* tr_nc);
 
+/* A level sweep: n level calls (ells[i], Ms[i]) into outs[i], each continuing
+ * its level's outer ids as nmlmc_sample(..., UINT64_MAX, ...) does and with the
+ * same results.  In the batched forms the calls' kernel chains are captured as
+ * one HIP graph and run with one wait (NMLMC_SWEEP_GRAPH=0: enqueued on the
+ * stream directly); the plain form runs them one by one.  No counterpart in the
+ * reference, whose mlmc_t::run calls its sampler level by level
+ * (mlmc.hpp:306-314); the native driver uses it for each iteration's levels.
+ * nmlmc_last_kernel_ms then gives the whole sweep's device time. */
+int nmlmc_sample_levels(uint32_t n, const uint32_t* ells, const uint64_t* Ms, sample_output* outs);
+
 /* Mixed form diagnostics of the last call: outer samples re-evaluated in FP64,
  * and indicator flips (FP32 decision differing from the FP64 one). */
 int nmlmc_mixed_stats(uint64_t* reevaluated, uint64_t* flips);

@@ -115,6 +115,11 @@ void Engine::free_batched() {
                     (void*)bat_ctl, (void*)pf_R})
         if (p) hipFree(p);
     if (bat_ctl_h) hipHostFree(bat_ctl_h);
+    if (sweep_slots) hipHostFree(sweep_slots);
+    if (sweep_exec) hipGraphExecDestroy(sweep_exec);
+    sweep_slots = nullptr;
+    sweep_slot_cap = 0;
+    sweep_exec = nullptr;
     bat_items = nullptr;
     bat_tf[0] = bat_tf[1] = bat_tc[0] = bat_tc[1] = nullptr;
     bat_reeval = nullptr;
@@ -410,6 +415,25 @@ int Engine::batched_common(uint32_t ell, uint64_t M, uint64_t outer_offset, Batc
 int Engine::run_batched(uint32_t ell, uint64_t M, BatchedCommon& c, sample_output* out,
                         const std::function<hipError_t()>& launch, double* tr_phi_f, double* tr_phi_c,
                         uint64_t* tr_nf, uint64_t* tr_nc) {
+    if (sweep_on) {
+        // Level sweep: enqueue the chunks' chains, each followed by a copy of
+        // its control block into its host slot; sample_levels waits once.
+        const uint64_t chunk = batched_chunk();
+        const uint64_t offset0 = c.offset;
+        for (uint64_t start = 0; start < M; start += chunk) {
+            const uint32_t k = (uint32_t)sweep_chain_level.size();
+            if (k >= sweep_slot_cap) return set_error(NMLMC_ERR_STATE, "level sweep: no control slot left");
+            c.M = std::min(chunk, M - start);
+            c.offset = offset0 + start;
+            hipError_t e = launch();
+            if (e != hipSuccess) return hip_fail(e, "launch batched level");
+            e = hipMemcpyAsync(sweep_slots + k, bat_ctl, sizeof(BatchedCtl), hipMemcpyDeviceToHost, stream);
+            if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync ctl");
+            sweep_chain_level.push_back(sweep_level);
+        }
+        c.offset = offset0;
+        return NMLMC_OK;
+    }
     TraceBufs tb;
     int rc = trace_alloc(M, tr_phi_f || tr_phi_c || tr_nf || tr_nc, tb);
     if (rc) return rc;
@@ -472,6 +496,159 @@ int Engine::run_batched(uint32_t ell, uint64_t M, BatchedCommon& c, sample_outpu
     mixed_flips = h_out->stats[7];
     rc = trace_finish(M, tb, tr_phi_f, tr_phi_c, tr_nf, tr_nc);
     if (rc) return rc;
+    return set_error(NMLMC_OK, "");
+}
+
+// Several level calls of the batched forms enqueued back to back with one wait
+// at the end, captured from the stream as one HIP graph (NMLMC_SWEEP_GRAPH=0:
+// enqueued directly).  The graph is captured anew on every call, so it always
+// holds the current arguments, and an existing executable graph of the same
+// shape is updated in place instead of instantiated again.  Each call
+// continues its level's outer ids as a lone call would, and its results equal
+// those of the lone call.  The plain form runs the calls one by one.
+int Engine::sample_levels(uint32_t n, const uint32_t* ells, const uint64_t* Ms, sample_output* outs) {
+    if (!inited) return set_error(NMLMC_ERR_STATE, "init_nestedmc_sampler has not been called");
+    if (problem < 0) return set_error(NMLMC_ERR_STATE, "no problem staged (model_init / portfolio_init)");
+    if (n == 0) return set_error(NMLMC_OK, "");
+    if (!ells || !Ms || !outs) return set_error(NMLMC_ERR_ARG, "sample_levels: NULL argument");
+    for (uint32_t i = 0; i < n; i++)
+        if (ells[i] > 40) return set_error(NMLMC_ERR_ARG, "level %u out of range", ells[i]);
+    if (form == NMLMC_FORM_PLAIN) {
+        for (uint32_t i = 0; i < n; i++) {
+            const int rc = sample(ells[i], Ms[i], UINT64_MAX, 0, &outs[i], nullptr, nullptr, nullptr, nullptr);
+            if (rc) return rc;
+        }
+        return set_error(NMLMC_OK, "");
+    }
+    const uint64_t chunk = batched_chunk();
+    uint64_t maxM = 1;
+    uint64_t chains = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        maxM = std::max(maxM, std::min(Ms[i], chunk));
+        chains += (Ms[i] + chunk - 1) / chunk;
+    }
+    // Buffers for the largest chunk before anything is enqueued: nothing may be
+    // reallocated while chains are in flight or being captured.
+    BatchedCommon tmp;
+    std::memset(&tmp, 0, sizeof(tmp));
+    int rc = prepare_batched(maxM, tmp);
+    if (rc) return rc;
+    if (problem == PROBLEM_PORTFOLIO && (rc = ensure_pf_R(maxM))) return rc;
+    hipError_t e;
+    if (chains > sweep_slot_cap) {
+        if (sweep_slots) hipHostFree(sweep_slots);
+        sweep_slots = nullptr;
+        sweep_slot_cap = 0;
+        if ((e = hipHostMalloc(&sweep_slots, sizeof(BatchedCtl) * chains)) != hipSuccess)
+            return hip_fail(e, "hipHostMalloc sweep slots");
+        sweep_slot_cap = (uint32_t)chains;
+    }
+    std::vector<uint64_t> off(n);
+    {
+        std::vector<uint64_t> nxt = next_outer;
+        for (uint32_t i = 0; i < n; i++) {
+            if (ells[i] >= nxt.size()) nxt.resize(ells[i] + 1, 0);
+            off[i] = nxt[ells[i]];
+            nxt[ells[i]] += Ms[i];
+        }
+    }
+    const char* env = getenv("NMLMC_SWEEP_GRAPH");
+    const bool graph = !(env && env[0] == '0');
+    if (graph) {
+        if ((e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+            return hip_fail(e, "hipStreamBeginCapture");
+    } else if ((e = hipEventRecord(ev0, stream)) != hipSuccess) {
+        return hip_fail(e, "hipEventRecord");
+    }
+    sweep_on = true;
+    sweep_chain_level.clear();
+    for (uint32_t i = 0; i < n && !rc; i++) {
+        if (Ms[i] == 0) continue;
+        sweep_level = i;
+        outs[i].M = Ms[i];
+        rc = sample_batched(ells[i], Ms[i], off[i], 0, &outs[i], nullptr, nullptr, nullptr, nullptr);
+    }
+    sweep_on = false;
+    if (graph) {
+        hipGraph_t g = nullptr;
+        e = hipStreamEndCapture(stream, &g);
+        if (rc || e != hipSuccess) {
+            if (g) hipGraphDestroy(g);
+            return rc ? rc : hip_fail(e, "hipStreamEndCapture");
+        }
+        // Update the executable graph in place when the new capture has the
+        // same shape (same node count; the update checks the rest), else
+        // instantiate anew.  A refused update is not an error: its status is
+        // cleared so that no later call reports it.
+        size_t nodes = 0;
+        hipGraphGetNodes(g, nullptr, &nodes);
+        bool updated = false;
+        if (sweep_exec && nodes == sweep_exec_nodes) {
+            hipGraphNode_t bad = nullptr;
+            hipGraphExecUpdateResult res = hipGraphExecUpdateError;
+            updated = hipGraphExecUpdate(sweep_exec, g, &bad, &res) == hipSuccess && res == hipGraphExecUpdateSuccess;
+            if (!updated) (void)hipGetLastError();
+        }
+        if (!updated) {
+            if (sweep_exec) hipGraphExecDestroy(sweep_exec);
+            sweep_exec = nullptr;
+            if ((e = hipGraphInstantiate(&sweep_exec, g, nullptr, nullptr, 0)) != hipSuccess) {
+                sweep_exec = nullptr;
+                hipGraphDestroy(g);
+                return hip_fail(e, "hipGraphInstantiate");
+            }
+            sweep_exec_nodes = nodes;
+        }
+        hipGraphDestroy(g);
+        if ((e = hipEventRecord(ev0, stream)) != hipSuccess) return hip_fail(e, "hipEventRecord");
+        if ((e = hipGraphLaunch(sweep_exec, stream)) != hipSuccess) return hip_fail(e, "hipGraphLaunch");
+    } else if (rc) {
+        hipStreamSynchronize(stream);
+        return rc;
+    }
+    if ((e = hipEventRecord(ev1, stream)) != hipSuccess) return hip_fail(e, "hipEventRecord");
+    if ((e = hipStreamSynchronize(stream)) != hipSuccess) return hip_fail(e, "level sweep kernels");
+    float ms = 0;
+    hipEventElapsedTime(&ms, ev0, ev1);
+    double ms_total = ms;
+    // Per level call: the sum of its chains' control blocks.  A call whose
+    // final-group pool overflowed is redone on its own (run_batched grows the
+    // pool; the draws are keyed, so the redo is identical).
+    std::vector<Partial> tot(n);
+    std::memset(tot.data(), 0, sizeof(Partial) * n);
+    std::vector<char> redo(n, 0);
+    for (size_t k = 0; k < sweep_chain_level.size(); k++) {
+        const BatchedCtl& b = sweep_slots[k];
+        Partial& t = tot[sweep_chain_level[k]];
+        if (b.overflow) redo[sweep_chain_level[k]] = 1;
+        for (int j = 0; j < kMaxPhi * kMaxMoments; j++) {
+            t.f[j] += b.acc.f[j];
+            t.d[j] += b.acc.d[j];
+        }
+        t.work += b.acc.work;
+        for (int j = 0; j < kStatsExt; j++) t.stats[j] += b.acc.stats[j];
+    }
+    uint64_t reev = 0, flips = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        outs[i].M = Ms[i];
+        if (redo[i]) {
+            if ((rc = sample_batched(ells[i], Ms[i], off[i], 0, &outs[i], nullptr, nullptr, nullptr, nullptr)))
+                return rc;
+            ms_total += last_ms;
+            reev += mixed_reeval;
+            flips += mixed_flips;
+        } else {
+            std::memcpy(h_out, &tot[i], sizeof(Partial));
+            copy_out(ells[i], &outs[i]);
+            reev += tot[i].stats[6];
+            flips += tot[i].stats[7];
+        }
+        if (ells[i] >= next_outer.size()) next_outer.resize(ells[i] + 1, 0);
+        next_outer[ells[i]] = off[i] + Ms[i];
+    }
+    last_ms = (float)ms_total;
+    mixed_reeval = reev;
+    mixed_flips = flips;
     return set_error(NMLMC_OK, "");
 }
 
@@ -737,6 +914,10 @@ int nmlmc_mixed_stats(uint64_t* reevaluated, uint64_t* flips) {
 double nmlmc_last_kernel_ms(void) { return g_engine.last_ms; }
 const char* nmlmc_last_kernel_name(void) { return g_engine.last_kernel; }
 uint32_t nmlmc_last_group_lanes(void) { return g_engine.last_G; }
+
+int nmlmc_sample_levels(uint32_t n, const uint32_t* ells, const uint64_t* Ms, sample_output* outs) {
+    return g_engine.sample_levels(n, ells, Ms, outs);
+}
 
 int nmlmc_autotune_level(uint32_t ell, uint64_t M, uint32_t* segment, uint32_t* blocks_per_cu, double* ms) {
     nmlmc::Engine::Shape best;

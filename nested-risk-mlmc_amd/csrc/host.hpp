@@ -108,6 +108,17 @@ struct Engine {
     Shape force{};                    // the candidate autotune_level is timing
     Shape batched_shape(uint32_t ell, uint64_t M, uint64_t spu, const BatchedCommon& c, uint32_t lanes_override);
     int autotune_level(uint32_t ell, uint64_t M, Shape& best, double& best_ms);
+    // Level sweeps (sample_levels): several batched level calls enqueued back
+    // to back, each chain's control block copied to its own pinned host slot.
+    bool sweep_on = false;                    // run_batched enqueues and returns
+    uint32_t sweep_level = 0;                 // index of the level call being enqueued
+    std::vector<uint32_t> sweep_chain_level;  // per enqueued chain: its level call
+    BatchedCtl* sweep_slots = nullptr;        // pinned, one per chain
+    uint32_t sweep_slot_cap = 0;
+    hipGraphExec_t sweep_exec = nullptr;      // the last captured sweep, updated in place
+    size_t sweep_exec_nodes = 0;              // its node count
+    int sample_levels(uint32_t n, const uint32_t* ells, const uint64_t* Ms, sample_output* outs);
+    int ensure_pf_R(uint64_t M);
     int prepare_batched(uint64_t M, BatchedCommon& c);
     int batched_common(uint32_t ell, uint64_t M, uint64_t outer_offset, BatchedCommon& c);
     int run_batched(uint32_t ell, uint64_t M, BatchedCommon& c, sample_output* out,

@@ -144,6 +144,15 @@ struct mlmc_t {
         typedef std::function<double(const mlmc_output_t& out)> fn_est_bias_t;
         mlmc_input_t() { est_bias = [](const mlmc_output_t& out) { return default_bias_estimate(out); }; }
         fn_sample_level_t sample_level;
+        // Optional, no counterpart in the reference: the missing samples of
+        // every level of one step in one call ((ells, Ms) -> one output per
+        // level), e.g. a level sweep captured as one graph.  run() then calls
+        // it once per step with the (ell, M) pairs the reference's loop would
+        // pass to sample_level one by one, in the same order.
+        typedef std::function<std::vector<mc_diff_output_t>(const std::vector<uint32_t>& ells,
+                                                            const std::vector<uint64_t>& Ms)>
+            fn_sample_levels_t;
+        fn_sample_levels_t sample_levels;
         fn_est_bias_t est_bias;
         double targetTOL{1e-3};
         double startTOL{0.5};
@@ -207,6 +216,20 @@ struct mlmc_t {
             out.starting_level = better_starting_level(out, in.starting_level_coeff);
             while (true) {
                 get_optimal_Ml(in.theta * POW2(out.TOL) / in.Ca, in.M0, out, Ml);
+                if (in.sample_levels) {
+                    std::vector<uint32_t> ells;
+                    std::vector<uint64_t> Ms;
+                    for (uint32_t ell = 0; ell < out.levels.size(); ell++)
+                        if (out.levels[ell].M < Ml[ell]) {
+                            ells.push_back(ell);
+                            Ms.push_back(Ml[ell] - out.levels[ell].M);
+                            if (in.verbose) std::cout << "# Sampling " << Ms.back() << " of " << ell << std::endl;
+                        }
+                    if (!ells.empty()) {
+                        const std::vector<mc_diff_output_t> r = in.sample_levels(ells, Ms);
+                        for (size_t i = 0; i < ells.size(); i++) out.levels[ells[i]] += r[i];
+                    }
+                }
                 for (uint32_t ell = 0; ell < out.levels.size(); ell++) {
                     while (out.levels[ell].M < Ml[ell]) {
                         uint64_t M = Ml[ell] - out.levels[ell].M;

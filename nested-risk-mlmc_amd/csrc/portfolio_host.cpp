@@ -265,6 +265,21 @@ int Engine::sample_portfolio_parity(uint32_t ell, uint64_t M, uint32_t, sample_o
     return set_error(NMLMC_OK, "");
 }
 
+// Stock states at tau of M outer samples (batched forms; grown, never shrunk).
+int Engine::ensure_pf_R(uint64_t M) {
+    const size_t need = (size_t)M * pf.d;
+    if (need <= pf_R_cap) return NMLMC_OK;
+    size_t cap = pf_R_cap ? pf_R_cap : 1024;
+    if (pf_R) hipFree(pf_R);
+    pf_R = nullptr;
+    pf_R_cap = 0;
+    while (cap < need) cap *= 2;
+    const hipError_t e = hipMalloc(&pf_R, sizeof(double) * cap);
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc outer states");
+    pf_R_cap = cap;
+    return NMLMC_OK;
+}
+
 int Engine::sample_portfolio_batched(uint32_t ell, uint64_t M, uint64_t outer_offset, uint32_t lanes_override,
                                      sample_output* out, double* tr_phi_f, double* tr_phi_c, uint64_t* tr_nf,
                                      uint64_t* tr_nc) {
@@ -280,17 +295,7 @@ int Engine::sample_portfolio_batched(uint32_t ell, uint64_t M, uint64_t outer_of
     const Shape sh = batched_shape(ell, M, 1, H.c, lanes_override);
     H.c.S = sh.S;
     last_G = H.c.S;
-    const size_t need = (size_t)std::min(M, batched_chunk()) * pf.d;
-    if (need > pf_R_cap) {
-        size_t cap = pf_R_cap ? pf_R_cap : 1024;
-        if (pf_R) hipFree(pf_R);
-        pf_R = nullptr;
-        pf_R_cap = 0;
-        while (cap < need) cap *= 2;
-        hipError_t e = hipMalloc(&pf_R, sizeof(double) * cap);
-        if (e != hipSuccess) return hip_fail(e, "hipMalloc outer states");
-        pf_R_cap = cap;
-    }
+    if ((rc = ensure_pf_R(std::min(M, batched_chunk())))) return rc;
     H.R_tau = pf_R;
     H.bs_work = pf_bs_work;
     H.blocks = (uint32_t)device_cus() * sh.bpc;

@@ -85,7 +85,7 @@ EXPORTS = [
     "nmlmc_last_kernel_ms", "nmlmc_last_kernel_name", "nmlmc_last_group_lanes",
     "nmlmc_mlmc_run", "nmlmc_mlmc_run_cb", "portfolio_sample_delta_unbiased",
     "portfolio_estimate_delta_for_unbiased", "portfolio_gen_data", "portfolio_free_data",
-    "nmlmc_mlmc_last_table", "nmlmc_autotune_level",
+    "nmlmc_mlmc_last_table", "nmlmc_autotune_level", "nmlmc_sample_levels",
 ]
 
 _lib = None
@@ -134,6 +134,7 @@ def load(path=LIB_PATH):
         "nmlmc_last_kernel_name": (C.c_char_p, []),
         "nmlmc_last_group_lanes": (u32, []),
         "nmlmc_autotune_level": (i32, [u32, u64, C.POINTER(u32), C.POINTER(u32), dp]),
+        "nmlmc_sample_levels": (i32, [u32, C.POINTER(u32), C.POINTER(u64), C.POINTER(SampleOutput)]),
         "nmlmc_mlmc_run": (i32, [dbl, u64, dbl, dbl, dbl, i32, u32, C.POINTER(LevelStats),
                                  C.POINTER(C.c_uint32), dp]),
         "nmlmc_mlmc_run_cb": (i32, [SAMPLE_LEVEL_FN, C.c_void_p, dbl, u64, dbl, dbl, dbl, i32, u32,
@@ -230,6 +231,21 @@ def sample(ell, M, outer_offset=None, trace=False, moments=4, phi_size=1):
     if trace:
         out.update(phi_f=tr[0], phi_c=tr[1], nf=tr[2], nc=tr[3])
     return out
+
+
+def sample_levels(ells, Ms, moments=4, phi_size=1):
+    """nmlmc_sample_levels: a level sweep (one captured graph in the batched
+    forms); returns one dict per level call, as sample() does."""
+    lib = load()
+    n = len(ells)
+    outs = [Output(moments, phi_size) for _ in range(n)]
+    arr = (SampleOutput * n)(*[o.c for o in outs])
+    check(lib.nmlmc_sample_levels(n, (C.c_uint32 * n)(*ells), (C.c_uint64 * n)(*Ms), arr))
+    res = []
+    for i, o in enumerate(outs):
+        o.c = arr[i]
+        res.append(o.as_dict())
+    return res
 
 
 def mixed_stats():

@@ -133,6 +133,63 @@ def test_autotuned_launch_shape_keeps_results(nm, oracle_f64):
     nm.cleanup_nestedmc_sampler()
 
 
+def _sums(r):
+    return (tuple(r["fsums"]), tuple(r["diffsums"]), r["work"], tuple(int(x) for x in r["stats"]), r["M"])
+
+
+@pytest.mark.parametrize("graph", ["1", "0"])
+@pytest.mark.parametrize("form", [1, 2])
+def test_level_sweep_matches_lone_calls(nm, oracle_f64, form, graph, monkeypatch):
+    """A level sweep (captured as one HIP graph, or enqueued directly) gives
+    every level call the results of the lone call on the same outer samples.
+    Three sweeps with eager calls and waits in between: each sweep (a fresh
+    capture updating the executable graph) equals its lone calls, and levels
+    continue their outer ids across sweeps and lone calls.  Level 0 runs as
+    three chunks.  Indicator sums, work and counts are integers, so equal."""
+    monkeypatch.setenv("NMLMC_SWEEP_GRAPH", graph)
+    monkeypatch.setenv("NMLMC_BATCH_CHUNK", "1000")
+    ells, Ms = [0, 1, 2, 3], [2500, 900, 300, 100]
+    _shift_setup(nm, oracle_f64, 1, form)
+    ref = []
+    for _ in range(3):
+        ref.append([_sums(nm.sample(l, M)) for l, M in zip(ells, Ms)] + [_sums(nm.sample(5, 40))])
+    nm.cleanup_nestedmc_sampler()
+    _shift_setup(nm, oracle_f64, 1, form)
+    for rep in range(3):
+        got = [_sums(r) for r in nm.sample_levels(ells, Ms)]
+        got.append(_sums(nm.sample(5, 40)))
+        assert got == ref[rep], rep
+    nm.cleanup_nestedmc_sampler()
+    nm.set_form(nm.FORM_PLAIN)
+
+
+def test_mlmc_driver_sweeps_match_lone_calls(nm, oracle_f64):
+    """The native MLMC driver in the batched form samples each step's levels as
+    one sweep; run over lone level calls instead (the callback driver, same
+    control flow) it makes the same (l, M) calls and gets the same sums."""
+    _shift_setup(nm, oracle_f64, 1, nm.FORM_MIXED)
+    a = nm.mlmc_run(2e-4, 2000)
+    nm.cleanup_nestedmc_sampler()
+    _shift_setup(nm, oracle_f64, 1, nm.FORM_MIXED)
+    calls = []
+
+    def level(ell, M):
+        calls.append((ell, M))
+        r = nm.sample(ell, M)
+        return dict(sums_fine=r["fsums"], sums_diff=r["diffsums"], work=r["work"], M=r["M"])
+
+    b = nm.mlmc_run_cb(level, 2e-4, 2000)
+    nm.cleanup_nestedmc_sampler()
+    assert len(calls) > 3
+    assert len(a["levels"]) == len(b["levels"])
+    for la, lb in zip(a["levels"], b["levels"]):
+        assert la["M"] == lb["M"]
+        assert list(la["sums_fine"]) == list(lb["sums_fine"])
+        assert list(la["sums_diff"]) == list(lb["sums_diff"])
+        assert la["work"] == lb["work"]
+    nm.set_form(nm.FORM_PLAIN)
+
+
 @pytest.mark.parametrize("form", [1, 2])
 def test_batched_chunked_level_calls(nm, oracle_f64, form, monkeypatch):
     """A level call larger than the chunk runs as several launch chains (here
