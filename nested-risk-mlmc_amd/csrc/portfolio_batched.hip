@@ -47,7 +47,7 @@ struct PortfolioPolicyT {
     }
 
     // Outer draw (sample_outer, portfolio_sampler.cpp:293-311): the one-factor
-    // model here; with a Cholesky factor, k_pf_outer_chol has already written
+    // model here; with a Cholesky factor, k_pf_outer_mfma has already written
     // the stock levels.  Then the delta control variate's constant and, without
     // sub-sampling, the pre-summed BS options (init_inner_sampler, 312-324).
     __device__ static void init_item(const Args& A, uint64_t m, ItemState& it, LaneAcc& acc) {
@@ -205,47 +205,63 @@ struct PortfolioPolicyT {
 };
 
 // Correlated outer draw with a Cholesky factor L: W = sqrt(tau) L z,
-// R_i = S0_i exp(sigma_i W_i + (mu_i - sigma_i^2 / 2) tau).  A block takes 64
-// outer samples, one per lane: their d normals (Box-Muller pairs of the outer
-// stream {seed, key}, counter {b, 0} for pair b) go to LDS transposed, z[j][lane],
-// then each wave forms every fourth row of L z, summing over j in order (as the
-// one-lane restatement does) from conflict-free LDS reads and wave-uniform
-// reads of L.  STAGED: the 64 x d stock levels are collected in LDS and written
-// as contiguous rows; otherwise each lane writes its own row.
-template <class P, bool STAGED>
-__global__ __launch_bounds__(256) void k_pf_outer_chol(typename P::Args A) {
-    extern __shared__ double zt[];  // [d][64], then (STAGED) [64][d + 1]
+// R_i = S0_i exp(sigma_i W_i + (mu_i - sigma_i^2 / 2) tau), as a matrix product
+// on the FP64 matrix cores.  A block of 16 waves takes 64 outer samples: their
+// d normals (Box-Muller pairs of the outer stream {seed, key}, counter {b, 0}
+// for pair b, pairs spread over the waves) go to LDS as z[k][m], rows up to
+// the next multiple of 4 zero.  Then wave w forms rows 16 (w % 4) .. +15 of
+// Z L^T ([64 x d] x [d x d]) for the 16-stock tiles w / 4, w / 4 + 4, ... with
+// v_mfma_f64_16x16x4f64, K in steps of 4 up to the tile's last row (L is lower
+// triangular).  FP64 operands and accumulation keep the stock levels those of
+// the FP64 forms, which the mixed form's FP64 redo must see too; only the order
+// of each row's sum differs from the one-lane restatement's.  Each lane writes
+// its accumulator elements: for one element index a wave writes 4 rows x 16
+// consecutive stocks.  Sixteen waves per block keep enough loads of L in flight
+// when only a few blocks run (the high levels have a few hundred outer samples).
+//
+// MFMA lane layouts (16x16x4 FP64, checked on gfx950): lane l holds A[l % 16][l / 16]
+// and B[l / 16][l % 16]; accumulator element i of lane l is D[l / 16 + 4 i][l % 16].
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+constexpr uint32_t kOuterWaves = 16;
+
+template <class P>
+__global__ __launch_bounds__(64 * kOuterWaves) void k_pf_outer_mfma(typename P::Args A) {
+    extern __shared__ double zt[];  // [4 ks][64]
     const PortfolioDevice& pf = A.pf;
-    const uint32_t d = pf.d;
-    double* rt = zt + 64 * d;
+    const uint32_t d = pf.d, ks = (d + 3) / 4, nt = (d + 15) / 16;
     const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const uint64_t m0 = (uint64_t)blockIdx.x * 64;
-    const uint64_t m = m0 + lane;
-    const bool valid = m < A.c.M;
-    const uint64_t k1 = P::key1(A, valid ? m : 0);
-    for (uint32_t b = wv; 2 * b < d; b += 4) {
-        const u64x2 blk = threefry2x64_20(b, 0, A.c.seed, k1);
-        double x, y;
-        boxmuller_f64(blk.a, blk.b, x, y);  // words 2b (element b) and 2b + 1 (element a)
-        zt[(2 * b) * 64 + lane] = x;
-        if (2 * b + 1 < d) zt[(2 * b + 1) * 64 + lane] = y;
+    {
+        const uint64_t m = m0 + lane;
+        const uint64_t k1 = P::key1(A, m < A.c.M ? m : 0);
+        for (uint32_t b = wv; b < 2 * ks; b += kOuterWaves) {  // pairs 0 .. 2 ks - 1 fill rows 0 .. 4 ks - 1
+            double x = 0., y = 0.;
+            if (2 * b < d) {
+                const u64x2 blk = threefry2x64_20(b, 0, A.c.seed, k1);
+                boxmuller_f64(blk.a, blk.b, x, y);  // words 2b (element b) and 2b + 1 (element a)
+            }
+            zt[(2 * b) * 64 + lane] = x;
+            zt[(2 * b + 1) * 64 + lane] = 2 * b + 1 < d ? y : 0.;
+        }
     }
     __syncthreads();
-    for (uint32_t i = wv; i < d; i += 4) {
-        const double* Li = pf.chol + (size_t)i * d;
-        double a = 0.;
-        for (uint32_t j = 0; j <= i; j++) a += Li[j] * zt[j * 64 + lane];
-        const StockRec st = pf.stocks[i];
-        const double Ri = simulate_stock_d(st.S0, st.mu, st.sigma, pf.tau, pf.stau * a);
-        if (STAGED) rt[lane * (d + 1) + i] = Ri;
-        else if (valid) A.R_tau[m * d + i] = Ri;
-    }
-    if (STAGED) {
-        __syncthreads();
-        const uint64_t nv = A.c.M - m0 < 64 ? A.c.M - m0 : 64;
-        for (uint32_t k = threadIdx.x; k < nv * d; k += blockDim.x) {
-            const uint32_t r = k / d, i = k - r * d;
-            A.R_tau[(m0 + r) * d + i] = rt[r * (d + 1) + i];
+    const uint32_t mt = wv & 3, ar = 16 * mt + (lane & 15), kq = lane >> 4, col = lane & 15;
+    for (uint32_t t = wv >> 2; t < nt; t += kOuterWaves / 4) {
+        const uint32_t n = 16 * t + col;
+        const uint32_t n_hi = 16 * (t + 1) < d ? 16 * (t + 1) : d;
+        const uint32_t s_end = (n_hi + 3) / 4;  // columns k < n_hi
+        f64x4 acc = {0., 0., 0., 0.};
+        for (uint32_t s = 0; s < s_end; s++) {
+            const uint32_t k = 4 * s + kq;
+            const double b = (n < d && k <= n) ? pf.chol[(size_t)n * d + k] : 0.;
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(zt[k * 64 + ar], b, acc, 0, 0, 0);
+        }
+        if (n >= d) continue;
+        const StockRec st = pf.stocks[n];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const uint64_t m = m0 + 16 * mt + (lane >> 4) + 4 * i;
+            if (m < A.c.M) A.R_tau[m * d + n] = simulate_stock_d(st.S0, st.mu, st.sigma, pf.tau, pf.stau * acc[i]);
         }
     }
 }
@@ -254,12 +270,9 @@ template <bool EXACT_ONLY>
 hipError_t PortfolioPolicyT<EXACT_ONLY>::launch_outer(const Args& A, hipStream_t s) {
     if (!A.pf.chol || A.c.M == 0) return hipSuccess;
     const uint32_t blocks = (uint32_t)((A.c.M + 63) / 64);
-    const size_t lds_z = sizeof(double) * 64 * A.pf.d, lds_r = sizeof(double) * 64 * (A.pf.d + 1);
-    using P = PortfolioPolicyT<EXACT_ONLY>;
-    if (lds_z + lds_r <= 160 * 1024)
-        hipLaunchKernelGGL((k_pf_outer_chol<P, true>), dim3(blocks), dim3(256), lds_z + lds_r, s, A);
-    else
-        hipLaunchKernelGGL((k_pf_outer_chol<P, false>), dim3(blocks), dim3(256), lds_z, s, A);
+    const size_t lds = sizeof(double) * 64 * 4 * ((A.pf.d + 3) / 4);  // <= 128 KB for d <= 256
+    hipLaunchKernelGGL((k_pf_outer_mfma<PortfolioPolicyT<EXACT_ONLY>>), dim3(blocks), dim3(64 * kOuterWaves), lds, s,
+                       A);
     return hipGetLastError();
 }
 
