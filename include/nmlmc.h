@@ -199,6 +199,22 @@ int nmlmc_sample(uint32_t ell, uint64_t M, uint64_t outer_offset, sample_output*
  * nmlmc_last_kernel_ms then gives the whole sweep's device time. */
 int nmlmc_sample_levels(uint32_t n, const uint32_t* ells, const uint64_t* Ms, sample_output* outs);
 
+/* ---- A batch of portfolio problems (no counterpart in the reference, which
+ * holds one problem per process) ------------------------------------------
+ * nmlmc_batch_begin(n) opens a batch of n <= 64 instances (dropping any earlier
+ * batch); nmlmc_batch_stage(i, data, seed, chol, d) builds instance i's tables
+ * as portfolio_update_data does, with its own seed and optional Cholesky
+ * factor (this replaces the engine's own staged problem, which must be staged
+ * again before it is sampled).  nmlmc_batch_sample(ell, M, outs) is one level
+ * call over all instances in one launch chain per chunk (batched and mixed
+ * forms): M[i] outer samples of instance i, continuing that instance's level
+ * ids, results into outs[i] - equal to those of instance i staged alone with
+ * portfolio_init(data, seed) and sampled with the same calls.  The instances
+ * share the sampler parameters and need one phi type (Indicator or Max). */
+int nmlmc_batch_begin(uint32_t n);
+int nmlmc_batch_stage(uint32_t i, const portfolio_data* data, uint64_t seed, const double* chol, uint32_t d);
+int nmlmc_batch_sample(uint32_t ell, const uint64_t* M, sample_output* outs);
+
 /* Mixed form diagnostics of the last call: outer samples re-evaluated in FP64,
  * and indicator flips (FP32 decision differing from the FP64 one). */
 int nmlmc_mixed_stats(uint64_t* reevaluated, uint64_t* flips);

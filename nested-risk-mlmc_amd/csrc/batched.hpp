@@ -52,7 +52,7 @@ struct ItemState {
     uint32_t known;  // bit 0: fine count known, bit 1: coarse count known (atomic or)
     uint32_t left;   // unfinished final groups (atomic)
     uint32_t near;   // mixed: some final mean within the FP32 bound (atomic or)
-    uint32_t pad;
+    uint32_t inst;   // instance of a batch of problems (0 for a single problem)
 };
 
 struct TestEntry {
@@ -274,6 +274,8 @@ struct BatchedCommon {
     double* redo_acc;    // [kRedoSlots x kRedoGroups] FP64 group sums of the redo
     uint32_t* redo_cnt;  // [kRedoSlots] parts of the redo done
     BatchedCtl* ctl;
+    Partial* inst_out;   // batch of problems: per-instance level accumulators (else NULL: ctl->acc)
+    uint32_t n_inst;     // instances in the batch (0 for a single problem)
     double* tr_phi_f;
     double* tr_phi_c;
     uint64_t* tr_nf;
@@ -281,9 +283,96 @@ struct BatchedCommon {
 };
 
 
-// Defined once (reduce.hip): clears the control block; turns the final-group
+// Level accumulators of a block, behind one interface for the two layouts:
+//   MULTI = false (one problem): LaneAcc registers and the lanes' LDS moment
+//     columns, flushed into c.ctl->acc;
+//   MULTI = true (a batch of problems; every outer sample carries its
+//     instance): per-instance sums in LDS, added with LDS atomics and flushed
+//     into c.inst_out[instance].
+// MOM: the kernel finishes outer samples (moment sums are kept).  init and
+// flush must be reached by every thread of the block.
+constexpr uint32_t kMaxInst = 64;
+constexpr int kInstD = 2 * kMaxMoments + 1;  // fine moments, difference moments, work
+__device__ __forceinline__ double* inst_sums() {
+    __shared__ double a[kMaxInst * kInstD];
+    return a;
+}
+__device__ __forceinline__ unsigned long long* inst_stats() {
+    __shared__ unsigned long long a[kMaxInst * kStatsExt];
+    return a;
+}
+
+template <bool MULTI>
+struct BlockAcc {
+    LaneAcc r;
+    template <bool MOM>
+    __device__ __forceinline__ void init(const BatchedCommon& c) {
+        if (!MULTI) {
+            r.zero();
+            if (MOM) zero_lane_moments();
+            return;
+        }
+        for (uint32_t i = threadIdx.x; i < c.n_inst * kInstD; i += blockDim.x) inst_sums()[i] = 0.;
+        for (uint32_t i = threadIdx.x; i < c.n_inst * kStatsExt; i += blockDim.x) inst_stats()[i] = 0;
+        __syncthreads();
+    }
+    __device__ __forceinline__ void work(uint32_t inst, double v) {
+        if (MULTI) atomicAdd(&inst_sums()[inst * kInstD + 2 * kMaxMoments], v);
+        else r.work += v;
+    }
+    __device__ __forceinline__ void st(uint32_t inst, int k, uint64_t v) {
+        if (MULTI) {
+            if (v) atomicAdd(&inst_stats()[inst * kStatsExt + k], (unsigned long long)v);
+        } else {
+            r.st[k] += v;
+        }
+    }
+    // add_moments of phi (fine) and, when diff, of dphi (nested_sampler.hpp:87-100).
+    __device__ __forceinline__ void moments(uint32_t inst, double phi, uint32_t fm, bool diff, double dphi,
+                                            uint32_t dm) {
+        if (!MULTI) {
+            add_lane_moments(phi, fm, false);
+            if (diff) add_lane_moments(dphi, dm, true);
+            return;
+        }
+        double* a = inst_sums() + inst * kInstD;
+        double p = 1.;
+        for (uint32_t j = 0; j < fm; j++) {
+            p *= phi;
+            atomicAdd(&a[j], p);
+        }
+        if (diff) {
+            p = 1.;
+            for (uint32_t j = 0; j < dm; j++) {
+                p *= dphi;
+                atomicAdd(&a[kMaxMoments + j], p);
+            }
+        }
+    }
+    template <bool MOM>
+    __device__ __forceinline__ void flush(const BatchedCommon& c) {
+        if (!MULTI) {
+            block_flush<MOM>(r, &c.ctl->acc);
+            return;
+        }
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < c.n_inst * kInstD; i += blockDim.x) {
+            const double v = inst_sums()[i];
+            const uint32_t q = i / kInstD, f = i - q * kInstD;
+            Partial* out = &c.inst_out[q];
+            if (v != 0.) atomicAdd(f < kMaxMoments ? &out->f[f] : (f < 2 * kMaxMoments ? &out->d[f - kMaxMoments] : &out->work), v);
+        }
+        for (uint32_t i = threadIdx.x; i < c.n_inst * kStatsExt; i += blockDim.x) {
+            const unsigned long long v = inst_stats()[i];
+            if (v) atomicAdd((unsigned long long*)&c.inst_out[i / kStatsExt].stats[i % kStatsExt], v);
+        }
+    }
+};
+
+// Defined once (reduce.hip): clears the control block (and a batch's
+// per-instance accumulators); turns the final-group
 // class counts into offsets and flags an over-full pool.
-__global__ void k_b_zero(BatchedCtl* ctl);
+__global__ void k_b_zero(BatchedCtl* ctl, Partial* inst_out, uint32_t n_inst);
 __global__ void k_b_final_alloc(BatchedCtl* ctl, uint32_t fcap);
 
 }  // namespace nmlmc

@@ -62,8 +62,8 @@ __device__ __forceinline__ uint64_t next_count(const BatchedCommon& c, const Lev
 template <class P>
 __global__ __launch_bounds__(256) void k_b_init(typename P::Args A) {
     const BatchedCommon& c = A.c;
-    LaneAcc acc;
-    acc.zero();
+    BlockAcc<P::kMulti> acc;
+    acc.template init<false>(c);
     const bool test_f = c.cf.max > 2 * c.cf.base;
     const bool test_c = c.ell > 0 && c.cc.max > 2 * c.cc.base;
     const bool acc_f = test_f && (c.cf.base + P::spu - 1) / P::spu > c.S;
@@ -76,39 +76,40 @@ __global__ __launch_bounds__(256) void k_b_init(typename P::Args A) {
     for (uint64_t m = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; m < c.M; m += stride) {
         ItemState& it = c.items[m];
         P::init_item(A, m, it, acc);
+        const uint32_t q = it.inst;
         uint64_t nf = 0, nc = 0;
         if (!test_f) {
             nf = c.cf.max;
-            acc.st[1] += 1;
-            acc.st[2] += 1;
-            acc.st[3] += (c.cf.max == c.cf.base);
+            acc.st(q, 1, 1);
+            acc.st(q, 2, 1);
+            acc.st(q, 3, c.cf.max == c.cf.base);
         } else {
-            acc.st[0] += c.cf.base;
+            acc.st(q, 0, c.cf.base);
         }
         if (c.ell > 0) {
             if (!test_c) nc = c.cc.max;
-            else acc.st[0] += c.cc.base;
+            else acc.st(q, 0, c.cc.base);
         }
         it.nf = nf;
         it.nc = nc;
         if (acc_f) c.tf[0][m] = TestEntry{0., 0., 0., (uint32_t)m, 0};
         if (acc_c) c.tc[0][m] = TestEntry{0., 0., 0., (uint32_t)m, 0};
     }
-    block_flush<false>(acc, &c.ctl->acc);
+    acc.template flush<false>(c);
 }
 
 // Test round r of every outer sample still refining its fine or coarse count.
 template <class P, bool MIXED, int RULE>
 __global__ __launch_bounds__(256, MIXED ? NMLMC_MIXED_WAVES : 1) void k_b_test(typename P::Args A, uint32_t r) {
     const BatchedCommon& c = A.c;
-    LaneAcc acc;
-    acc.zero();
     const uint32_t nfe = aload_u32(&c.ctl->cnt_f[r]), nce = aload_u32(&c.ctl->cnt_c[r]);
     const uint64_t cur_f = c.cf.base << r, cur_c = c.cc.base << r;
     const uint64_t uf = (cur_f + P::spu - 1) / P::spu, uc = (cur_c + P::spu - 1) / P::spu;
     const uint64_t spe_f = (uf + c.S - 1) / c.S, spe_c = (uc + c.S - 1) / c.S;
     const uint64_t segs_f = (uint64_t)nfe * spe_f, total = segs_f + (uint64_t)nce * spe_c;
     if (total == 0) return;  // nothing refined into this round: nothing to flush
+    BlockAcc<P::kMulti> acc;
+    acc.template init<false>(c);
     TestEntry* const lf = c.tf[r & 1];
     TestEntry* const lc_ = c.tc[r & 1];
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -124,9 +125,10 @@ __global__ __launch_bounds__(256, MIXED ? NMLMC_MIXED_WAVES : 1) void k_b_test(t
         const uint64_t u0 = j * c.S, u1 = (u0 + c.S < units) ? u0 + c.S : units;
         SegAcc s;
         P::template units<MIXED>(A, m, it, (fine ? TAG_FINE_TEST : TAG_COARSE_TEST) + r, u0, u1, 0, cur, s);
-        acc.work += s.work;
-        acc.st[4] += s.nu;
-        acc.st[5] += s.nn;
+        const uint32_t q = it.inst;
+        acc.work(q, s.work);
+        acc.st(q, 4, s.nu);
+        acc.st(q, 5, s.nn);
         bool complete = spe == 1;
         double s1 = s.s1, s2 = s.s2;
         if (__ballot(spe > 1)) {  // wave-uniform
@@ -144,20 +146,20 @@ __global__ __launch_bounds__(256, MIXED ? NMLMC_MIXED_WAVES : 1) void k_b_test(t
         }
         bool append = false;
         if (complete) {
-            acc.work += P::group_work(A);
+            acc.work(q, P::group_work(A, it));
             const LevelCounts& lc = fine ? c.cf : c.cc;
             bool more;
             const uint64_t n = next_count<RULE>(c, lc, it.start, cur, s1, s2, more);
             if (more) {
                 append = true;
-                acc.st[0] += n;
+                acc.st(q, 0, n);
             } else {
                 ItemState& w = c.items[m];
                 if (fine) {
                     w.nf = n;
-                    acc.st[1] += 1;
-                    acc.st[2] += (n == lc.max);
-                    acc.st[3] += (n == lc.base);
+                    acc.st(q, 1, 1);
+                    acc.st(q, 2, n == lc.max);
+                    acc.st(q, 3, n == lc.base);
                 } else {
                     w.nc = n;
                 }
@@ -171,7 +173,7 @@ __global__ __launch_bounds__(256, MIXED ? NMLMC_MIXED_WAVES : 1) void k_b_test(t
             else c.tc[(r + 1) & 1][sc] = TestEntry{0., 0., 0., m, 0};
         }
     }
-    block_flush<false>(acc, &c.ctl->acc);
+    acc.template flush<false>(c);
 }
 
 // Final draw groups of an outer sample: count and shape.
@@ -215,8 +217,8 @@ __global__ __launch_bounds__(256) void k_b_final_count(typename P::Args A) {
     __shared__ uint32_t hist[kMaxClasses + 1];
     for (int i = threadIdx.x; i <= kMaxClasses; i += blockDim.x) hist[i] = 0;
     __syncthreads();
-    LaneAcc acc;
-    acc.zero();
+    BlockAcc<P::kMulti> acc;
+    acc.template init<false>(c);
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     const uint64_t n_items = c.fused ? aload_u32(&c.ctl->n_pend) : c.M;
     for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n_items; k += stride) {
@@ -224,7 +226,7 @@ __global__ __launch_bounds__(256) void k_b_final_count(typename P::Args A) {
         const ItemState& it = c.items[m];
         const FinalShape s = final_shape(c, it);
         const bool plain2 = c.ell > 0 && !c.ep.antithetic;
-        acc.st[0] += plain2 ? it.nf + it.nc : s.big;
+        acc.st(it.inst, 0, plain2 ? it.nf + it.nc : s.big);
         if (final_units<P>(c, it, s) <= c.S) {
             atomicAdd(&hist[kMaxClasses], 1u);  // one lane finishes it alone
         } else if (plain2) {
@@ -238,7 +240,7 @@ __global__ __launch_bounds__(256) void k_b_final_count(typename P::Args A) {
     for (int i = threadIdx.x; i < kMaxClasses; i += blockDim.x)
         if (hist[i]) atomicAdd(&c.ctl->class_cnt[i], hist[i]);
     if (threadIdx.x == 0 && hist[kMaxClasses]) atomicAdd(&c.ctl->n_whole, hist[kMaxClasses]);
-    block_flush<false>(acc, &c.ctl->acc);
+    acc.template flush<false>(c);
 }
 
 // Writes each outer sample's final entries: whole samples after the class
@@ -305,10 +307,10 @@ __global__ __launch_bounds__(256) void k_b_final_fill(typename P::Args A) {
 }
 
 // Records a finished outer sample: moments, trace (the reference's lines 446-451).
+template <bool MULTI>
 __device__ __forceinline__ void finish_outer(const BatchedCommon& c, uint64_t m, const ItemState& it,
-                                             double phi_f, double phi_c, LaneAcc& acc) {
-    add_lane_moments(phi_f, c.ep.fine_moments, false);
-    if (c.ell > 0) add_lane_moments(phi_f - phi_c, c.ep.diff_moments, true);
+                                             double phi_f, double phi_c, BlockAcc<MULTI>& acc) {
+    acc.moments(it.inst, phi_f, c.ep.fine_moments, c.ell > 0, phi_f - phi_c, c.ep.diff_moments);
     if (c.tr_phi_f) c.tr_phi_f[m] = phi_f;
     if (c.tr_phi_c) c.tr_phi_c[m] = c.ell == 0 ? 0. : phi_c;
     if (c.tr_nf) c.tr_nf[m] = it.nf;
@@ -348,7 +350,7 @@ __device__ __forceinline__ SegAcc lane_group(const typename P::Args& A, uint32_t
 // to the level accumulator with atomics.
 __device__ __forceinline__ void redo_finish(const BatchedCommon& c, uint64_t m, const ItemState& it, double phi_f,
                                          double phi_c) {
-    Partial* out = &c.ctl->acc;
+    Partial* out = c.inst_out ? &c.inst_out[it.inst] : &c.ctl->acc;
     double p = 1.;
     for (uint32_t j = 0; j < c.ep.fine_moments; j++) {
         p *= phi_f;
@@ -476,16 +478,16 @@ __global__ __launch_bounds__(256) void k_b_redo_seq(typename P::Args A) {
 // in FP64 instead of being finished).
 template <class P, bool MIXED>
 __device__ __forceinline__ bool whole_final(const typename P::Args& A, uint32_t m, const ItemState& it,
-                                            LaneAcc& acc, double& phi_f, double& phi_c) {
+                                            BlockAcc<P::kMulti>& acc, double& phi_f, double& phi_c) {
     const BatchedCommon& c = A.c;
     const FinalShape sh = final_shape(c, it);
     bool near = false;
     if (c.ell > 0 && !c.ep.antithetic) {
         const SegAcc a = lane_group<P, MIXED>(A, m, it, TAG_FINAL, 0, it.nf);
         const SegAcc b = lane_group<P, MIXED>(A, m, it, TAG_FINAL_COARSE, 0, it.nc);
-        acc.work += a.work + b.work + 2 * P::group_work(A);
-        acc.st[4] += a.nu + b.nu;
-        acc.st[5] += a.nn + b.nn;
+        acc.work(it.inst, a.work + b.work + 2 * P::group_work(A, it));
+        acc.st(it.inst, 4, a.nu + b.nu);
+        acc.st(it.inst, 5, a.nn + b.nn);
         const double mf = it.start + a.s1 / (double)it.nf, mc = it.start + b.s1 / (double)it.nc;
         near = MIXED && (fabs(mf) <= P::mixed_bound(a.sab / (double)it.nf, it.start) ||
                          fabs(mc) <= P::mixed_bound(b.sab / (double)it.nc, it.start));
@@ -496,9 +498,9 @@ __device__ __forceinline__ bool whole_final(const typename P::Args& A, uint32_t 
         double s1_big = 0., sab_big = 0., phi_small = 0.;
         for (uint32_t g = 0; g < sh.groups; g++) {
             const SegAcc a = lane_group<P, MIXED>(A, m, it, TAG_FINAL, (uint64_t)g * sh.n, (uint64_t)(g + 1) * sh.n);
-            acc.work += a.work + P::group_work(A);
-            acc.st[4] += a.nu;
-            acc.st[5] += a.nn;
+            acc.work(it.inst, a.work + P::group_work(A, it));
+            acc.st(it.inst, 4, a.nu);
+            acc.st(it.inst, 5, a.nn);
             const double mean = it.start + a.s1 / (double)sh.n;
             if (MIXED) near |= fabs(mean) <= P::mixed_bound(a.sab / (double)sh.n, it.start);
             phi_small = phi_scalar(c.phi_type, mean, coeff, phi_small);
@@ -532,11 +534,10 @@ __device__ __forceinline__ void list_reeval(const BatchedCommon& c, bool reeval,
 template <class P, bool MIXED>
 __global__ __launch_bounds__(256, MIXED ? NMLMC_MIXED_WAVES : 1) void k_b_final(typename P::Args A) {
     const BatchedCommon& c = A.c;
-    LaneAcc acc;
-    acc.zero();
-    zero_lane_moments();
+    BlockAcc<P::kMulti> acc;
+    acc.template init<true>(c);
     if (c.ctl->overflow) {
-        block_flush<true>(acc, &c.ctl->acc);
+        acc.template flush<true>(c);
         return;
     }
     // Split entries (per size class, largest first in memory order of classes)
@@ -590,10 +591,11 @@ __global__ __launch_bounds__(256, MIXED ? NMLMC_MIXED_WAVES : 1) void k_b_final(
             const uint64_t i1 = i0 + n_cls;
             const uint64_t ub = i0 / P::spu, ue = (i1 + P::spu - 1) / P::spu;
             const uint64_t u0 = ub + j * c.S, u1 = (u0 + c.S < ue) ? u0 + c.S : ue;
-            P::template units<MIXED>(A, m, c.items[m], fe.kind == 2 ? TAG_FINAL_COARSE : TAG_FINAL, u0, u1, i0, i1, s);
-            acc.work += s.work;
-            acc.st[4] += s.nu;
-            acc.st[5] += s.nn;
+            const ItemState& it = c.items[m];
+            P::template units<MIXED>(A, m, it, fe.kind == 2 ? TAG_FINAL_COARSE : TAG_FINAL, u0, u1, i0, i1, s);
+            acc.work(it.inst, s.work);
+            acc.st(it.inst, 4, s.nu);
+            acc.st(it.inst, 5, s.nn);
         }
         bool complete = split && spe == 1;
         double s1 = s.s1, sab = s.sab;
@@ -615,7 +617,7 @@ __global__ __launch_bounds__(256, MIXED ? NMLMC_MIXED_WAVES : 1) void k_b_final(
             const FinalEntry& fe = c.fpool[fidx];
             const uint32_t m = fe.item;
             ItemState& it = c.items[m];
-            acc.work += P::group_work(A);
+            acc.work(it.inst, P::group_work(A, it));
             const double mean = it.start + s1 / (double)n_cls;
             const bool near = MIXED && fabs(mean) <= P::mixed_bound(sab / (double)n_cls, it.start);
             if (near) __hip_atomic_fetch_or(&it.near, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -653,7 +655,7 @@ __global__ __launch_bounds__(256, MIXED ? NMLMC_MIXED_WAVES : 1) void k_b_final(
         }
         if (MIXED) list_reeval(c, reeval, m_done);
     }
-    block_flush<true>(acc, &c.ctl->acc);
+    acc.template flush<true>(c);
 }
 
 // Fused first pass, for level calls whose round-0 test groups fit one segment:
@@ -668,49 +670,49 @@ __global__ __launch_bounds__(256, MIXED ? NMLMC_MIXED_WAVES : 1) void k_b_final(
 template <class P, bool MIXED, int RULE>
 __global__ __launch_bounds__(256, MIXED ? NMLMC_MIXED_WAVES : 1) void k_b_fused(typename P::Args A) {
     const BatchedCommon& c = A.c;
-    LaneAcc acc;
-    acc.zero();
-    zero_lane_moments();
+    BlockAcc<P::kMulti> acc;
+    acc.template init<true>(c);
     const bool test_f = c.cf.max > 2 * c.cf.base;
     const bool test_c = c.ell > 0 && c.cc.max > 2 * c.cc.base;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t m = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; m < c.M; m += stride) {
         ItemState it;
         P::init_item(A, m, it, acc);
+        const uint32_t q = it.inst;
         it.nf = it.nc = 0;
         bool app_f = false, app_c = false;
         if (!test_f) {
             it.nf = c.cf.max;
-            acc.st[1] += 1;
-            acc.st[2] += 1;
-            acc.st[3] += (c.cf.max == c.cf.base);
+            acc.st(q, 1, 1);
+            acc.st(q, 2, 1);
+            acc.st(q, 3, (c.cf.max == c.cf.base));
         } else {
-            acc.st[0] += c.cf.base;
+            acc.st(q, 0, c.cf.base);
             const SegAcc t = lane_group<P, MIXED>(A, (uint32_t)m, it, TAG_FINE_TEST, 0, c.cf.base);
-            acc.work += t.work + P::group_work(A);
-            acc.st[4] += t.nu;
-            acc.st[5] += t.nn;
+            acc.work(q, t.work + P::group_work(A, it));
+            acc.st(q, 4, t.nu);
+            acc.st(q, 5, t.nn);
             const uint64_t n = next_count<RULE>(c, c.cf, it.start, c.cf.base, t.s1, t.s2, app_f);
             if (app_f) {
-                acc.st[0] += n;
+                acc.st(q, 0, n);
             } else {
                 it.nf = n;
-                acc.st[1] += 1;
-                acc.st[2] += (n == c.cf.max);
-                acc.st[3] += (n == c.cf.base);
+                acc.st(q, 1, 1);
+                acc.st(q, 2, (n == c.cf.max));
+                acc.st(q, 3, (n == c.cf.base));
             }
         }
         if (c.ell > 0) {
             if (!test_c) {
                 it.nc = c.cc.max;
             } else {
-                acc.st[0] += c.cc.base;
+                acc.st(q, 0, c.cc.base);
                 const SegAcc t = lane_group<P, MIXED>(A, (uint32_t)m, it, TAG_COARSE_TEST, 0, c.cc.base);
-                acc.work += t.work + P::group_work(A);
-                acc.st[4] += t.nu;
-                acc.st[5] += t.nn;
+                acc.work(q, t.work + P::group_work(A, it));
+                acc.st(q, 4, t.nu);
+                acc.st(q, 5, t.nn);
                 const uint64_t n = next_count<RULE>(c, c.cc, it.start, c.cc.base, t.s1, t.s2, app_c);
-                if (app_c) acc.st[0] += n;
+                if (app_c) acc.st(q, 0, n);
                 else it.nc = n;
             }
         }
@@ -719,7 +721,7 @@ __global__ __launch_bounds__(256, MIXED ? NMLMC_MIXED_WAVES : 1) void k_b_fused(
             const FinalShape sh = final_shape(c, it);
             if (final_units<P>(c, it, sh) <= c.S) {
                 // The final's inner samples, as k_b_final_count counts them.
-                acc.st[0] += (c.ell > 0 && !c.ep.antithetic) ? it.nf + it.nc : sh.big;
+                acc.st(q, 0, (c.ell > 0 && !c.ep.antithetic) ? it.nf + it.nc : sh.big);
                 double phi_f, phi_c;
                 if (whole_final<P, MIXED>(A, (uint32_t)m, it, acc, phi_f, phi_c)) {
                     it.phi32_f = phi_f;
@@ -738,6 +740,7 @@ __global__ __launch_bounds__(256, MIXED ? NMLMC_MIXED_WAVES : 1) void k_b_fused(
             w.start = it.start;
             w.nf = it.nf;
             w.nc = it.nc;
+            w.inst = it.inst;
             if (reeval) {
                 w.phi32_f = it.phi32_f;
                 w.phi32_c = it.phi32_c;
@@ -751,13 +754,14 @@ __global__ __launch_bounds__(256, MIXED ? NMLMC_MIXED_WAVES : 1) void k_b_fused(
         if (pending) c.pend[sp] = (uint32_t)m;
         if (MIXED) list_reeval(c, reeval, (uint32_t)m);
     }
-    block_flush<true>(acc, &c.ctl->acc);
+    acc.template flush<true>(c);
 }
 
 // True when the fused first pass applies: both round-0 test groups (those
 // that run) fit one segment.
 template <class P>
 bool fused_applies(const BatchedCommon& c) {
+    if (!P::kFuse) return false;
     const bool test_f = c.cf.max > 2 * c.cf.base;
     const bool test_c = c.ell > 0 && c.cc.max > 2 * c.cc.base;
     const bool fit_f = !test_f || (c.cf.base + P::spu - 1) / P::spu <= c.S;
@@ -771,15 +775,18 @@ hipError_t launch_batched_level(const typename P::Args& A0, uint32_t blocks, hip
     typename P::Args A = A0;
     A.c.fused = fused_applies<P>(A.c) ? 1u : 0u;
     const bool rule1 = A.c.ep.adapt_rule == 1;
-    hipLaunchKernelGGL(k_b_zero, dim3(1), dim3(256), 0, s, A.c.ctl);
+    hipLaunchKernelGGL(k_b_zero, dim3(1), dim3(256), 0, s, A.c.ctl, A.c.inst_out, A.c.n_inst);
     hipError_t e = P::launch_outer(A, s);
     if (e != hipSuccess) return e;
     uint32_t r0 = 0;
-    if (A.c.fused) {
-        if (rule1) hipLaunchKernelGGL((k_b_fused<P, MIXED, 1>), dim3(blocks), dim3(256), 0, s, A);
-        else hipLaunchKernelGGL((k_b_fused<P, MIXED, 0>), dim3(blocks), dim3(256), 0, s, A);
-        r0 = 1;
-    } else {
+    if constexpr (P::kFuse) {
+        if (A.c.fused) {
+            if (rule1) hipLaunchKernelGGL((k_b_fused<P, MIXED, 1>), dim3(blocks), dim3(256), 0, s, A);
+            else hipLaunchKernelGGL((k_b_fused<P, MIXED, 0>), dim3(blocks), dim3(256), 0, s, A);
+            r0 = 1;
+        }
+    }
+    if (!A.c.fused) {
         hipLaunchKernelGGL((k_b_init<P>), dim3(blocks), dim3(256), 0, s, A);
     }
     for (uint32_t r = r0; r < A.c.rounds; r++)

@@ -825,6 +825,7 @@ void init_nestedmc_sampler(uint32_t cores, const nested_sampler_params* params) 
 
 // cleanup_nestedmc_sampler (common.cpp:28-31).
 void cleanup_nestedmc_sampler(void) {
+    g_engine.free_batch();
     g_engine.free_samplers();
     g_engine.free_portfolio();
     g_engine.free_batched();

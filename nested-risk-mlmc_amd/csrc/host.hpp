@@ -119,6 +119,33 @@ struct Engine {
     size_t sweep_exec_nodes = 0;              // its node count
     int sample_levels(uint32_t n, const uint32_t* ells, const uint64_t* Ms, sample_output* outs);
     int ensure_pf_R(uint64_t M);
+    int ensure_R(size_t elems);
+
+    // A batch of portfolio problems sampled in one launch chain per level call
+    // (nmlmc_batch_*): each instance keeps its tables, seed and outer ids.
+    struct BatchInst {
+        PortfolioDevice pf{};
+        std::vector<void*> allocs;
+        double bs_work = 0.;
+        uint64_t seed = 0;
+        bool staged = false, exact_only = false;
+        std::vector<uint64_t> next_outer;
+    };
+    std::vector<BatchInst> batch;
+    bool batch_dirty = true;                 // device copies of the instance records are stale
+    PortfolioDevice* d_bpf = nullptr;        // [n] instance records
+    uint64_t* d_bseed = nullptr;             // [n]
+    double* d_bwork = nullptr;               // [n]
+    void* d_btab = nullptr;                  // per-chunk segment tables (grown)
+    void* h_btab = nullptr;                  // pinned staging of the same
+    size_t btab_cap = 0;
+    Partial* d_inst_out = nullptr;           // [chunks x n] per-instance accumulators (grown)
+    Partial* h_inst_out = nullptr;           // pinned
+    size_t inst_out_cap = 0;
+    int batch_begin(uint32_t n);
+    int batch_stage(uint32_t i, const portfolio_data* data, uint64_t seed, const double* chol, uint32_t d);
+    int batch_sample(uint32_t ell, const uint64_t* Ms, sample_output* outs);
+    void free_batch();
     int prepare_batched(uint64_t M, BatchedCommon& c);
     int batched_common(uint32_t ell, uint64_t M, uint64_t outer_offset, BatchedCommon& c);
     int run_batched(uint32_t ell, uint64_t M, BatchedCommon& c, sample_output* out,

@@ -19,9 +19,11 @@ struct ModelPolicy {
         ModelParams mp;
     };
     static constexpr uint64_t spu = (VAR == MODEL_SHIFT) ? 2 : 1;
+    static constexpr bool kMulti = false;
+    static constexpr bool kFuse = true;  // k_b_fused is instantiated
 
     // init_inner_sampler does no work for the model (model_sampler.cpp:64).
-    __device__ static double group_work(const Args&) { return 0.; }
+    __device__ static double group_work(const Args&, const ItemState&) { return 0.; }
 
     // Mixed form: an FP32 mean within this distance of the threshold is
     // re-evaluated in FP64 (mean_mag: mean of the inner terms' magnitudes).
@@ -38,13 +40,15 @@ struct ModelPolicy {
 
     // model::sampler::sample_outer (model_sampler.cpp:57-59) and
     // init_inner_sampler (64).
-    __device__ static void init_item(const Args& A, uint64_t m, ItemState& it, LaneAcc& acc) {
+    template <class ACC>
+    __device__ static void init_item(const Args& A, uint64_t m, ItemState& it, ACC& acc) {
         const u64x2 w = threefry2x64_20(0, 0, A.c.seed, key1(A, m));
         double x, y;
         boxmuller_f64(w.a, w.b, x, y);
         it.Y = x;
         it.start = -A.mp.max_loss;
-        acc.st[5] += 1;
+        it.inst = 0;
+        acc.st(0, 5, 1);
     }
 
     // Inner samples of units [u0, u1) restricted to samples [i0, i1)

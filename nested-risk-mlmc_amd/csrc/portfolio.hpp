@@ -104,6 +104,21 @@ struct PortfolioDeltaHost {
 };
 hipError_t launch_portfolio_delta(const PortfolioDeltaHost& H, hipStream_t s);
 
+// A batch of portfolio problems in one chain: the chain's outer samples come in
+// segments, each of one instance (device tables; host-built per chunk).
+struct PortfolioBatchTables {
+    const PortfolioDevice* inst;  // [n_inst]
+    const uint64_t* inst_seed;    // [n_inst]
+    const double* inst_bs_work;   // [n_inst]
+    const uint64_t* seg_item0;    // [n_seg + 1]: first item of each segment, then the chunk's item count
+    const uint32_t* seg_inst;     // [n_seg]
+    const uint64_t* seg_outer0;   // [n_seg]: outer id of the segment's first item
+    const uint32_t* seg_blk0;     // [n_seg + 1]: first block of each segment in the outer-draw kernel
+    uint32_t n_seg;
+    uint32_t outer_blocks;        // blocks of the outer-draw kernel (0: no instance has a Cholesky factor)
+    uint32_t dmax;                // largest d of the batch (row stride of R_tau)
+};
+
 struct PortfolioBatchedHost {
     BatchedCommon c;
     PortfolioDevice pf;
@@ -112,6 +127,8 @@ struct PortfolioBatchedHost {
     uint32_t blocks;
     bool mixed;
     bool exact_only;  // sub-sampling mode 1 and every option Exact (method 1)
+    bool multi;       // a batch of problems (tables below; pf and bs_work unused)
+    PortfolioBatchTables bt;
 };
 hipError_t launch_portfolio_batched(const PortfolioBatchedHost& H, hipStream_t s);
 

@@ -16,17 +16,45 @@ namespace nmlmc {
 // EXACT_ONLY: every option is priced by Exact-GBM inner sampling with
 // sub-sampling on (the benchmark books); the other methods' code is left out
 // of the kernels.  Draws and results are those of the general policy.
-template <bool EXACT_ONLY>
+// MULTI: a batch of problems in one chain (PortfolioBatchTables): an outer
+// sample takes the tables, seed and outer id of its instance, so that its
+// draws and results are those of the same outer sample of that problem staged
+// alone.
+template <bool EXACT_ONLY, bool MULTI = false>
 struct PortfolioPolicyT {
     struct Args {
         BatchedCommon c;
         PortfolioDevice pf;
-        double* R_tau;  // [M x d]: stock i of outer sample m at R_tau[m * d + i]
+        double* R_tau;  // [M x d] (MULTI: [M x dmax]): stock i of outer sample m at R_tau[m * stride + i]
         double bs_work; // work of one init_inner_sampler call (sub-sampling off)
+        PortfolioBatchTables bt;
     };
     static constexpr uint64_t spu = 1;
+    static constexpr bool kMulti = MULTI;
+    // The fused first pass only for the Exact-only books: the general policy's
+    // inner loop (BS, Exact, Unbiased) is too large to share a kernel with the
+    // outer draw and the final at a useful occupancy.
+    static constexpr bool kFuse = EXACT_ONLY;
 
-    __device__ static double group_work(const Args& A) { return A.bs_work; }
+    __device__ static const PortfolioDevice& pf_of(const Args& A, const ItemState& it) {
+        return MULTI ? A.bt.inst[it.inst] : A.pf;
+    }
+    __device__ static uint64_t seed_of(const Args& A, const ItemState& it) {
+        return MULTI ? A.bt.inst_seed[it.inst] : A.c.seed;
+    }
+    __device__ static double group_work(const Args& A, const ItemState& it) {
+        return MULTI ? A.bt.inst_bs_work[it.inst] : A.bs_work;
+    }
+    // MULTI: the segment holding item m (largest k with seg_item0[k] <= m).
+    __device__ static uint32_t segment(const Args& A, uint64_t m) {
+        uint32_t lo = 0, hi = A.bt.n_seg - 1;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi + 1) >> 1;
+            if (A.bt.seg_item0[mid] <= m) lo = mid;
+            else hi = mid - 1;
+        }
+        return lo;
+    }
 
     // Mixed form: each FP32 inner term differs from its FP64 counterpart by at
     // most 2^-18 of its magnitude (inner_exact_mix; for the other methods' FP32
@@ -36,29 +64,35 @@ struct PortfolioPolicyT {
     __device__ static double mixed_bound(double mean_mag, double) { return 0x1p-18 * mean_mag; }
 
     __device__ static uint64_t key1(const Args& A, uint64_t m) {
+        if (MULTI) {
+            const uint32_t k = segment(A, m);
+            return ((uint64_t)A.c.ell << 46) | (A.bt.seg_outer0[k] + (m - A.bt.seg_item0[k]));
+        }
         return ((uint64_t)A.c.ell << 46) | (A.c.offset + m);
     }
 
-    // Stock levels of outer sample m, column-wise over the chunk's outer samples.
     // Stock levels of outer sample m: one row of d values (kept in cache while
     // the outer sample's inner samples gather from it).
+    __device__ static uint32_t rstride(const Args& A) { return MULTI ? A.bt.dmax : A.pf.d; }
     __device__ static StridedR stocks_at_tau(const Args& A, uint64_t m) {
-        return StridedR{A.R_tau + m * A.pf.d, 1};
+        return StridedR{A.R_tau + m * rstride(A), 1};
     }
 
     // Outer draw (sample_outer, portfolio_sampler.cpp:293-311): the one-factor
     // model here; with a Cholesky factor, k_pf_outer_mfma has already written
     // the stock levels.  Then the delta control variate's constant and, without
     // sub-sampling, the pre-summed BS options (init_inner_sampler, 312-324).
-    __device__ static void init_item(const Args& A, uint64_t m, ItemState& it, LaneAcc& acc) {
-        const PortfolioDevice& pf = A.pf;
-        double* Rw = A.R_tau + m * pf.d;
+    template <class ACC>
+    __device__ static void init_item(const Args& A, uint64_t m, ItemState& it, ACC& acc) {
+        it.inst = MULTI ? A.bt.seg_inst[segment(A, m)] : 0;
+        const PortfolioDevice& pf = pf_of(A, it);
+        double* Rw = A.R_tau + m * rstride(A);
         const uint64_t stride = 1;
         if (pf.chol) {
-            acc.st[5] += pf.d;
+            acc.st(it.inst, 5, pf.d);
         } else {
             DStream os;
-            os.init(A.c.seed, key1(A, m), 0, 0, 0., 0);
+            os.init(seed_of(A, it), key1(A, m), 0, 0, 0., 0);
             const double dW_sys = pf.stau * os.randn();
             if (pf.d == 1) {
                 const StockRec st = pf.stocks[0];
@@ -71,7 +105,7 @@ struct PortfolioPolicyT {
                     Rw[(uint64_t)i * stride] = simulate_stock_d(st.S0, st.mu, st.sigma, pf.tau, W);
                 }
             }
-            acc.st[5] += os.nn;
+            acc.st(it.inst, 5, os.nn);
         }
         const StridedR R = stocks_at_tau(A, m);
         double cv = 0.;
@@ -88,7 +122,8 @@ struct PortfolioPolicyT {
     template <bool F32>
     __device__ static void units(const Args& A, uint32_t m, const ItemState& it, uint32_t tag, uint64_t u0,
                                  uint64_t u1, uint64_t, uint64_t, SegAcc& s) {
-        const PortfolioDevice& pf = A.pf;
+        const PortfolioDevice& pf = pf_of(A, it);
+        const uint64_t seed = seed_of(A, it);
         const StridedR R = stocks_at_tau(A, m);
         const uint64_t k1 = key1(A, m);
         double a1 = 0., a2 = 0., ab = 0., work = 0.;
@@ -113,23 +148,23 @@ struct PortfolioPolicyT {
             uint64_t u = u0;
             for (; u + 1 < u1; u += 2) {
                 const uint64_t ca = ((uint64_t)tag << 40) | u, cb = ca + 1;
-                const u64x2 a0 = threefry2x64_20(0, ca, A.c.seed, k1);
-                const u64x2 b0 = threefry2x64_20(0, cb, A.c.seed, k1);
+                const u64x2 a0 = threefry2x64_20(0, ca, seed, k1);
+                const u64x2 b0 = threefry2x64_20(0, cb, seed, k1);
                 const uint32_t ia = pick_cdf(pf, 1, u01_f64(a0.a) * cp);
                 const uint32_t ib = pick_cdf(pf, 1, u01_f64(b0.a) * cp);
                 const MixRec qa = pf.mix[ia], qb = pf.mix[ib];
                 const float Ra = (float)R[qa.stype >> 8], Rb = (float)R[qb.stype >> 8];
-                const u64x2 a1 = threefry2x64_20(1, ca, A.c.seed, k1);
-                const u64x2 b1k = threefry2x64_20(1, cb, A.c.seed, k1);
+                const u64x2 a1 = threefry2x64_20(1, ca, seed, k1);
+                const u64x2 b1k = threefry2x64_20(1, cb, seed, k1);
                 term(qa, Ra, a1, ia);
                 term(qb, Rb, b1k, ib);
             }
             if (u < u1) {
                 const uint64_t ca = ((uint64_t)tag << 40) | u;
-                const u64x2 a0 = threefry2x64_20(0, ca, A.c.seed, k1);
+                const u64x2 a0 = threefry2x64_20(0, ca, seed, k1);
                 const uint32_t ia = pick_cdf(pf, 1, u01_f64(a0.a) * cp);
                 const MixRec qa = pf.mix[ia];
-                term(qa, (float)R[qa.stype >> 8], threefry2x64_20(1, ca, A.c.seed, k1), ia);
+                term(qa, (float)R[qa.stype >> 8], threefry2x64_20(1, ca, seed, k1), ia);
             }
             s.nu += 2 * (u1 - u0);
             s.nn += 2 * (u1 - u0);
@@ -145,8 +180,8 @@ struct PortfolioPolicyT {
                 // uniform (only method 1 has options), 1 option uniform (block 0),
                 // 2 and 3 the Box-Muller pair (block 1).
                 const uint64_t cu = ((uint64_t)tag << 40) | u;
-                const u64x2 blk0 = threefry2x64_20(0, cu, A.c.seed, k1);
-                const u64x2 blk1 = threefry2x64_20(1, cu, A.c.seed, k1);
+                const u64x2 blk0 = threefry2x64_20(0, cu, seed, k1);
+                const u64x2 blk1 = threefry2x64_20(1, cu, seed, k1);
                 const uint32_t idx = pick_cdf(pf, 1, u01_f64(blk0.a) * pf.compute_prob[1]);
                 s.nu += 2;
                 s.nn += 2;
@@ -161,7 +196,7 @@ struct PortfolioPolicyT {
                 continue;
             }
             DStream st;
-            st.init(A.c.seed, k1, ((uint64_t)tag << 40) | u, 0, 0., 0);
+            st.init(seed, k1, ((uint64_t)tag << 40) | u, 0, 0., 0);
             if (F32 && pf.subsample != 0) {
                 uint32_t method, idx;
                 const double prob = pick_option(pf, st, st, method, idx);
@@ -224,20 +259,38 @@ struct PortfolioPolicyT {
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 constexpr uint32_t kOuterWaves = 16;
 
+// A batch of problems (P::kMulti): block b draws 64 outer samples of one
+// segment (one instance), whose first block is seg_blk0[k]; blocks of instances
+// without a Cholesky factor return at once (their draw is in init_item).
 template <class P>
 __global__ __launch_bounds__(64 * kOuterWaves) void k_pf_outer_mfma(typename P::Args A) {
     extern __shared__ double zt[];  // [4 ks][64]
-    const PortfolioDevice& pf = A.pf;
-    const uint32_t d = pf.d, ks = (d + 3) / 4, nt = (d + 15) / 16;
+    uint64_t m0 = (uint64_t)blockIdx.x * 64, m_end = A.c.M;
+    const PortfolioDevice* ppf = &A.pf;
+    uint64_t seed = A.c.seed;
+    if (P::kMulti) {
+        uint32_t lo = 0, hi = A.bt.n_seg - 1;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi + 1) >> 1;
+            if (A.bt.seg_blk0[mid] <= blockIdx.x) lo = mid;
+            else hi = mid - 1;
+        }
+        m0 = A.bt.seg_item0[lo] + 64 * (uint64_t)(blockIdx.x - A.bt.seg_blk0[lo]);
+        m_end = A.bt.seg_item0[lo + 1];
+        ppf = &A.bt.inst[A.bt.seg_inst[lo]];
+        seed = A.bt.inst_seed[A.bt.seg_inst[lo]];
+        if (!ppf->chol) return;
+    }
+    const PortfolioDevice& pf = *ppf;
+    const uint32_t d = pf.d, ks = (d + 3) / 4, nt = (d + 15) / 16, rs = P::rstride(A);
     const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const uint64_t m0 = (uint64_t)blockIdx.x * 64;
     {
         const uint64_t m = m0 + lane;
-        const uint64_t k1 = P::key1(A, m < A.c.M ? m : 0);
+        const uint64_t k1 = P::key1(A, m < m_end ? m : m0);
         for (uint32_t b = wv; b < 2 * ks; b += kOuterWaves) {  // pairs 0 .. 2 ks - 1 fill rows 0 .. 4 ks - 1
             double x = 0., y = 0.;
             if (2 * b < d) {
-                const u64x2 blk = threefry2x64_20(b, 0, A.c.seed, k1);
+                const u64x2 blk = threefry2x64_20(b, 0, seed, k1);
                 boxmuller_f64(blk.a, blk.b, x, y);  // words 2b (element b) and 2b + 1 (element a)
             }
             zt[(2 * b) * 64 + lane] = x;
@@ -261,30 +314,38 @@ __global__ __launch_bounds__(64 * kOuterWaves) void k_pf_outer_mfma(typename P::
 #pragma unroll
         for (int i = 0; i < 4; i++) {
             const uint64_t m = m0 + 16 * mt + (lane >> 4) + 4 * i;
-            if (m < A.c.M) A.R_tau[m * d + n] = simulate_stock_d(st.S0, st.mu, st.sigma, pf.tau, pf.stau * acc[i]);
+            if (m < m_end) A.R_tau[m * rs + n] = simulate_stock_d(st.S0, st.mu, st.sigma, pf.tau, pf.stau * acc[i]);
         }
     }
 }
 
-template <bool EXACT_ONLY>
-hipError_t PortfolioPolicyT<EXACT_ONLY>::launch_outer(const Args& A, hipStream_t s) {
-    if (!A.pf.chol || A.c.M == 0) return hipSuccess;
-    const uint32_t blocks = (uint32_t)((A.c.M + 63) / 64);
-    const size_t lds = sizeof(double) * 64 * 4 * ((A.pf.d + 3) / 4);  // <= 128 KB for d <= 256
-    hipLaunchKernelGGL((k_pf_outer_mfma<PortfolioPolicyT<EXACT_ONLY>>), dim3(blocks), dim3(64 * kOuterWaves), lds, s,
-                       A);
+template <bool EXACT_ONLY, bool MULTI>
+hipError_t PortfolioPolicyT<EXACT_ONLY, MULTI>::launch_outer(const Args& A, hipStream_t s) {
+    uint32_t blocks, dk;
+    if (MULTI) {
+        blocks = A.bt.outer_blocks;
+        dk = A.bt.dmax;
+    } else {
+        blocks = A.pf.chol ? (uint32_t)((A.c.M + 63) / 64) : 0;
+        dk = A.pf.d;
+    }
+    if (blocks == 0 || A.c.M == 0) return hipSuccess;
+    const size_t lds = sizeof(double) * 64 * 4 * ((dk + 3) / 4);  // <= 128 KB for d <= 256
+    hipLaunchKernelGGL((k_pf_outer_mfma<PortfolioPolicyT<EXACT_ONLY, MULTI>>), dim3(blocks), dim3(64 * kOuterWaves),
+                       lds, s, A);
     return hipGetLastError();
 }
 
-template <bool EXACT_ONLY>
+template <bool EXACT_ONLY, bool MULTI>
 static hipError_t launch_kind(const PortfolioBatchedHost& H, hipStream_t s) {
-    using P = PortfolioPolicyT<EXACT_ONLY>;
-    typename P::Args A{H.c, H.pf, H.R_tau, H.bs_work};
+    using P = PortfolioPolicyT<EXACT_ONLY, MULTI>;
+    typename P::Args A{H.c, H.pf, H.R_tau, H.bs_work, H.bt};
     return H.mixed ? launch_batched_level<P, true>(A, H.blocks, s) : launch_batched_level<P, false>(A, H.blocks, s);
 }
 
 hipError_t launch_portfolio_batched(const PortfolioBatchedHost& H, hipStream_t s) {
-    return H.exact_only ? launch_kind<true>(H, s) : launch_kind<false>(H, s);
+    if (H.multi) return H.exact_only ? launch_kind<true, true>(H, s) : launch_kind<false, true>(H, s);
+    return H.exact_only ? launch_kind<true, false>(H, s) : launch_kind<false, false>(H, s);
 }
 
 }  // namespace nmlmc

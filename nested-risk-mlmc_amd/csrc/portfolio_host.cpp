@@ -266,8 +266,9 @@ int Engine::sample_portfolio_parity(uint32_t ell, uint64_t M, uint32_t, sample_o
 }
 
 // Stock states at tau of M outer samples (batched forms; grown, never shrunk).
-int Engine::ensure_pf_R(uint64_t M) {
-    const size_t need = (size_t)M * pf.d;
+int Engine::ensure_pf_R(uint64_t M) { return ensure_R((size_t)M * pf.d); }
+
+int Engine::ensure_R(size_t need) {
     if (need <= pf_R_cap) return NMLMC_OK;
     size_t cap = pf_R_cap ? pf_R_cap : 1024;
     if (pf_R) hipFree(pf_R);

@@ -50,10 +50,13 @@ hipError_t launch_reduce_partials(const Partial* P, uint32_t n, Partial* out, hi
     return hipGetLastError();
 }
 
-__global__ void k_b_zero(BatchedCtl* ctl) {
+__global__ void k_b_zero(BatchedCtl* ctl, Partial* inst_out, uint32_t n_inst) {
     uint32_t* p = (uint32_t*)ctl;
     const uint32_t n = sizeof(BatchedCtl) / 4;
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
+    uint32_t* q = (uint32_t*)inst_out;
+    const uint32_t nq = inst_out ? n_inst * (uint32_t)(sizeof(Partial) / 4) : 0;
+    for (uint32_t i = threadIdx.x; i < nq; i += blockDim.x) q[i] = 0;
 }
 
 __global__ void k_b_final_alloc(BatchedCtl* ctl, uint32_t fcap) {

@@ -86,6 +86,7 @@ EXPORTS = [
     "nmlmc_mlmc_run", "nmlmc_mlmc_run_cb", "portfolio_sample_delta_unbiased",
     "portfolio_estimate_delta_for_unbiased", "portfolio_gen_data", "portfolio_free_data",
     "nmlmc_mlmc_last_table", "nmlmc_autotune_level", "nmlmc_sample_levels",
+    "nmlmc_batch_begin", "nmlmc_batch_stage", "nmlmc_batch_sample",
 ]
 
 _lib = None
@@ -135,6 +136,9 @@ def load(path=LIB_PATH):
         "nmlmc_last_group_lanes": (u32, []),
         "nmlmc_autotune_level": (i32, [u32, u64, C.POINTER(u32), C.POINTER(u32), dp]),
         "nmlmc_sample_levels": (i32, [u32, C.POINTER(u32), C.POINTER(u64), C.POINTER(SampleOutput)]),
+        "nmlmc_batch_begin": (i32, [u32]),
+        "nmlmc_batch_stage": (i32, [u32, C.POINTER(PortfolioData), u64, dp, u32]),
+        "nmlmc_batch_sample": (i32, [u32, C.POINTER(u64), C.POINTER(SampleOutput)]),
         "nmlmc_mlmc_run": (i32, [dbl, u64, dbl, dbl, dbl, i32, u32, C.POINTER(LevelStats),
                                  C.POINTER(C.c_uint32), dp]),
         "nmlmc_mlmc_run_cb": (i32, [SAMPLE_LEVEL_FN, C.c_void_p, dbl, u64, dbl, dbl, dbl, i32, u32,
@@ -428,6 +432,36 @@ def portfolio_set_correlation(chol):
 
 def portfolio_cleanup():
     load().portfolio_cleanup()
+
+
+def batch_begin(n):
+    """Open a batch of n portfolio problems (nmlmc_batch_begin)."""
+    check(load().nmlmc_batch_begin(n))
+
+
+def batch_stage(i, pf, seed, chol=None):
+    """Stage instance i of the batch: tables of Portfolio pf, its seed and
+    optional Cholesky factor."""
+    lib = load()
+    if chol is None:
+        check(lib.nmlmc_batch_stage(i, C.byref(pf.c), seed, None, 0))
+        return
+    a = np.ascontiguousarray(chol, np.float64)
+    check(lib.nmlmc_batch_stage(i, C.byref(pf.c), seed, a.ctypes.data_as(C.POINTER(C.c_double)), a.shape[0]))
+
+
+def batch_sample(ell, Ms, moments=4, phi_size=1):
+    """One level call over every instance of the batch: one dict per instance."""
+    lib = load()
+    n = len(Ms)
+    outs = [Output(moments, phi_size) for _ in range(n)]
+    arr = (SampleOutput * n)(*[o.c for o in outs])
+    check(lib.nmlmc_batch_sample(ell, (C.c_uint64 * n)(*Ms), arr))
+    res = []
+    for i, o in enumerate(outs):
+        o.c = arr[i]
+        res.append(o.as_dict())
+    return res
 
 
 def portfolio_sample_delta_unbiased(option_index, M, moments=2):

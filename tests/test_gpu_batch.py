@@ -1,0 +1,102 @@
+"""A batch of portfolio problems in one launch chain per level call: each
+instance's results equal those of the same problem staged alone with the same
+seed and sampled with the same level calls (per-sample draws are keyed by the
+instance's seed and outer ids, so the level sums are the same numbers)."""
+import importlib
+
+import numpy as np
+import pytest
+
+from portfolio_util import synthetic, to_library
+
+pytestmark = pytest.mark.gpu
+
+SCAL = dict(r=0.05, tau=0.02, rho=0.5, dt_h0inv=8, dt_beta=4, dt_prob_expo=-1.5)
+
+
+def _books(nm):
+    """Three Exact-only books of different sizes and dimensions, the second
+    with a Cholesky factor."""
+    out = []
+    for k, (P, Q, chol) in enumerate(((120, 3, False), (400, 7, True), (60, 2, False))):
+        d = synthetic(P=P, Q=Q, counts=(0, P, 0), digitals=True, seed=11 + k)
+        sc = dict(SCAL, max_loss=0.4 * (k + 1), subsample=1, use_delta_cv=True, use_antithetic_risk=True,
+                  phi_type=0, phi_size=1)
+        L = None
+        if chol:
+            rs = np.random.RandomState(5)
+            A = rs.uniform(0, 0.5, (Q, 2))
+            C = A @ A.T
+            np.fill_diagonal(C, 1.0)
+            L = np.linalg.cholesky(C)
+        out.append((d, sc, L, 100 + 7 * k))
+    return out
+
+
+def _alone(nm, form, book, calls):
+    d, sc, L, seed = book
+    nm.init_nestedmc_sampler(1, 16, 2, 0.0, True)
+    nm.set_adaptive_rule(1, 3)
+    nm.set_form(form)
+    pf, _, _ = to_library(nm, d, **sc)
+    nm.portfolio_init(pf, seed)
+    if L is not None:
+        nm.portfolio_set_correlation(L)
+    res = [nm.sample(ell, M) for ell, M in calls]
+    nm.cleanup_nestedmc_sampler()
+    return res
+
+
+def _key(r):
+    return (tuple(r["fsums"]), tuple(r["diffsums"]), r["work"], tuple(int(x) for x in r["stats"]), r["M"])
+
+
+@pytest.mark.parametrize("chunk", [None, "700"])
+@pytest.mark.parametrize("form", [1, 2])
+def test_batch_matches_instances_alone(nm, form, chunk, monkeypatch):
+    """Three instances (different d, one with a Cholesky factor, different
+    thresholds and seeds) in one chain per level call, over levels 0..2 and a
+    repeated level (continuing outer ids); with a chunk of 700 outer samples the
+    instances straddle chunk boundaries.  Level sums, work and counts equal the
+    lone runs' (indicator sums and counts are integers; the work sums of
+    integers too)."""
+    if chunk:
+        monkeypatch.setenv("NMLMC_BATCH_CHUNK", chunk)
+    books = _books(nm)
+    Ms = [[600, 250, 900], [200, 90, 300], [80, 40, 120], [500, 100, 50]]
+    ells = [0, 1, 2, 0]
+    alone = [_alone(nm, form, b, [(e, M[i]) for e, M in zip(ells, Ms)]) for i, b in enumerate(books)]
+    nm.init_nestedmc_sampler(1, 16, 2, 0.0, True)
+    nm.set_adaptive_rule(1, 3)
+    nm.set_form(form)
+    nm.batch_begin(len(books))
+    keep = []
+    for i, (d, sc, L, seed) in enumerate(books):
+        pf, _, _ = to_library(nm, d, **sc)
+        keep.append(pf)
+        nm.batch_stage(i, pf, seed, L)
+    for c, (ell, M) in enumerate(zip(ells, Ms)):
+        got = nm.batch_sample(ell, M)
+        for i in range(len(books)):
+            assert _key(got[i]) == _key(alone[i][c]), (c, i)
+    nm.cleanup_nestedmc_sampler()
+    nm.set_form(nm.FORM_PLAIN)
+
+
+def test_batch_errors(nm):
+    """A batch needs staged instances and a batched form; at most 64 instances."""
+    nm.init_nestedmc_sampler(1, 16, 2, 0.0, True)
+    nm.set_form(nm.FORM_MIXED)
+    with pytest.raises(nm.NmlmcError):
+        nm.batch_begin(65)
+    nm.batch_begin(2)
+    d, sc, L, seed = _books(nm)[0]
+    pf, _, _ = to_library(nm, d, **sc)
+    nm.batch_stage(0, pf, seed)
+    with pytest.raises(nm.NmlmcError):
+        nm.batch_sample(0, [10, 10])  # instance 1 not staged
+    nm.batch_stage(1, pf, seed + 1)
+    nm.set_form(nm.FORM_PLAIN)
+    with pytest.raises(nm.NmlmcError):
+        nm.batch_sample(0, [10, 10])
+    nm.cleanup_nestedmc_sampler()
