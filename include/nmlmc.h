@@ -271,6 +271,17 @@ int nmlmc_mlmc_run_cb(nmlmc_sample_level_fn fn, void* user, double targetTOL, ui
                       uint32_t max_levels, nmlmc_level_stats* levels_out, uint32_t* n_levels,
                       double* summary_out);
 
+/* The MLMC drivers of the staged batch (nmlmc_batch_*) run together (no
+ * counterpart in the reference): instance i runs mlmc_t::run to targetTOL[i]
+ * with the other parameters given, in a host thread of its own; each step's
+ * level requests of all running instances are sampled by one batch level call
+ * per level.  Per instance: its levels at levels_out[i * max_levels ...], the
+ * level count at n_levels[i], the summary at summary_out[i * 5 ...] as
+ * nmlmc_mlmc_run gives them, with the whole batch's wall time. */
+int nmlmc_batch_mlmc_run(const double* targetTOL, uint64_t M0, double theta, double Ca,
+                         double starting_level_coeff, uint32_t max_levels, nmlmc_level_stats* levels_out,
+                         uint32_t* n_levels, double* summary_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,9 +1,14 @@
 // C entry points of the MLMC driver (mlmc.hpp:290-337) over the GPU level call
 // or a caller-supplied level sampler.
 #include <chrono>
+#include <condition_variable>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <stdexcept>
 #include <string>
+#include <thread>
+#include <vector>
 
 #include "../../include/nmlmc.h"
 #include "host.hpp"
@@ -18,17 +23,9 @@ struct abort_run {
 
 std::string g_last_table;  // mlmc_output_t::to_string of the last completed run
 
-int run_driver(mlmc_real::mlmc_input_t& in, uint32_t max_levels, nmlmc_level_stats* levels_out,
-               uint32_t* n_levels, double* summary_out) {
-    const auto t0 = std::chrono::steady_clock::now();
-    mlmc_real::mlmc_output_t out;
-    try {
-        out = mlmc_real::run(in);
-    } catch (const abort_run& a) {
-        return a.code;
-    }
-    const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    g_last_table = out.to_string();
+// Copies a finished run into the C outputs.
+void export_run(const mlmc_real::mlmc_output_t& out, double wall, uint32_t max_levels,
+                nmlmc_level_stats* levels_out, uint32_t* n_levels, double* summary_out) {
     const uint32_t n = (uint32_t)out.levels.size();
     if (n_levels) *n_levels = n;
     if (levels_out) {
@@ -50,6 +47,20 @@ int run_driver(mlmc_real::mlmc_input_t& in, uint32_t max_levels, nmlmc_level_sta
         summary_out[3] = out.TOL;
         summary_out[4] = wall;
     }
+}
+
+int run_driver(mlmc_real::mlmc_input_t& in, uint32_t max_levels, nmlmc_level_stats* levels_out,
+               uint32_t* n_levels, double* summary_out) {
+    const auto t0 = std::chrono::steady_clock::now();
+    mlmc_real::mlmc_output_t out;
+    try {
+        out = mlmc_real::run(in);
+    } catch (const abort_run& a) {
+        return a.code;
+    }
+    const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    g_last_table = out.to_string();
+    export_run(out, wall, max_levels, levels_out, n_levels, summary_out);
     return NMLMC_OK;
 }
 
@@ -62,6 +73,83 @@ void fill_input(mlmc_real::mlmc_input_t& in, double targetTOL, uint64_t M0, doub
     in.starting_level_coeff = slc;
     in.verbose = verbose != 0;
 }
+
+// The MLMC drivers of a batch of problems, run together: one host thread per
+// instance runs mlmc_t::run unchanged, and its level requests go to the
+// calling thread, which waits until every running instance has asked for its
+// next step (or finished), then samples each requested level for all of them
+// in one batch level call (Engine::batch_sample) and hands the results back.
+// Only the calling thread touches the GPU.
+struct BatchDriver {
+    struct Req {
+        std::vector<uint32_t> ells;
+        std::vector<uint64_t> Ms;
+    };
+    std::mutex mu;
+    std::condition_variable cv_main, cv_work;
+    std::vector<char> waiting, done;
+    std::vector<Req> req;
+    std::vector<std::vector<mlmc_real::mc_diff_output_t>> res;
+    int error = 0;
+
+    std::vector<mlmc_real::mc_diff_output_t> ask(uint32_t i, const std::vector<uint32_t>& ells,
+                                                 const std::vector<uint64_t>& Ms) {
+        std::unique_lock<std::mutex> lk(mu);
+        req[i] = Req{ells, Ms};
+        waiting[i] = 1;
+        cv_main.notify_one();
+        cv_work.wait(lk, [&] { return !waiting[i] || error; });
+        if (error) throw abort_run{error};
+        return std::move(res[i]);
+    }
+
+    // Samples every pending request: one batch level call per requested level.
+    int serve(uint32_t n) {
+        std::map<uint32_t, std::vector<uint64_t>> levels;
+        for (uint32_t i = 0; i < n; i++) {
+            if (!waiting[i]) continue;
+            res[i].assign(req[i].ells.size(), mlmc_real::mc_diff_output_t());
+            for (size_t j = 0; j < req[i].ells.size(); j++) {
+                std::vector<uint64_t>& Ms = levels[req[i].ells[j]];
+                Ms.resize(n, 0);
+                Ms[i] += req[i].Ms[j];
+            }
+        }
+        const size_t w = kMaxPhi * kMaxMoments;
+        std::vector<double> fs(n * w), ds(n * w);
+        std::vector<sample_output> outs(n);
+        for (const auto& lv : levels) {
+            std::fill(fs.begin(), fs.end(), 0.);
+            std::fill(ds.begin(), ds.end(), 0.);
+            for (uint32_t i = 0; i < n; i++) {
+                std::memset(&outs[i], 0, sizeof(sample_output));
+                outs[i].fsums = &fs[i * w];
+                outs[i].diffsums = &ds[i * w];
+            }
+            const auto t0 = std::chrono::steady_clock::now();
+            const int rc = g_engine.batch_sample(lv.first, lv.second.data(), outs.data());
+            if (rc) return rc;
+            const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            double wsum = 0.;
+            for (uint32_t i = 0; i < n; i++) wsum += outs[i].total_work;
+            for (uint32_t i = 0; i < n; i++) {
+                if (!waiting[i] || lv.second[i] == 0) continue;
+                for (size_t j = 0; j < req[i].ells.size(); j++) {
+                    if (req[i].ells[j] != lv.first) continue;
+                    mlmc_real::mc_diff_output_t& r = res[i][j];
+                    for (int k = 0; k < 4; k++) {
+                        r.sums_fine[k] = fs[i * w + k];
+                        r.sums_diff[k] = ds[i * w + k];
+                    }
+                    r.work = outs[i].total_work;
+                    r.M = outs[i].M;
+                    r.time = wsum > 0. ? dt * outs[i].total_work / wsum : 0.;
+                }
+            }
+        }
+        return NMLMC_OK;
+    }
+};
 }  // namespace
 
 }  // namespace nmlmc
@@ -161,6 +249,78 @@ int nmlmc_mlmc_run_cb(nmlmc_sample_level_fn fn, void* user, double targetTOL, ui
     };
     int rc = run_driver(in, max_levels, levels_out, n_levels, summary_out);
     if (rc) return set_error(NMLMC_ERR_ARG, "sample_level callback returned %d", rc);
+    return set_error(NMLMC_OK, "");
+}
+
+// A batch of MLMC runs (no counterpart in the reference, which runs one
+// problem per process): the instances of the staged batch, instance i to
+// targetTOL[i], all with the other driver parameters given; per instance its
+// levels at levels_out[i * max_levels ...], the level count at n_levels[i] and
+// the summary at summary_out[i * 5 ...] as nmlmc_mlmc_run gives them (wall =
+// the whole batch's time).
+int nmlmc_batch_mlmc_run(const double* targetTOL, uint64_t M0, double theta, double Ca,
+                         double starting_level_coeff, uint32_t max_levels, nmlmc_level_stats* levels_out,
+                         uint32_t* n_levels, double* summary_out) {
+    const uint32_t n = (uint32_t)g_engine.batch.size();
+    if (n == 0) return set_error(NMLMC_ERR_STATE, "no batch (nmlmc_batch_begin)");
+    if (!targetTOL) return set_error(NMLMC_ERR_ARG, "targetTOL is NULL");
+    for (uint32_t i = 0; i < n; i++)
+        if (!(targetTOL[i] > 0)) return set_error(NMLMC_ERR_ARG, "targetTOL[%u] must be positive", i);
+    BatchDriver B;
+    B.waiting.assign(n, 0);
+    B.done.assign(n, 0);
+    B.req.resize(n);
+    B.res.resize(n);
+    std::vector<mlmc_real::mlmc_output_t> outs(n);
+    std::vector<int> codes(n, 0);
+    const auto t0 = std::chrono::steady_clock::now();
+    std::vector<std::thread> th;
+    for (uint32_t i = 0; i < n; i++) {
+        th.emplace_back([&, i]() {
+            mlmc_real::mlmc_input_t in;
+            fill_input(in, targetTOL[i], M0, theta, Ca, starting_level_coeff, 0);
+            in.sample_levels = [&, i](const std::vector<uint32_t>& ells, const std::vector<uint64_t>& Ms) {
+                return B.ask(i, ells, Ms);
+            };
+            in.sample_level = [&, i](uint32_t ell, uint64_t M) { return B.ask(i, {ell}, {M})[0]; };
+            try {
+                outs[i] = mlmc_real::run(in);
+            } catch (const abort_run& a) {
+                codes[i] = a.code;
+            }
+            std::lock_guard<std::mutex> lk(B.mu);
+            B.done[i] = 1;
+            B.cv_main.notify_one();
+        });
+    }
+    int rc = NMLMC_OK;
+    while (true) {
+        std::unique_lock<std::mutex> lk(B.mu);
+        B.cv_main.wait(lk, [&] {
+            for (uint32_t i = 0; i < n; i++)
+                if (!B.done[i] && !B.waiting[i]) return false;
+            return true;
+        });
+        bool any = false;
+        for (uint32_t i = 0; i < n; i++) any = any || B.waiting[i];
+        if (!any) break;  // every run has finished
+        // Every running instance is blocked in ask(): serve them without the lock.
+        lk.unlock();
+        rc = B.serve(n);
+        lk.lock();
+        if (rc) B.error = rc;
+        for (uint32_t i = 0; i < n; i++) B.waiting[i] = 0;
+        B.cv_work.notify_all();
+        if (rc) break;
+    }
+    for (std::thread& t : th) t.join();
+    if (rc) return rc;
+    for (uint32_t i = 0; i < n; i++)
+        if (codes[i]) return codes[i];
+    const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    for (uint32_t i = 0; i < n; i++)
+        export_run(outs[i], wall, max_levels, levels_out ? levels_out + (size_t)i * max_levels : nullptr,
+                   n_levels ? n_levels + i : nullptr, summary_out ? summary_out + (size_t)i * 5 : nullptr);
     return set_error(NMLMC_OK, "");
 }
 

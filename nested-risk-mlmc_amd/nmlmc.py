@@ -86,7 +86,7 @@ EXPORTS = [
     "nmlmc_mlmc_run", "nmlmc_mlmc_run_cb", "portfolio_sample_delta_unbiased",
     "portfolio_estimate_delta_for_unbiased", "portfolio_gen_data", "portfolio_free_data",
     "nmlmc_mlmc_last_table", "nmlmc_autotune_level", "nmlmc_sample_levels",
-    "nmlmc_batch_begin", "nmlmc_batch_stage", "nmlmc_batch_sample",
+    "nmlmc_batch_begin", "nmlmc_batch_stage", "nmlmc_batch_sample", "nmlmc_batch_mlmc_run",
 ]
 
 _lib = None
@@ -139,6 +139,8 @@ def load(path=LIB_PATH):
         "nmlmc_batch_begin": (i32, [u32]),
         "nmlmc_batch_stage": (i32, [u32, C.POINTER(PortfolioData), u64, dp, u32]),
         "nmlmc_batch_sample": (i32, [u32, C.POINTER(u64), C.POINTER(SampleOutput)]),
+        "nmlmc_batch_mlmc_run": (i32, [dp, u64, dbl, dbl, dbl, u32, C.POINTER(LevelStats),
+                                       C.POINTER(u32), dp]),
         "nmlmc_mlmc_run": (i32, [dbl, u64, dbl, dbl, dbl, i32, u32, C.POINTER(LevelStats),
                                  C.POINTER(C.c_uint32), dp]),
         "nmlmc_mlmc_run_cb": (i32, [SAMPLE_LEVEL_FN, C.c_void_p, dbl, u64, dbl, dbl, dbl, i32, u32,
@@ -448,6 +450,39 @@ def batch_stage(i, pf, seed, chol=None):
         return
     a = np.ascontiguousarray(chol, np.float64)
     check(lib.nmlmc_batch_stage(i, C.byref(pf.c), seed, a.ctypes.data_as(C.POINTER(C.c_double)), a.shape[0]))
+
+
+def batch_mlmc_run(targetTOLs, M0, theta=0.5, Ca=3.0, starting_level_coeff=1.5, max_levels=64):
+    """The MLMC drivers of the staged batch run together (nmlmc_batch_mlmc_run):
+    one result dict per instance (as mlmc_run gives), wall = the whole batch's."""
+    lib = load()
+    n = len(targetTOLs)
+    levels = (LevelStats * (n * max_levels))()
+    nl = (C.c_uint32 * n)()
+    summ = (C.c_double * (5 * n))()
+    check(lib.nmlmc_batch_mlmc_run((C.c_double * n)(*targetTOLs), M0, theta, Ca, starting_level_coeff,
+                                   max_levels, levels, nl, summ))
+    res = []
+    for i in range(n):
+        lv = []
+        for l in range(min(nl[i], max_levels)):
+            L = levels[i * max_levels + l]
+            lv.append(dict(sums_fine=list(L.sums_fine), sums_diff=list(L.sums_diff), work=L.work,
+                           time=L.time, M=L.M))
+        s = summ[5 * i:5 * i + 5]
+        res.append(dict(levels=lv, starting_level=int(s[0]), bias_est=s[1], stat_error=s[2], TOL=s[3],
+                        wall=s[4]))
+    return res
+
+
+def mlmc_estimate(r):
+    """The MLMC estimate of a driver result: the starting level's fine mean plus
+    the higher levels' difference means."""
+    s = r["starting_level"]
+    e = r["levels"][s]["sums_fine"][0] / r["levels"][s]["M"]
+    for lv in r["levels"][s + 1:]:
+        e += lv["sums_diff"][0] / lv["M"]
+    return e
 
 
 def batch_sample(ell, Ms, moments=4, phi_size=1):

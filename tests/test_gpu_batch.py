@@ -100,3 +100,42 @@ def test_batch_errors(nm):
     with pytest.raises(nm.NmlmcError):
         nm.batch_sample(0, [10, 10])
     nm.cleanup_nestedmc_sampler()
+
+
+def test_batch_mlmc_drivers_match_runs_alone(nm):
+    """The MLMC drivers of a batch run together give every instance the run it
+    gets alone (same levels, sample counts and sums), each to its own
+    tolerance; the estimates are finite probabilities."""
+    books = _books(nm)
+    tols = [4e-3, 6e-3, 5e-3]
+    alone = []
+    for (d, sc, L, seed), tol in zip(books, tols):
+        nm.init_nestedmc_sampler(1, 16, 2, 0.0, True)
+        nm.set_adaptive_rule(1, 3)
+        nm.set_form(nm.FORM_MIXED)
+        pf, _, _ = to_library(nm, d, **sc)
+        nm.portfolio_init(pf, seed)
+        if L is not None:
+            nm.portfolio_set_correlation(L)
+        alone.append(nm.mlmc_run(tol, 200))
+        nm.cleanup_nestedmc_sampler()
+    nm.init_nestedmc_sampler(1, 16, 2, 0.0, True)
+    nm.set_adaptive_rule(1, 3)
+    nm.set_form(nm.FORM_MIXED)
+    nm.batch_begin(len(books))
+    keep = []
+    for i, (d, sc, L, seed) in enumerate(books):
+        pf, _, _ = to_library(nm, d, **sc)
+        keep.append(pf)
+        nm.batch_stage(i, pf, seed, L)
+    got = nm.batch_mlmc_run(tols, 200)
+    nm.cleanup_nestedmc_sampler()
+    nm.set_form(nm.FORM_PLAIN)
+    for a, g in zip(alone, got):
+        assert len(a["levels"]) == len(g["levels"]) and a["starting_level"] == g["starting_level"]
+        for la, lg in zip(a["levels"], g["levels"]):
+            assert la["M"] == lg["M"]
+            assert la["sums_fine"] == lg["sums_fine"] and la["sums_diff"] == lg["sums_diff"]
+            assert la["work"] == lg["work"]
+        e = nm.mlmc_estimate(g)
+        assert 0.0 <= e <= 1.0
