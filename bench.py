@@ -237,11 +237,63 @@ def portfolio_bench(nm, which, levels, M0, rel_eps, tte, plain_cores, tte_M0, tu
     return out
 
 
+def batch_bench(nm, n, rel_eps, M0):
+    """Batched risk: n independent books (configs.batch_books: d in {10, 50,
+    100}, P log-uniform in [1e3, 1e5], thresholds at the 95/99/99.5% pilot
+    quantiles), each driven by its own MLMC run to eps_i = rel_eps (1 - q_i),
+    all runs together over one batch of problems (one launch chain per level
+    call for all instances): the whole batch's time to tolerance in the mixed
+    and the batched FP64 forms, and whether each instance's two estimates agree
+    within 3 combined standard errors."""
+    cfg = portfolio_configs(nm)
+    t0 = time.perf_counter()
+    books = cfg.batch_books(n)
+    gen_s = time.perf_counter() - t0
+    tols = [rel_eps * (1 - q) for (_, _, _, q) in books]
+    out = {"workload": "%d independent books: d in {10,50,100}, P log-uniform in [1e3,1e5] Exact "
+                       "options (40/40/20 calls/puts/digitals, t3 weights), factor-model correlation, "
+                       "thresholds at the pilot 95/99/99.5%% quantiles; per-instance eps = %g (1 - q)"
+                       % (n, rel_eps),
+           "instances": n, "options_total": int(sum(len(b[0]["w"]) for b in books)),
+           "generate_s": round(gen_s, 2)}
+    runs = {}
+    for form, label in ((nm.FORM_MIXED, "mixed"), (nm.FORM_BATCHED, "batched_fp64")):
+        nm.init_nestedmc_sampler(1, 64, 2, 0.0, True)
+        nm.set_adaptive_rule(1, 4)
+        nm.set_form(form)
+        nm.batch_begin(n)
+        keep = []
+        for i, (a, L, sc, q) in enumerate(books):
+            pf = nm.Portfolio(a["S0"], a["mu"], a["sigma"], a["counts"], a["strike"], a["maturity"],
+                              a["w"], a["stock_ind"], a["type"], a["prob"], a["W_est"], **sc)
+            pf.comp_params()
+            keep.append(pf)
+            nm.batch_stage(i, pf, 1000 + i, L)
+        t1 = time.perf_counter()
+        r = nm.batch_mlmc_run(tols, M0)
+        dt = time.perf_counter() - t1
+        nm.cleanup_nestedmc_sampler()
+        runs[label] = r
+        out[label] = {"seconds": round(dt, 3), "max_levels": max(len(x["levels"]) for x in r),
+                      "outer_samples": int(sum(lv["M"] for x in r for lv in x["levels"]))}
+        print("# batch %s: whole-batch time to tolerance %.2f s" % (label, dt), file=sys.stderr, flush=True)
+    em = [nm.mlmc_estimate(x) for x in runs["mixed"]]
+    eb = [nm.mlmc_estimate(x) for x in runs["batched_fp64"]]
+    within = [abs(a - b) <= 3 * math.hypot(x["stat_error"], y["stat_error"])
+              for a, b, x, y in zip(em, eb, runs["mixed"], runs["batched_fp64"])]
+    out["speedup_time_to_tolerance"] = round(out["batched_fp64"]["seconds"] / out["mixed"]["seconds"], 2)
+    out["instances_within_3_sigma"] = int(sum(within))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--batch-n", type=int, default=64, help="instances of the batched-risk leg (0: skip)")
+    ap.add_argument("--batch-rel-eps", type=float, default=0.25)
+    ap.add_argument("--batch-M0", type=int, default=1024)
     ap.add_argument("--no-autotune", action="store_true",
                     help="use the default launch shapes instead of autotuning each level first")
     ap.add_argument("--M0", type=float, default=1e7)
@@ -346,6 +398,8 @@ def main():
         extra["portfolio"] = portfolio_bench(nm, args.portfolio, args.pf_levels, args.pf_M0,
                                              args.pf_rel_eps, not args.no_tte, args.plain_cores,
                                              args.pf_tte_M0, tune=not args.no_autotune)
+    if args.batch_n > 0:
+        extra["batch"] = batch_bench(nm, args.batch_n, args.batch_rel_eps, args.batch_M0)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args.cpu_M0, args.levels)

@@ -123,3 +123,47 @@ def large_portfolio(seed=4, n_pilot=1000, P=100_000, d=100, factors=5):
     sc = _scalars(1.0 / 52, 0.0)
     sc["max_loss"] = pilot_threshold(arrays, chol, sc, 0.995, n_pilot, seed + 1)
     return arrays, chol, sc
+
+
+def _batch_book(seed, k, n_pilot):
+    rs = np.random.RandomState(seed * 1000 + k)
+    d = (10, 50, 100)[rs.randint(3)]
+    P = 10 * int(round(10 ** rs.uniform(3, 5) / 10))  # the 40/40/10/10 split stays whole
+    q = (0.95, 0.99, 0.995)[rs.randint(3)]
+    factors = min(5, d)
+    B = rs.uniform(0.0, 0.5, (d, factors))
+    ss = np.sum(B * B, axis=1)
+    B *= np.sqrt(np.minimum(1.0, 0.95 / ss))[:, None]
+    C = B @ B.T
+    C[np.diag_indices(d)] = 1.0
+    chol = np.linalg.cholesky(C)
+    arrays = dict(S0=np.full(d, 100.0), mu=np.full(d, 0.05), sigma=rs.uniform(0.1, 0.4, d))
+    arrays.update(_book(rs, P, d, [(1, 0.4), (-1, 0.4), (2, 0.1), (-2, 0.1)], "t3"))
+    sc = _scalars(1.0 / 52, 0.0)
+    sc["max_loss"] = pilot_threshold(arrays, chol, sc, q, n_pilot, seed * 1000 + k + 500)
+    return arrays, chol, sc, q
+
+
+def _one_blas_thread():
+    # Each generator process runs its small matrix products on one thread:
+    # BLAS pools of every process spinning on the same cores are far slower.
+    from threadpoolctl import threadpool_limits
+    threadpool_limits(1)
+
+
+def batch_books(n=64, seed=5, n_pilot=200, workers=16):
+    """Independent problems for batched risk.  Instance k (its own random
+    stream, seed * 1000 + k) draws d in {10, 50, 100} assets (S0 = 100,
+    sigma ~ U(0.1, 0.4), drift 0.05, factor-model correlation as in the large
+    book with min(5, d) factors), P log-uniform in [10^3, 10^5] Exact options
+    (40% calls, 40% puts, 20% cash-or-nothing digitals, Student-t(3) weights),
+    tau = 1/52, and a threshold at the pilot 95%, 99% or 99.5% quantile of its
+    loss.  The instances are generated in up to `workers` processes.  Returns a
+    list of (arrays, chol, scalars, q)."""
+    import os
+    from concurrent.futures import ProcessPoolExecutor
+    w = max(1, min(workers, os.cpu_count() or 1, n))
+    if w == 1:
+        return [_batch_book(seed, k, n_pilot) for k in range(n)]
+    with ProcessPoolExecutor(w, initializer=_one_blas_thread) as ex:
+        return list(ex.map(_batch_book, [seed] * n, range(n), [n_pilot] * n))
