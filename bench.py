@@ -398,7 +398,7 @@ def main():
         extra["portfolio"] = portfolio_bench(nm, args.portfolio, args.pf_levels, args.pf_M0,
                                              args.pf_rel_eps, not args.no_tte, args.plain_cores,
                                              args.pf_tte_M0, tune=not args.no_autotune)
-    if args.batch_n > 0:
+    if args.batch_n > 0 and not args.no_tte:
         extra["batch"] = batch_bench(nm, args.batch_n, args.batch_rel_eps, args.batch_M0)
     cpu = None
     if not args.no_cpu:
