@@ -237,14 +237,16 @@ def portfolio_bench(nm, which, levels, M0, rel_eps, tte, plain_cores, tte_M0, tu
     return out
 
 
-def batch_bench(nm, n, rel_eps, M0):
+def batch_bench(nm, n, rel_eps, M0, plain_cores=0):
     """Batched risk: n independent books (configs.batch_books: d in {10, 50,
     100}, P log-uniform in [1e3, 1e5], thresholds at the 95/99/99.5% pilot
     quantiles), each driven by its own MLMC run to eps_i = rel_eps (1 - q_i),
     all runs together over one batch of problems (one launch chain per level
     call for all instances): the whole batch's time to tolerance in the mixed
     and the batched FP64 forms, and whether each instance's two estimates agree
-    within 3 combined standard errors."""
+    within 3 combined standard errors; with plain_cores, also the plain FP64
+    form run problem after problem (the reference's way: one problem per
+    process)."""
     cfg = portfolio_configs(nm)
     t0 = time.perf_counter()
     books = cfg.batch_books(n)
@@ -277,6 +279,24 @@ def batch_bench(nm, n, rel_eps, M0):
         out[label] = {"seconds": round(dt, 3), "max_levels": max(len(x["levels"]) for x in r),
                       "outer_samples": int(sum(lv["M"] for x in r for lv in x["levels"]))}
         print("# batch %s: whole-batch time to tolerance %.2f s" % (label, dt), file=sys.stderr, flush=True)
+    if plain_cores:
+        # The reference's way: one problem at a time, plain FP64 form.
+        t1 = time.perf_counter()
+        for i, (a, L, sc, q) in enumerate(books):
+            nm.init_nestedmc_sampler(plain_cores, 64, 2, 0.0, True)
+            nm.set_adaptive_rule(1, 4)
+            nm.set_form(nm.FORM_PLAIN)
+            pf = nm.Portfolio(a["S0"], a["mu"], a["sigma"], a["counts"], a["strike"], a["maturity"],
+                              a["w"], a["stock_ind"], a["type"], a["prob"], a["W_est"], **sc)
+            pf.comp_params()
+            nm.portfolio_init(pf, 1000 + i)
+            nm.portfolio_set_correlation(L)
+            nm.mlmc_run(tols[i], M0)
+            nm.cleanup_nestedmc_sampler()
+        dt = time.perf_counter() - t1
+        out["plain_fp64_one_by_one"] = {"seconds": round(dt, 3), "virtual_threads": plain_cores}
+        out["speedup_vs_plain_one_by_one"] = round(dt / out["mixed"]["seconds"], 2)
+        print("# batch plain FP64 one problem at a time: %.2f s" % dt, file=sys.stderr, flush=True)
     em = [nm.mlmc_estimate(x) for x in runs["mixed"]]
     eb = [nm.mlmc_estimate(x) for x in runs["batched_fp64"]]
     within = [abs(a - b) <= 3 * math.hypot(x["stat_error"], y["stat_error"])
@@ -399,7 +419,8 @@ def main():
                                              args.pf_rel_eps, not args.no_tte, args.plain_cores,
                                              args.pf_tte_M0, tune=not args.no_autotune)
     if args.batch_n > 0 and not args.no_tte:
-        extra["batch"] = batch_bench(nm, args.batch_n, args.batch_rel_eps, args.batch_M0)
+        extra["batch"] = batch_bench(nm, args.batch_n, args.batch_rel_eps, args.batch_M0,
+                                     0 if args.no_plain else args.plain_cores)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args.cpu_M0, args.levels)

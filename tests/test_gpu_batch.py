@@ -139,3 +139,27 @@ def test_batch_mlmc_drivers_match_runs_alone(nm):
             assert la["work"] == lg["work"]
         e = nm.mlmc_estimate(g)
         assert 0.0 <= e <= 1.0
+
+
+@pytest.mark.parametrize("form", [1, 2])
+def test_portfolio_level_sweep_and_autotune(nm, form):
+    """Portfolio problems in a level sweep (one captured graph) and after the
+    launch-shape autotuner: the same level sums as lone level calls on the same
+    outer samples (the correlated book goes through the FP64 matrix-core outer
+    draw)."""
+    d, sc, L, seed = _books(nm)[1]
+    calls = [(0, 700), (1, 300), (2, 120)]
+    ref = _alone(nm, form, (d, sc, L, seed), calls)
+    nm.init_nestedmc_sampler(1, 16, 2, 0.0, True)
+    nm.set_adaptive_rule(1, 3)
+    nm.set_form(form)
+    pf, _, _ = to_library(nm, d, **sc)
+    nm.portfolio_init(pf, seed)
+    nm.portfolio_set_correlation(L)
+    for ell, M in calls:
+        nm.autotune_level(ell, M)
+    got = nm.sample_levels([e for e, _ in calls], [M for _, M in calls])
+    nm.cleanup_nestedmc_sampler()
+    nm.set_form(nm.FORM_PLAIN)
+    for g, r in zip(got, ref):
+        assert _key(g) == _key(r)
