@@ -103,10 +103,13 @@ __device__ __forceinline__ double bs_delta_d(int type, double S0, double strike,
     return norm_cdf_d(d1) - (double)(type == -1);
 }
 
+// Types 3 / -3 (arithmetic-average Asian call / put, an extension; Unbiased
+// method only) take the path's average over (tau, T] in place of S_T.
+__device__ __forceinline__ bool is_asian(int type) { return type == 3 || type == -3; }
 __device__ __forceinline__ double payoff_d(int type, double S, double K) {
     if (type == 2) return (double)(S > K);
     if (type == -2) return (double)(S < K);
-    const double v = (double)type * (S - K);
+    const double v = (double)(is_asian(type) ? type / 3 : type) * (S - K);
     return v > 0 ? v : 0.;
 }
 
@@ -115,7 +118,7 @@ __device__ __forceinline__ double diff_from_stock_d(const OptionRec& o, double S
     return o.w * exp(-r * o.maturity) * (payoff_d(o.type, S0_T, o.strike) - payoff_d(o.type, St_T, o.strike));
 }
 __device__ __forceinline__ double derv_from_stock_d(const OptionRec& o, double St_T, double S0, double r) {
-    if (is_digital(o.type)) return 0.;
+    if (is_digital(o.type) || is_asian(o.type)) return 0.;
     const double df = (double)((o.type == 1) - (St_T < o.strike));
     return o.w * (St_T / S0) * exp(-r * o.maturity) * df;
 }
@@ -163,8 +166,12 @@ __device__ __forceinline__ double inner_exact_d(const PortfolioDevice& pf, const
 }
 
 // simulate_stock_dt (gbm_approx.hpp:18-57) for 3 fine (+ 3 coarse) paths.
-__device__ __forceinline__ void simulate_dt_d(DStream& rn, double* S, uint64_t N, double mu, double sigma,
-                                              double t, bool with_coarse, bool milstein, uint32_t coarse_step) {
+// AVG (the Asian extension): A[i] becomes the average of path i's values after
+// each of its steps (fine: N steps, coarse: N / coarse_step), in step order.
+template <bool AVG>
+__device__ __forceinline__ void simulate_dt_d(DStream& rn, double* S, double* A, uint64_t N, double mu,
+                                              double sigma, double t, bool with_coarse, bool milstein,
+                                              uint32_t coarse_step) {
     const double dt = t / (double)N;
     const double sqrt_dt = sqrt(dt);
     double dW_c = 0.;
@@ -174,6 +181,7 @@ __device__ __forceinline__ void simulate_dt_d(DStream& rn, double* S, uint64_t N
         for (int i = 0; i < 3; i++) {
             const double m = milstein ? (0.5 * (sigma * sigma) * (dW * dW - dt)) : 0.;
             S[i] += S[i] * (mu * dt + sigma * dW + m);
+            if (AVG) A[i] += S[i];
         }
         if (with_coarse) {
             dW_c += dW;
@@ -181,10 +189,16 @@ __device__ __forceinline__ void simulate_dt_d(DStream& rn, double* S, uint64_t N
                 for (int i = 0; i < 3; i++) {
                     const double m = milstein ? (0.5 * (sigma * sigma) * (dW_c * dW_c - dt_c)) : 0.;
                     S[i + 3] += S[i + 3] * (mu * dt_c + sigma * dW_c + m);
+                    if (AVG) A[i + 3] += S[i + 3];
                 }
                 dW_c = 0.;
             }
         }
+    }
+    if (AVG) {
+        for (int i = 0; i < 3; i++) A[i] /= (double)N;
+        if (with_coarse)
+            for (int i = 0; i < 3; i++) A[i + 3] /= (double)(N / coarse_step);
     }
 }
 
@@ -209,12 +223,19 @@ __device__ __forceinline__ double inner_unbiased_d(const PortfolioDevice& pf, co
     const uint64_t N = pf.dt_h0inv * N_l;
     work += (double)N_l * o.W_est;
     double S[6] = {S0_t_1, R_tau[si], S0_t_2, S0_t_1, R_tau[si], S0_t_2};
+    double A[6] = {0., 0., 0., 0., 0., 0.};
+    const bool asian = is_asian(o.type);
+    const double T_tau = o.maturity - pf.tau;
     if (L) {
-        simulate_dt_d(inner, S, N, pf.r, st.sigma, o.maturity - pf.tau, true, pf.use_milstein, pf.dt_beta);
+        if (asian) simulate_dt_d<true>(inner, S, A, N, pf.r, st.sigma, T_tau, true, pf.use_milstein, pf.dt_beta);
+        else simulate_dt_d<false>(inner, S, A, N, pf.r, st.sigma, T_tau, true, pf.use_milstein, pf.dt_beta);
         work += (double)(N_l / pf.dt_beta) * o.W_est;
     } else {
-        simulate_dt_d(inner, S, N, pf.r, st.sigma, o.maturity - pf.tau, false, pf.use_milstein, 2);
+        if (asian) simulate_dt_d<true>(inner, S, A, N, pf.r, st.sigma, T_tau, false, pf.use_milstein, 2);
+        else simulate_dt_d<false>(inner, S, A, N, pf.r, st.sigma, T_tau, false, pf.use_milstein, 2);
     }
+    // An Asian option's payoffs take the paths' averages over (tau, T].
+    const double* V = asian ? A : S;
     double val_f = 0., val_c = 0., val_d_f = 0., val_d_c = 0.;
     const double m = pf.use_antithetic_risk ? 0.5 : 1.;
     const double diff = R_tau[si] - st.S0;
@@ -222,10 +243,10 @@ __device__ __forceinline__ double inner_unbiased_d(const PortfolioDevice& pf, co
     const bool do_d_c = pf.use_delta_cv && pf.use_coarse_delta_cv;
     for (int i = 0; i < (pf.use_antithetic_risk ? 2 : 1); i++) {
         const int ic = 2 * i;
-        val_f += m * diff_from_stock_d(o, S[ic], S[1], pf.r);
+        val_f += m * diff_from_stock_d(o, V[ic], V[1], pf.r);
         if (do_d_f) val_d_f += m * derv_from_stock_d(o, S[ic], st.S0, pf.r);
         if (L == 0) continue;
-        val_c += m * diff_from_stock_d(o, S[ic + 3], S[4], pf.r);
+        val_c += m * diff_from_stock_d(o, V[ic + 3], V[4], pf.r);
         if (do_d_c) val_d_c += m * derv_from_stock_d(o, S[ic + 3], st.S0, pf.r);
     }
     const double ret = (val_f - val_c) + diff * (val_d_f - val_d_c);

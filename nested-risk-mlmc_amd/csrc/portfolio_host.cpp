@@ -102,8 +102,11 @@ int Engine::stage_portfolio(const portfolio_data* data) {
                 return set_error(NMLMC_ERR_ARG, "options[%d][%zu]: probabilities must be positive and "
                                                 "sorted in descending order", m, j);
             if (o.stock_ind >= d) return set_error(NMLMC_ERR_ARG, "options[%d][%zu]: stock index out of range", m, j);
-            if (o.type != 1 && o.type != -1 && o.type != 2 && o.type != -2)
+            if (o.type != 1 && o.type != -1 && o.type != 2 && o.type != -2 && o.type != 3 && o.type != -3)
                 return set_error(NMLMC_ERR_ARG, "options[%d][%zu]: unknown option type", m, j);
+            if ((o.type == 3 || o.type == -3) && m != 2)
+                return set_error(NMLMC_ERR_ARG, "options[%d][%zu]: Asian options need the Unbiased method (2)",
+                                 m, j);
             cdf[j] = c;
             prev = o.prob;
             c += o.prob;
@@ -373,7 +376,9 @@ void portfolio_init(const portfolio_data* data, uint64_t seed) {
 // portfolio_comp_params (portfolio_sampler.cpp:687-700).  A cash-or-nothing
 // digital priced by simulation (Exact / Unbiased) has a zero pathwise delta,
 // so its delta estimate is 0: the control variate's constant then matches the
-// zero correction its inner samples add.
+// zero correction its inner samples add.  The same holds for the Asian
+// options (types 3 / -3), which have no Black-Scholes value either (start 0:
+// the Unbiased method does not use it).
 void portfolio_comp_params(portfolio_data* data) {
     if (!data) {
         set_error(NMLMC_ERR_ARG, "portfolio data is NULL");
@@ -385,6 +390,11 @@ void portfolio_comp_params(portfolio_data* data) {
         for (size_t j = 0; j < data->options[m].size; j++) {
             option_params& o = ops[j];
             const stock_params& st = stocks[o.stock_ind];
+            if (o.type == 3 || o.type == -3) {
+                o.start = 0.;
+                o.delta_est = 0.;
+                continue;
+            }
             o.start = nmlmc::detail::bs_price_h(o.type, st.S0, o.strike, data->r, o.maturity, st.sigma);
             o.delta_est = nmlmc::detail::bs_delta_h(o.type, st.S0, o.strike, data->r, o.maturity, st.sigma);
             if (m > 0 && (o.type == 2 || o.type == -2)) o.delta_est = 0.;
@@ -426,6 +436,8 @@ void portfolio_estimate_delta_for_unbiased(portfolio_data* pdata, double TOL) {
     }
     option_params* ops = (option_params*)pdata->options[2].data;
     for (size_t j = 0; j < pdata->options[2].size; j++) {
+        // An Asian option's inner samples add no pathwise delta: its estimate stays 0.
+        if (ops[j].type == 3 || ops[j].type == -3) continue;
         uint64_t M = 1000, total_M = 0;
         double total[2] = {0., 0.};
         double sums[2], work = 0.;

@@ -345,11 +345,14 @@ static inline real_t option_derv_from_bs(const orc_option* o, real_t S0, real_t 
     return o->w * bs_delta_r(o->type, S0, o->strike, r, o->maturity, sigma);
 }
 
-/* Terminal payoff; MAX(a, 0) as in common.hpp:135-137. */
+/* Terminal payoff; MAX(a, 0) as in common.hpp:135-137.  Types 3 / -3
+ * (arithmetic-average Asian call / put, an extension) take the path's average
+ * in place of S_T. */
+static inline int is_asian(int type) { return type == 3 || type == -3; }
 static inline real_t payoff_r(int type, real_t S_T, real_t strike) {
     if (type == 2) return (real_t)(S_T > strike);
     if (type == -2) return (real_t)(S_T < strike);
-    real_t mod = (real_t)type;
+    real_t mod = (real_t)(is_asian(type) ? type / 3 : type);
     real_t v = mod * (S_T - strike);
     return v > 0 ? v : 0;
 }
@@ -363,16 +366,18 @@ static inline real_t option_diff_from_stock(const orc_option* o, real_t S0_T, re
 
 /* option_derv_from_stock (221-225): pathwise delta; zero for digitals. */
 static inline real_t option_derv_from_stock(const orc_option* o, real_t St_T, real_t S0, real_t r) {
-    if (o->type == 2 || o->type == -2) return 0;
+    if (o->type == 2 || o->type == -2 || is_asian(o->type)) return 0;
     real_t df = (real_t)((o->type == 1) - (St_T < o->strike));
     return o->w * (St_T / S0) * REXP(-r * o->maturity) * df;
 }
 
 /* simulate_stock_dt (gbm_approx.hpp:18-57) with scalar mu/sigma and a
- * real_t-valued normal source. */
+ * real_t-valued normal source.  With A (the Asian extension), A[i] becomes the
+ * average of path i's values after each of its steps (fine paths: N steps;
+ * coarse paths: N / coarse_step), summed in step order. */
 typedef real_t (*randn_fn)(void* ctx);
 
-static real_t simulate_stock_dt_r(int with_coarse, randn_fn rn, void* rctx, sum_t* S, uint64_t N,
+static real_t simulate_stock_dt_r(int with_coarse, randn_fn rn, void* rctx, sum_t* S, sum_t* A, uint64_t N,
                                   real_t mu, real_t sigma, real_t t, uint32_t size, int milstein,
                                   uint32_t coarse_step) {
     real_t W = 0;
@@ -386,6 +391,7 @@ static real_t simulate_stock_dt_r(int with_coarse, randn_fn rn, void* rctx, sum_
         for (uint32_t i = 0; i < size; i++) {
             real_t m = milstein ? (real_t)(0.5 * (sigma * sigma) * (dW * dW - dt)) : 0;
             S[i] += S[i] * (mu * dt + sigma * dW + m);
+            if (A) A[i] += S[i];
         }
         if (with_coarse) {
             dW_c += dW;
@@ -393,10 +399,16 @@ static real_t simulate_stock_dt_r(int with_coarse, randn_fn rn, void* rctx, sum_
                 for (uint32_t i = 0; i < size; i++) {
                     real_t m = milstein ? (real_t)(0.5 * (sigma * sigma) * (dW_c * dW_c - dt_c)) : 0;
                     S[i + size] += S[i + size] * (mu * dt_c + sigma * dW_c + m);
+                    if (A) A[i + size] += S[i + size];
                 }
                 dW_c = 0;
             }
         }
+    }
+    if (A) {
+        for (uint32_t i = 0; i < size; i++) A[i] /= (sum_t)N;
+        if (with_coarse)
+            for (uint32_t i = 0; i < size; i++) A[i + size] /= (sum_t)(N / coarse_step);
     }
     return W;
 }
@@ -413,8 +425,23 @@ double orc_simulate_stock_dt(int with_coarse, const double* normals, double* S, 
                              double mu, double sig, double t, uint32_t size, int milstein,
                              uint32_t coarse_step) {
     fixed_normals f = {normals};
-    return simulate_stock_dt_r(with_coarse, fixed_randn, &f, S, N, (real_t)mu, (real_t)sig,
+    return simulate_stock_dt_r(with_coarse, fixed_randn, &f, S, NULL, N, (real_t)mu, (real_t)sig,
                                (real_t)t, size, milstein, coarse_step);
+}
+
+double orc_simulate_stock_dt_avg(int with_coarse, const double* normals, double* S, double* A, uint64_t N,
+                                 double mu, double sig, double t, uint32_t size, int milstein,
+                                 uint32_t coarse_step) {
+    fixed_normals f = {normals};
+    sum_t SS[8], AA[8] = {0};
+    for (uint32_t i = 0; i < 2 * size && i < 8; i++) SS[i] = (sum_t)S[i];
+    real_t W = simulate_stock_dt_r(with_coarse, fixed_randn, &f, SS, AA, N, (real_t)mu, (real_t)sig,
+                                   (real_t)t, size, milstein, coarse_step);
+    for (uint32_t i = 0; i < 2 * size && i < 8; i++) {
+        S[i] = (double)SS[i];
+        A[i] = (double)AA[i];
+    }
+    return W;
 }
 
 /* ---------------------------------------------------------------------------
@@ -600,15 +627,19 @@ static real_t sample_inner_approx(orc_ctx* c, orc_sampler* s, const orc_option* 
     uint64_t N = pf->dt_h0inv * N_l;
     *tw += (real_t)(N_l * o->W_est);
     sum_t S[6] = {S0_t_1_i, s->R_tau[si], S0_t_2_i, S0_t_1_i, s->R_tau[si], S0_t_2_i};
+    sum_t A[6] = {0, 0, 0, 0, 0, 0};
+    sum_t* A_p = is_asian(o->type) ? A : NULL;
     sampler_randn_ctx rc = {s};
     if (L) {
-        simulate_stock_dt_r(1, sampler_inner_randn, &rc, S, N, pf->r, st->sigma,
+        simulate_stock_dt_r(1, sampler_inner_randn, &rc, S, A_p, N, pf->r, st->sigma,
                             o->maturity - pf->tau, 3, pf->use_milstein, pf->dt_beta);
         *tw += (real_t)((N_l / pf->dt_beta) * o->W_est);
     } else {
-        simulate_stock_dt_r(0, sampler_inner_randn, &rc, S, N, pf->r, st->sigma,
+        simulate_stock_dt_r(0, sampler_inner_randn, &rc, S, A_p, N, pf->r, st->sigma,
                             o->maturity - pf->tau, 3, pf->use_milstein, 2);
     }
+    /* An Asian option's payoffs take the paths' averages over (tau, T]. */
+    const sum_t* V = A_p ? A : S;
     real_t val_f = 0, val_c = 0, val_d_f = 0, val_d_c = 0;
     const int cc = 3;
     const real_t m = pf->use_antithetic_risk ? 0.5 : 1.;
@@ -617,10 +648,10 @@ static real_t sample_inner_approx(orc_ctx* c, orc_sampler* s, const orc_option* 
     const int do_d_c = pf->use_delta_cv && pf->use_coarse_delta_cv;
     for (int i = 0; i < (pf->use_antithetic_risk ? 2 : 1); i++) {
         int ic = 2 * i;
-        val_f += m * option_diff_from_stock(o, (real_t)S[ic + 0], (real_t)S[1], pf->r);
+        val_f += m * option_diff_from_stock(o, (real_t)V[ic + 0], (real_t)V[1], pf->r);
         if (do_d_f) val_d_f += m * option_derv_from_stock(o, (real_t)S[ic + 0], st->S0, pf->r);
         if (L == 0) continue;
-        val_c += m * option_diff_from_stock(o, (real_t)S[ic + cc + 0], (real_t)S[cc + 1], pf->r);
+        val_c += m * option_diff_from_stock(o, (real_t)V[ic + cc + 0], (real_t)V[cc + 1], pf->r);
         if (do_d_c) val_d_c += m * option_derv_from_stock(o, (real_t)S[ic + cc + 0], st->S0, pf->r);
     }
     return ((val_f - val_c) + diff * (val_d_f - val_d_c));

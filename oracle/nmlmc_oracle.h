@@ -88,6 +88,10 @@ double orc_simulate_stock(double S0, double mu, double sig, double t, double dW)
 double orc_simulate_stock_dt(int with_coarse, const double* normals, double* S, uint64_t N,
                              double mu, double sig, double t, uint32_t size, int milstein,
                              uint32_t coarse_step);
+/* The same with A: path averages after each step (the Asian extension). */
+double orc_simulate_stock_dt_avg(int with_coarse, const double* normals, double* S, double* A, uint64_t N,
+                                 double mu, double sig, double t, uint32_t size, int milstein,
+                                 uint32_t coarse_step);
 uint64_t orc_samples_count(orc_ctx* c, uint32_t ell, int which);  /* 0 base, 1 max */
 
 /* portfolio_sample_delta_unbiased (portfolio_sampler.cpp:758-781) with the

@@ -85,6 +85,13 @@ class _Common:
         self._ssdt.argtypes = [C.c_int, _dp, _dp, C.c_uint64, C.c_double, C.c_double,
                                C.c_double, C.c_uint32, C.c_int, C.c_uint32]
         self._ssdt.restype = C.c_double
+        # The path averages belong to the Asian extension: the restatement has
+        # them, the reference build does not.
+        self._ssdta = getattr(lib, p + "simulate_stock_dt_avg", None)
+        if self._ssdta is not None:
+            self._ssdta.argtypes = [C.c_int, _dp, _dp, _dp, C.c_uint64, C.c_double, C.c_double,
+                                    C.c_double, C.c_uint32, C.c_int, C.c_uint32]
+            self._ssdta.restype = C.c_double
 
     def threefry(self, ctr, key):
         c = (C.c_uint64 * 2)(*ctr)
@@ -121,6 +128,17 @@ class _Common:
         W = self._ssdt(int(with_coarse), _ptr(normals, _dp), _ptr(S, _dp), N, mu, sig, t,
                        size, int(milstein), coarse_step)
         return W, S
+
+    def simulate_stock_dt_avg(self, with_coarse, normals, S, N, mu, sig, t, size, milstein,
+                              coarse_step):
+        """simulate_stock_dt with the paths' averages over their steps (the Asian
+        extension): returns (W, S, A)."""
+        normals = np.ascontiguousarray(normals, np.float64)
+        S = np.array(S, np.float64).copy()
+        A = np.zeros_like(S)
+        W = self._ssdta(int(with_coarse), _ptr(normals, _dp), _ptr(S, _dp), _ptr(A, _dp), N, mu, sig,
+                        t, size, int(milstein), coarse_step)
+        return W, S, A
 
 
 def _portfolio_arrays(pf):

@@ -124,6 +124,63 @@ def test_batched_fp64_matches_oracle(nm, oracle_f64, which):
     nm.set_form(nm.FORM_PLAIN)
 
 
+def _with_asians(seed):
+    """Exact options plus Unbiased ones of which about half are arithmetic-
+    average Asian calls / puts (types 3 / -3)."""
+    d = synthetic(Q=3, counts=(0, 20, 16), seed=seed)
+    n1 = int(d["counts"][1])
+    rs = np.random.RandomState(seed + 1)
+    t = d["type"].copy()
+    for j in range(n1, len(t)):
+        if rs.rand() < 0.5:
+            t[j] = 3 if rs.rand() < 0.5 else -3
+    d["type"] = t
+    return d
+
+
+@pytest.mark.parametrize("form,cores,cv", [(0, 1, True), (0, 3, False), (1, 1, True), (2, 1, False)])
+def test_asian_options_match_oracle(nm, oracle_f64, form, cores, cv):
+    """Arithmetic-average Asian options (the Unbiased method's Milstein paths
+    with their averages over (tau, T]) beside Exact and European Unbiased
+    options: plain form against the reference keying, batched and mixed forms
+    against the batched keying; per-sample results (the mixed form redoes a
+    near-threshold sample in FP64, and the Unbiased terms are FP64 anyway)."""
+    d = _with_asians(31)
+    assert np.any(np.abs(d["type"]) == 3)
+    scal = dict(SCAL, max_loss=0.01, subsample=1, use_delta_cv=cv, use_antithetic_risk=True,
+                phi_type=1 if form < 2 else 0, phi_size=1)
+    pf, o = _setup(nm, oracle_f64, d, cores, 8, 4, 1.5 if form == 0 else 0.0, scal, 17, form=form,
+                   rule=None if form == 0 else (1, 3))
+    for ell, M in ((0, 40), (1, 30), (2, 12)):
+        if form:
+            g = nm.sample(ell, M, outer_offset=0, trace=True)
+            r = o.sample(ell, M, keying=1, outer_offset=0, trace=True)
+        else:
+            g = nm.sample(ell, M, trace=True)
+            r = o.sample(ell, M, trace=True)
+        if form == 2:
+            same = (g["nf"] == r["nf"]) & (g["nc"] == r["nc"])
+            assert same.mean() > 0.9
+            assert np.array_equal(g["phi_f"][same], r["phi_f"][same])
+        else:
+            _cmp(g, r, rtol=RTOL_UNBIASED)
+    nm.cleanup_nestedmc_sampler()
+    nm.set_form(nm.FORM_PLAIN)
+
+
+def test_asian_options_need_the_unbiased_method(nm):
+    """Asian options have no exact terminal law and no Black-Scholes value:
+    staged as Exact or BS options they are refused."""
+    d = synthetic(Q=2, counts=(0, 6, 0), seed=3)
+    d["type"] = d["type"].copy()
+    d["type"][0] = 3
+    nm.init_nestedmc_sampler(1, 8, 2, 0.0, True)
+    pf, _, _ = to_library(nm, d, **dict(SCAL, max_loss=0.0, subsample=1, phi_type=0, phi_size=1))
+    with pytest.raises(nm.NmlmcError):
+        nm.portfolio_init(pf, 1)
+    nm.cleanup_nestedmc_sampler()
+
+
 def test_mixed_portfolio_agrees_with_fp64(nm, oracle_f64):
     """Mixed FP32+FP64 form on an Exact portfolio with digitals: wherever the
     FP32 refinement decisions agree, every indicator equals the FP64 one."""

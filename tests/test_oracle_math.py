@@ -152,6 +152,65 @@ def test_simulate_stock_dt_closed_forms(O):
     assert W == pytest.approx(math.sqrt(dt) * z.sum(), rel=1e-14)
 
 
+def test_simulate_stock_dt_path_averages(O):
+    """Path averages of the Asian extension: sigma = 0 gives the geometric
+    sums (1/N) sum_k S (1 + mu dt)^k for the fine and the coarse paths; with
+    noise, a brute-force Milstein recurrence with the averages summed in step
+    order (fine: after each step; coarse: after each coarse_step-th), and the
+    terminal values unchanged by the averaging."""
+    N, mu, t = 16, 0.07, 0.5
+    W, S, A = O.simulate_stock_dt_avg(1, np.zeros(N), [100.0, 90.0, 100.0, 90.0], N, mu, 0.0, t, 2, 1, 4)
+    g = 1 + mu * t / N
+    assert A[0] == pytest.approx(100 * sum(g ** k for k in range(1, N + 1)) / N, rel=1e-14)
+    assert A[1] == pytest.approx(90 * sum(g ** k for k in range(1, N + 1)) / N, rel=1e-14)
+    gc = 1 + mu * t / (N / 4)
+    assert A[2] == pytest.approx(100 * sum(gc ** k for k in range(1, N // 4 + 1)) / (N // 4), rel=1e-14)
+    rs = np.random.RandomState(3)
+    sig, N, cs = 0.35, 12, 3
+    z = rs.randn(N)
+    W, S, A = O.simulate_stock_dt_avg(1, z, [100.0, 80.0], N, mu, sig, t, 1, 1, cs)
+    W2, S2 = O.simulate_stock_dt(1, z, [100.0, 80.0], N, mu, sig, t, 1, 1, cs)
+    assert list(S) == list(S2) and W == W2
+    dt, dtc = t / N, t / (N // cs)
+    s, sc, a, ac, dwc = 100.0, 80.0, 0.0, 0.0, 0.0
+    for n in range(N):
+        dW = math.sqrt(dt) * z[n]
+        s += s * (mu * dt + sig * dW + 0.5 * sig * sig * (dW * dW - dt))
+        a += s
+        dwc += dW
+        if (n + 1) % cs == 0:
+            sc += sc * (mu * dtc + sig * dwc + 0.5 * sig * sig * (dwc * dwc - dtc))
+            ac += sc
+            dwc = 0.0
+    assert A[0] == pytest.approx(a / N, rel=1e-14)
+    assert A[1] == pytest.approx(ac / (N // cs), rel=1e-14)
+
+
+def test_asian_inner_mean_closed_form(O):
+    """An Asian call (type 3) priced by the Unbiased method with sigma = 0:
+    both paths grow deterministically, and the randomised-level estimator's
+    mean is the continuously averaged payoff difference
+    w e^{-rT} (A(S0 e^{r tau}) - A(S0 e^{mu tau})) with A(x) = x (e^{r s} - 1)/(r s) - K,
+    s = T - tau.  The terminal-value payoff, or a dropped coarse term, misses
+    it by far more than the tolerance (r s = 0.6 separates the average from
+    the terminal value by a third)."""
+    r, mu, tau, T, K, w, S0 = 0.3, 0.1, 0.1, 2.1, 50.0, 1.0, 100.0
+    pf = dict(S0=[S0], mu=[mu], sigma=[0.0], counts=np.array([0, 0, 1], np.uint32), strike=[K],
+              maturity=[T], start=[0.0], w=[w], delta_est=[0.0], stock_ind=np.array([0], np.uint32),
+              type=np.array([3], np.int32), prob=[1.0], W_est=[1.0], r=r, rho=0.0, max_loss=0.0,
+              tau=tau, dt_h0inv=8, dt_beta=4, dt_prob_expo=-1.5, phi_type=2, phi_size=1, subsample=1,
+              use_delta_cv=False, use_coarse_delta_cv=False, use_antithetic_risk=True, use_milstein=True)
+    O.new(1, 4096, 2, 0.0, True, moments=1)
+    O.portfolio_init(pf, 5)
+    o = O.sample(0, 16)
+    s_ = T - tau
+    avg = lambda x: x * (math.exp(r * s_) - 1) / (r * s_)
+    exact = w * math.exp(-r * T) * ((avg(S0 * math.exp(r * tau)) - K) - (avg(S0 * math.exp(mu * tau)) - K))
+    assert o["fsums"][0] / 16 == pytest.approx(exact, rel=0.02)
+    terminal = w * math.exp(-r * T) * (S0 * math.exp(r * tau) - S0 * math.exp(mu * tau)) * math.exp(r * s_)
+    assert abs(terminal - exact) > 0.2 * abs(exact)
+
+
 def test_sample_counts(O):
     """get_samples_count's N_base / N_max: reference rule N0 (beta/2)^l, N0 beta^l;
     capped rule N0 beta^l and << cap."""
