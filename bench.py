@@ -237,7 +237,7 @@ def portfolio_bench(nm, which, levels, M0, rel_eps, tte, plain_cores, tte_M0, tu
     return out
 
 
-def batch_bench(nm, n, rel_eps, M0, plain_cores=0):
+def batch_bench(nm, n, rel_eps, M0, plain_cores=0, asian_frac=0.0):
     """Batched risk: n independent books (configs.batch_books: d in {10, 50,
     100}, P log-uniform in [1e3, 1e5], thresholds at the 95/99/99.5% pilot
     quantiles), each driven by its own MLMC run to eps_i = rel_eps (1 - q_i),
@@ -249,13 +249,14 @@ def batch_bench(nm, n, rel_eps, M0, plain_cores=0):
     process)."""
     cfg = portfolio_configs(nm)
     t0 = time.perf_counter()
-    books = cfg.batch_books(n)
+    books = cfg.batch_books(n, asian_frac=asian_frac)
     gen_s = time.perf_counter() - t0
     tols = [rel_eps * (1 - q) for (_, _, _, q) in books]
-    out = {"workload": "%d independent books: d in {10,50,100}, P log-uniform in [1e3,1e5] Exact "
-                       "options (40/40/20 calls/puts/digitals, t3 weights), factor-model correlation, "
-                       "thresholds at the pilot 95/99/99.5%% quantiles; per-instance eps = %g (1 - q)"
-                       % (n, rel_eps),
+    out = {"workload": "%d independent books: d in {10,50,100}, P log-uniform in [1e3,1e5] options "
+                       "(%g%% arithmetic Asians by randomised-level Milstein paths, 20%% digitals, the "
+                       "rest calls/puts; t3 weights), factor-model correlation, thresholds at the pilot "
+                       "95/99/99.5%% quantiles; per-instance eps = %g (1 - q)"
+                       % (n, 100 * asian_frac, rel_eps),
            "instances": n, "options_total": int(sum(len(b[0]["w"]) for b in books)),
            "generate_s": round(gen_s, 2)}
     runs = {}
@@ -314,6 +315,8 @@ def main():
     ap.add_argument("--batch-n", type=int, default=64, help="instances of the batched-risk leg (0: skip)")
     ap.add_argument("--batch-rel-eps", type=float, default=0.25)
     ap.add_argument("--batch-M0", type=int, default=1024)
+    ap.add_argument("--batch-asian-frac", type=float, default=0.0,
+                    help="share of Asian options in the batched-risk books")
     ap.add_argument("--no-autotune", action="store_true",
                     help="use the default launch shapes instead of autotuning each level first")
     ap.add_argument("--M0", type=float, default=1e7)
@@ -420,7 +423,7 @@ def main():
                                              args.pf_tte_M0, tune=not args.no_autotune)
     if args.batch_n > 0 and not args.no_tte:
         extra["batch"] = batch_bench(nm, args.batch_n, args.batch_rel_eps, args.batch_M0,
-                                     0 if args.no_plain else args.plain_cores)
+                                     0 if args.no_plain else args.plain_cores, args.batch_asian_frac)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args.cpu_M0, args.levels)

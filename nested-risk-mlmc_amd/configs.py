@@ -31,19 +31,27 @@ def _ncdf(x):
 
 def bs_value(typ, S, K, r, t, sigma):
     """Black-Scholes value (calc_option_blackscholes, portfolio_sampler.cpp:140-150)
-    of calls (1), puts (-1) and cash-or-nothing digitals paying 1 (2, -2)."""
+    of calls (1), puts (-1) and cash-or-nothing digitals paying 1 (2, -2).  For
+    the arithmetic-average Asians (3, -3, averaging from now to t) it is the
+    geometric-average closed form (sigma / sqrt 3, carry (r - sigma^2 / 6) / 2),
+    a proxy that only serves the pilot threshold."""
     typ = np.asarray(typ)
-    d1 = (np.log(S / K) + (r + 0.5 * sigma * sigma) * t) / (sigma * np.sqrt(t))
-    d2 = d1 - sigma * np.sqrt(t)
-    mod = np.where(np.abs(typ) == 2, typ // 2, typ).astype(np.float64)
-    vanilla = mod * S * _ncdf(mod * d1) - mod * K * np.exp(-r * t) * _ncdf(mod * d2)
+    asian = np.abs(typ) == 3
+    sig = np.where(asian, sigma / np.sqrt(3.0), sigma)
+    b = np.where(asian, 0.5 * (r - sigma * sigma / 6.0), r)  # cost of carry
+    d1 = (np.log(S / K) + (b + 0.5 * sig * sig) * t) / (sig * np.sqrt(t))
+    d2 = d1 - sig * np.sqrt(t)
+    mod = np.where(np.abs(typ) == 2, typ // 2, np.where(asian, typ // 3, typ)).astype(np.float64)
+    vanilla = mod * S * np.exp((b - r) * t) * _ncdf(mod * d1) - mod * K * np.exp(-r * t) * _ncdf(mod * d2)
     digital = np.exp(-r * t) * _ncdf(mod * d2)
     return np.where(np.abs(typ) == 2, digital, vanilla)
 
 
 def _book(rs, P, d, kinds, weights):
     """P options on d stocks: kinds = list of (type, fraction); weights 'normal'
-    or 't3'.  All options are priced by Exact-GBM inner sampling (method 1)."""
+    or 't3'.  Asian options (types 3 / -3) are priced by the Unbiased method
+    (method 2, randomised-level Milstein paths), all others by Exact-GBM inner
+    sampling (method 1); each method's options sorted by descending prob = |w|."""
     types = np.concatenate([np.full(int(round(f * P)), t, np.int32) for t, f in kinds])
     assert len(types) == P
     rs.shuffle(types)
@@ -52,10 +60,12 @@ def _book(rs, P, d, kinds, weights):
     stock_ind = rs.randint(0, d, P).astype(np.uint32)
     w = rs.standard_normal(P) if weights == "normal" else rs.standard_t(3, P)
     prob = np.abs(w)
-    order = np.argsort(-prob, kind="stable")
+    method = np.where(np.abs(types) == 3, 2, 1)
+    order = np.lexsort((-prob, method))  # method-major, then descending prob
+    n2 = int(np.sum(method == 2))
     return dict(strike=strike[order], maturity=maturity[order], w=w[order],
                 stock_ind=stock_ind[order], type=types[order], prob=prob[order],
-                W_est=np.ones(P), counts=np.array([0, P, 0], np.uint32))
+                W_est=np.ones(P), counts=np.array([0, P - n2, n2], np.uint32))
 
 
 def conditional_loss(arrays, chol, scalars, z):
@@ -125,7 +135,7 @@ def large_portfolio(seed=4, n_pilot=1000, P=100_000, d=100, factors=5):
     return arrays, chol, sc
 
 
-def _batch_book(seed, k, n_pilot):
+def _batch_book(seed, k, n_pilot, asian_frac=0.2):
     rs = np.random.RandomState(seed * 1000 + k)
     d = (10, 50, 100)[rs.randint(3)]
     P = 10 * int(round(10 ** rs.uniform(3, 5) / 10))  # the 40/40/10/10 split stays whole
@@ -138,7 +148,9 @@ def _batch_book(seed, k, n_pilot):
     C[np.diag_indices(d)] = 1.0
     chol = np.linalg.cholesky(C)
     arrays = dict(S0=np.full(d, 100.0), mu=np.full(d, 0.05), sigma=rs.uniform(0.1, 0.4, d))
-    arrays.update(_book(rs, P, d, [(1, 0.4), (-1, 0.4), (2, 0.1), (-2, 0.1)], "t3"))
+    a = asian_frac / 2  # P is a multiple of 10 and a of 0.05: the counts stay whole
+    kinds = [(1, 0.4 - a), (-1, 0.4 - a), (2, 0.1), (-2, 0.1)] + ([(3, a), (-3, a)] if a else [])
+    arrays.update(_book(rs, P, d, kinds, "t3"))
     sc = _scalars(1.0 / 52, 0.0)
     sc["max_loss"] = pilot_threshold(arrays, chol, sc, q, n_pilot, seed * 1000 + k + 500)
     return arrays, chol, sc, q
@@ -151,12 +163,15 @@ def _one_blas_thread():
     threadpool_limits(1)
 
 
-def batch_books(n=64, seed=5, n_pilot=200, workers=16):
+def batch_books(n=64, seed=5, n_pilot=200, workers=16, asian_frac=0.2):
     """Independent problems for batched risk.  Instance k (its own random
     stream, seed * 1000 + k) draws d in {10, 50, 100} assets (S0 = 100,
     sigma ~ U(0.1, 0.4), drift 0.05, factor-model correlation as in the large
-    book with min(5, d) factors), P log-uniform in [10^3, 10^5] Exact options
-    (40% calls, 40% puts, 20% cash-or-nothing digitals, Student-t(3) weights),
+    book with min(5, d) factors), P log-uniform in [10^3, 10^5] options
+    (asian_frac arithmetic-average Asian calls and puts by the Unbiased
+    method's randomised-level Milstein paths; 20% cash-or-nothing digitals and
+    the rest calls and puts half and half, by Exact-GBM inner sampling;
+    Student-t(3) weights),
     tau = 1/52, and a threshold at the pilot 95%, 99% or 99.5% quantile of its
     loss.  The instances are generated in up to `workers` processes.  Returns a
     list of (arrays, chol, scalars, q)."""
@@ -164,6 +179,6 @@ def batch_books(n=64, seed=5, n_pilot=200, workers=16):
     from concurrent.futures import ProcessPoolExecutor
     w = max(1, min(workers, os.cpu_count() or 1, n))
     if w == 1:
-        return [_batch_book(seed, k, n_pilot) for k in range(n)]
+        return [_batch_book(seed, k, n_pilot, asian_frac) for k in range(n)]
     with ProcessPoolExecutor(w, initializer=_one_blas_thread) as ex:
-        return list(ex.map(_batch_book, [seed] * n, range(n), [n_pilot] * n))
+        return list(ex.map(_batch_book, [seed] * n, range(n), [n_pilot] * n, [asian_frac] * n))

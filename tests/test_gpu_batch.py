@@ -83,6 +83,47 @@ def test_batch_matches_instances_alone(nm, form, chunk, monkeypatch):
     nm.set_form(nm.FORM_PLAIN)
 
 
+def _asian_book(nm):
+    """A general book: Exact options and Unbiased ones of which about half are
+    arithmetic-average Asian calls / puts."""
+    d = synthetic(P=36, Q=3, counts=(0, 20, 16), digitals=True, seed=31)
+    rs = np.random.RandomState(32)
+    t = d["type"].copy()
+    for j in range(20, 36):
+        if rs.rand() < 0.5:
+            t[j] = 3 if rs.rand() < 0.5 else -3
+    d["type"] = t
+    sc = dict(SCAL, max_loss=0.01, subsample=1, use_delta_cv=True, use_antithetic_risk=True,
+              phi_type=0, phi_size=1)
+    return d, sc, None, 17
+
+
+@pytest.mark.parametrize("form", [1, 2])
+def test_batch_with_asian_book_matches_alone(nm, form):
+    """A book with Unbiased options (Asians among them) between two Exact-only
+    books: every instance's level sums, work and counts equal its lone run's."""
+    b = _books(nm)
+    books = [b[0], _asian_book(nm), b[2]]
+    Ms = [[300, 60, 200], [100, 40, 50], [40, 20, 30]]
+    ells = [0, 1, 2]
+    alone = [_alone(nm, form, bk, [(e, M[i]) for e, M in zip(ells, Ms)]) for i, bk in enumerate(books)]
+    nm.init_nestedmc_sampler(1, 16, 2, 0.0, True)
+    nm.set_adaptive_rule(1, 3)
+    nm.set_form(form)
+    nm.batch_begin(len(books))
+    keep = []
+    for i, (d, sc, L, seed) in enumerate(books):
+        pf, _, _ = to_library(nm, d, **sc)
+        keep.append(pf)
+        nm.batch_stage(i, pf, seed, L)
+    for c, (ell, M) in enumerate(zip(ells, Ms)):
+        got = nm.batch_sample(ell, M)
+        for i in range(len(books)):
+            assert _key(got[i]) == _key(alone[i][c]), (c, i)
+    nm.cleanup_nestedmc_sampler()
+    nm.set_form(nm.FORM_PLAIN)
+
+
 def test_batch_errors(nm):
     """A batch needs staged instances and a batched form; at most 64 instances."""
     nm.init_nestedmc_sampler(1, 16, 2, 0.0, True)
