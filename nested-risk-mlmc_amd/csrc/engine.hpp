@@ -69,6 +69,19 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
     return v;
 }
 
+// Lane `lane`'s value for every lane (lane the same in all lanes; it need not
+// be active for the read, but its value is only meaningful when it is).
+__device__ __forceinline__ uint32_t bcast_u32(uint32_t v, uint32_t lane) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)lane);
+}
+__device__ __forceinline__ uint64_t bcast_u64(uint64_t v, uint32_t lane) {
+    const uint32_t lo = bcast_u32((uint32_t)v, lane), hi = bcast_u32((uint32_t)(v >> 32), lane);
+    return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ double bcast_f64(double v, uint32_t lane) {
+    return __longlong_as_double((long long)bcast_u64((uint64_t)__double_as_longlong(v), lane));
+}
+
 // Reduces the group-uniform accumulators of a 256-thread block into one
 // Partial record (only group leaders contribute).  Every thread of the block
 // must call it.

@@ -119,6 +119,94 @@ struct PortfolioPolicyT {
 
     static hipError_t launch_outer(const Args& A, hipStream_t s);
 
+    // The Unbiased samples of level >= 2 of the active lanes' windows (bit j of
+    // lvw: unit w0 + j), one at a time, each by all active lanes: its owner
+    // lane makes the sample's draws up to its paths (unbiased_begin); the
+    // lanes take the chunks of its paths in turn (path_chunk; at most 64 chunks
+    // of whole coarse steps, set by the step count alone, so that the value
+    // does not depend on how many lanes are active); every lane chains the
+    // chunks in order, reading them from the lanes that made them, and the
+    // owner prices the option (unbiased_end).
+    template <class ADD>
+    __device__ static void unbiased_wave(const PortfolioDevice& pf, uint64_t seed, uint64_t k1, uint32_t tag,
+                                         uint64_t w0, const StridedR& R, uint64_t lvw, double& work, SegAcc& s,
+                                         ADD& add) {
+        const uint64_t act = __ballot(1);
+        const uint32_t lane = __lane_id();
+        const uint32_t rank = (uint32_t)__popcll(act & ((1ull << lane) - 1));
+        const uint32_t nact = (uint32_t)__popcll(act);
+        for (;;) {
+            const uint64_t owners = __ballot(lvw != 0);
+            if (owners == 0) break;
+            const uint32_t ow = (uint32_t)__builtin_ctzll(owners);
+            const bool me = lane == ow;
+            DStream st;
+            st.init(seed, k1, 0, 0, 0., 0);
+            UnbiasedPath p;
+            p.N = 0;
+            uint32_t idx = 0;
+            double prob = 1., sigma = 0., dt = 0., sqrt_dt = 0., dt_c = 0.;
+            if (me) {
+                const uint32_t j = (uint32_t)__builtin_ctzll(lvw);
+                lvw &= lvw - 1;
+                st.init(seed, k1, ((uint64_t)tag << 40) | (w0 + j), 0, 0., 0);
+                uint32_t method;
+                prob = pick_option(pf, st, st, method, idx);
+                const OptionRec o = pf.options[2][idx];
+                unbiased_begin(pf, o, R, st, st, work, p);
+                // simulate_dt_d's step sizes
+                const double t = o.maturity - pf.tau;
+                sigma = pf.stocks[o.stock_ind].sigma;
+                dt = t / (double)p.N;
+                sqrt_dt = sqrt(dt);
+                dt_c = t / (double)(p.N / pf.dt_beta);
+            }
+            DStream rn;
+            rn.init(bcast_u64(st.k0, ow), bcast_u64(st.k1, ow), bcast_u64(st.c1, ow), bcast_u64(st.P, ow),
+                    bcast_f64(st.saved, ow), bcast_u32(st.is_saved, ow));
+            const uint64_t N = bcast_u64(p.N, ow);
+            const uint32_t beta = bcast_u32(pf.dt_beta, ow);
+            const bool milstein = bcast_u32(pf.use_milstein ? 1u : 0u, ow) != 0;
+            const double mu = bcast_f64(pf.r, ow), sg = bcast_f64(sigma, ow);
+            const double dt_b = bcast_f64(dt, ow), sdt_b = bcast_f64(sqrt_dt, ow), dtc_b = bcast_f64(dt_c, ow);
+            const uint64_t K = N / beta;  // coarse steps
+            const uint64_t per = (K + 63) / 64;  // coarse steps per chunk
+            const uint32_t nch = per ? (uint32_t)((K + per - 1) / per) : 0u;  // <= 64
+            double xf = 1., qf = 0., xc = 1., qc = 0.;
+            for (uint32_t b0 = 0; b0 < nch; b0 += nact) {
+                // Round of chunks [b0, b0 + nact): the lane of rank r takes b0 + r.
+                PathChunk q{1., 0., 1., 0.};
+                const uint32_t ch = b0 + rank;
+                if (ch < nch) {
+                    const uint64_t c0 = (uint64_t)ch * per, c1 = (c0 + per < K) ? c0 + per : K;
+                    q = path_chunk(rn, c0, c1, mu, sg, dt_b, sdt_b, dtc_b, beta, milstein);
+                }
+                const uint32_t cnt = (nch - b0 < nact) ? nch - b0 : nact;
+                uint64_t lanes = act;  // the lane of rank r is the r-th set bit
+                for (uint32_t r = 0; r < cnt; r++) {
+                    const uint32_t l = (uint32_t)__builtin_ctzll(lanes);
+                    lanes &= lanes - 1;
+                    qf += xf * bcast_f64(q.qf, l);
+                    xf *= bcast_f64(q.xf, l);
+                    qc += xc * bcast_f64(q.qc, l);
+                    xc *= bcast_f64(q.xc, l);
+                }
+            }
+            if (me) {
+                for (int i = 0; i < 3; i++) {
+                    const double S0 = p.S[i];
+                    p.S[i] = S0 * xf;
+                    p.A[i] = S0 * qf / (double)N;
+                    p.S[i + 3] = S0 * xc;
+                    p.A[i + 3] = S0 * qc / (double)K;
+                }
+                add(unbiased_end(pf, pf.options[2][idx], R, p) / prob);
+                s.nu += st.nu;
+                s.nn += st.nn + N;
+            }
+        }
+    }
+
     template <bool F32>
     __device__ static void units(const Args& A, uint32_t m, const ItemState& it, uint32_t tag, uint64_t u0,
                                  uint64_t u1, uint64_t, uint64_t, SegAcc& s) {
@@ -174,8 +262,8 @@ struct PortfolioPolicyT {
             s.work += work;
             return;
         }
-        for (uint64_t u = u0; u < u1; u++) {
-            if (EXACT_ONLY) {
+        if constexpr (EXACT_ONLY) {
+            for (uint64_t u = u0; u < u1; u++) {
                 // FP64 (the FP32 loop returned above).  The sample's words: 0 method
                 // uniform (only method 1 has options), 1 option uniform (block 0),
                 // 2 and 3 the Box-Muller pair (block 1).
@@ -193,44 +281,92 @@ struct PortfolioPolicyT {
                 const double v = inner_exact_d(pf, o, R, z0, z1, w64) / prob;
                 a1 += v;
                 a2 += v * v;
-                continue;
             }
-            DStream st;
-            st.init(seed, k1, ((uint64_t)tag << 40) | u, 0, 0., 0);
-            if (F32 && pf.subsample != 0) {
-                uint32_t method, idx;
-                const double prob = pick_option(pf, st, st, method, idx);
-                const OptionRec o = pf.options[method][idx];
-                if (method == 1) {
-                    const uint64_t first = st.rand(), second = st.rand();
-                    st.nn += 2;
-                    float x, y, mag;
-                    bool exact64;
-                    boxmuller_f32(second, first, x, y);
-                    work += o.W_est;
-                    const float ip = (float)(1. / prob);
-                    const float d = inner_exact_f(pf, o, (float)R[o.stock_ind], x, y, mag, exact64) * ip;
-                    s.nu += st.nu;
-                    s.nn += st.nn;
-                    redo64 |= exact64;
-                    b1.add(d);
-                    b2.add(d * d);
-                    bab.add(mag * ip);
-                    continue;
-                }
-                double v = method == 0 ? inner_bs_d(pf, o, R, work) : inner_unbiased_d(pf, o, R, st, st, work);
-                v /= prob;
+        } else if (pf.subsample != 0) {
+            // Inner samples by cost.  An Unbiased sample runs 8 * 4^L Milstein
+            // steps at its random level L (P(L >= l) = 8^-l), the others a few
+            // transcendentals, and a wave whose lanes meet samples of different
+            // cost in one iteration waits for the dearest.  So each window of up
+            // to 64 units goes first through its BS and Exact samples, noting
+            // the level of each Unbiased one; then each lane takes its Unbiased
+            // samples of level 0, then those of level 1; then the wave takes the
+            // deeper ones one at a time together (unbiased_wave), splitting each
+            // sample's paths over its lanes.  A sample's value depends on its own
+            // draws only, whichever lanes compute it; only the order of a lane's
+            // sums differs from unit order.
+            auto add64 = [&](double v) {
                 a1 += v;
                 a2 += v * v;
                 ab += fabs(v);
-            } else {
+            };
+            for (uint64_t w0 = u0; w0 < u1; w0 += 64) {
+                const uint32_t nw = (uint32_t)((u1 - w0) < 64 ? (u1 - w0) : 64);
+                uint64_t lv0 = 0, lv1 = 0, lvw = 0;
+                for (uint32_t j = 0; j < nw; j++) {
+                    DStream st;
+                    st.init(seed, k1, ((uint64_t)tag << 40) | (w0 + j), 0, 0., 0);
+                    uint32_t method, idx;
+                    const double prob = pick_option(pf, st, st, method, idx);
+                    if (method == 2) {
+                        const uint32_t L = unbiased_level(pf, st);
+                        const uint64_t bit = 1ull << j;
+                        if (L == 0) lv0 |= bit;
+                        else if (L == 1) lv1 |= bit;
+                        else lvw |= bit;
+                        continue;
+                    }
+                    const OptionRec o = pf.options[method][idx];
+                    if (F32 && method == 1) {
+                        const uint64_t first = st.rand(), second = st.rand();
+                        st.nn += 2;
+                        float x, y, mag;
+                        bool exact64;
+                        boxmuller_f32(second, first, x, y);
+                        work += o.W_est;
+                        const float ip = (float)(1. / prob);
+                        const float d = inner_exact_f(pf, o, (float)R[o.stock_ind], x, y, mag, exact64) * ip;
+                        redo64 |= exact64;
+                        b1.add(d);
+                        b2.add(d * d);
+                        bab.add(mag * ip);
+                    } else if (method == 0) {
+                        add64(inner_bs_d(pf, o, R, work) / prob);
+                    } else {
+                        const double z0 = st.randn();
+                        const double z1 = st.randn();
+                        add64(inner_exact_d(pf, o, R, z0, z1, work) / prob);
+                    }
+                    s.nu += st.nu;
+                    s.nn += st.nn;
+                }
+                // Unbiased samples of level 0, then level 1, in this lane.
+                for (int pass = 0; pass < 2; pass++) {
+                    uint64_t mask = pass == 0 ? lv0 : lv1;
+                    while (mask) {
+                        const uint32_t j = (uint32_t)__builtin_ctzll(mask);
+                        mask &= mask - 1;
+                        DStream st;
+                        st.init(seed, k1, ((uint64_t)tag << 40) | (w0 + j), 0, 0., 0);
+                        uint32_t method, idx;
+                        const double prob = pick_option(pf, st, st, method, idx);
+                        add64(inner_unbiased_d(pf, pf.options[2][idx], R, st, st, work) / prob);
+                        s.nu += st.nu;
+                        s.nn += st.nn;
+                    }
+                }
+                unbiased_wave(pf, seed, k1, tag, w0, R, lvw, work, s, add64);
+            }
+        } else {
+            for (uint64_t u = u0; u < u1; u++) {
+                DStream st;
+                st.init(seed, k1, ((uint64_t)tag << 40) | u, 0, 0., 0);
                 const double d = inner_sample_d(pf, R, st, st, work);
                 a1 += d;
                 a2 += d * d;
                 ab += fabs(d);
+                s.nu += st.nu;
+                s.nn += st.nn;
             }
-            s.nu += st.nu;
-            s.nn += st.nn;
         }
         s.s1 += a1 + (F32 ? b1.value() : 0.);
         s.s2 += a2 + (F32 ? b2.value() : 0.);

@@ -62,6 +62,18 @@ struct DStream {
         nu++;
         return u01_f64(rand());
     }
+    // Move past the next k normals of a normal-only run of draws (the cached
+    // one first, then Box-Muller pairs): the stream is then where k randn()
+    // calls leave it, its last pair computed when k ends inside one.
+    __device__ void skip_normals(uint64_t k) {
+        if (k == 0) return;
+        if (is_saved) {
+            is_saved = 0;
+            k--;
+        }
+        P += 2 * (k >> 1);
+        if (k & 1) (void)randn();
+    }
 };
 
 constexpr double kSqrt2 = 1.414213562373095;
@@ -202,40 +214,95 @@ __device__ __forceinline__ void simulate_dt_d(DStream& rn, double* S, double* A,
     }
 }
 
+// The Milstein paths of simulate_dt_d split into chunks.  All 3 fine paths
+// take the same step S <- S + S g_n (g_n = mu dt + sigma dW_n + Milstein term),
+// so a path's value after step n is its starting value times x_n, where
+// x_0 = 1, x_n = x_{n-1} + x_{n-1} g_n, and its average over its steps is its
+// starting value times (sum of x_n) / N; the coarse paths likewise with the
+// sums of beta increments.  A chunk of coarse steps [c0, c1) (fine steps
+// [c0 beta, c1 beta)) gives the product of its factors and the sum of its
+// partial products, fine and coarse, from unit values; chained in order they
+// give the whole paths.  rn stands at the paths' first normal.
+struct PathChunk {
+    double xf, qf, xc, qc;
+};
+
+__device__ __forceinline__ PathChunk path_chunk(DStream rn, uint64_t c0, uint64_t c1, double mu, double sigma,
+                                                double dt, double sqrt_dt, double dt_c, uint32_t beta,
+                                                bool milstein) {
+    rn.skip_normals(c0 * beta);
+    PathChunk q{1., 0., 1., 0.};
+    for (uint64_t c = c0; c < c1; c++) {
+        double dW_c = 0.;
+        for (uint32_t i = 0; i < beta; i++) {
+            const double dW = sqrt_dt * rn.randn();
+            const double m = milstein ? (0.5 * (sigma * sigma) * (dW * dW - dt)) : 0.;
+            q.xf += q.xf * (mu * dt + sigma * dW + m);
+            q.qf += q.xf;
+            dW_c += dW;
+        }
+        const double m = milstein ? (0.5 * (sigma * sigma) * (dW_c * dW_c - dt_c)) : 0.;
+        q.xc += q.xc * (mu * dt_c + sigma * dW_c + m);
+        q.qc += q.xc;
+    }
+    return q;
+}
+
 __device__ __forceinline__ uint64_t upow(uint64_t x, uint64_t n) {
     uint64_t r = 1;
     while (n--) r *= x;
     return r;
 }
 
-// sample_inner_unbiased (521-537) + sample_inner_approx (539-604).
-template <class RT>
-__device__ __forceinline__ double inner_unbiased_d(const PortfolioDevice& pf, const OptionRec& o, RT R_tau,
-                                                   DStream& sync, DStream& inner, double& work) {
+// The randomised level of an Unbiased inner sample (sample_inner_unbiased,
+// 521-537): L = floor(ln u / (ln beta * e)) from one sync uniform, so that
+// P(L >= l) = beta^(e l).
+__device__ __forceinline__ uint32_t unbiased_level(const PortfolioDevice& pf, DStream& sync) {
     const double e = pf.dt_prob_expo, x = (double)pf.dt_beta;
-    const uint32_t L = (uint32_t)floor(log(sync.randu()) / (log(x) * e));
+    return (uint32_t)floor(log(sync.randu()) / (log(x) * e));
+}
+
+// One Unbiased inner sample in three parts: unbiased_begin draws its level and
+// initial increment and sets the paths' values at tau (3 fine paths: S0 and its
+// antithetic twin to tau, and the outer sample's level; the same 3 coarse);
+// the paths then run N fine (N / beta coarse) Milstein steps, in one lane
+// (simulate_dt_d) or split over a wave; unbiased_end prices the options on them.
+struct UnbiasedPath {
+    uint32_t L;
+    uint64_t N;  // fine steps
+    double S[6], A[6];
+};
+
+// sample_inner_approx (539-604) up to its call of simulate_stock_dt.
+template <class RT>
+__device__ __forceinline__ void unbiased_begin(const PortfolioDevice& pf, const OptionRec& o, RT R_tau, DStream& sync,
+                                               DStream& inner, double& work, UnbiasedPath& p) {
+    p.L = unbiased_level(pf, sync);
     const uint32_t si = o.stock_ind;
     const StockRec st = pf.stocks[si];
     const double dW_0_tau = pf.stau * inner.randn();
     const double S0_t_1 = simulate_stock_d(st.S0, pf.r, st.sigma, pf.tau, dW_0_tau);
     const double S0_t_2 = simulate_stock_d(st.S0, pf.r, st.sigma, pf.tau, -dW_0_tau);
-    const uint64_t N_l = upow(pf.dt_beta, L);
-    const uint64_t N = pf.dt_h0inv * N_l;
+    const uint64_t N_l = upow(pf.dt_beta, p.L);
+    p.N = pf.dt_h0inv * N_l;
     work += (double)N_l * o.W_est;
-    double S[6] = {S0_t_1, R_tau[si], S0_t_2, S0_t_1, R_tau[si], S0_t_2};
-    double A[6] = {0., 0., 0., 0., 0., 0.};
-    const bool asian = is_asian(o.type);
-    const double T_tau = o.maturity - pf.tau;
-    if (L) {
-        if (asian) simulate_dt_d<true>(inner, S, A, N, pf.r, st.sigma, T_tau, true, pf.use_milstein, pf.dt_beta);
-        else simulate_dt_d<false>(inner, S, A, N, pf.r, st.sigma, T_tau, true, pf.use_milstein, pf.dt_beta);
-        work += (double)(N_l / pf.dt_beta) * o.W_est;
-    } else {
-        if (asian) simulate_dt_d<true>(inner, S, A, N, pf.r, st.sigma, T_tau, false, pf.use_milstein, 2);
-        else simulate_dt_d<false>(inner, S, A, N, pf.r, st.sigma, T_tau, false, pf.use_milstein, 2);
-    }
+    if (p.L) work += (double)(N_l / pf.dt_beta) * o.W_est;
+    const double Rt = R_tau[si];
+    p.S[0] = S0_t_1, p.S[1] = Rt, p.S[2] = S0_t_2, p.S[3] = S0_t_1, p.S[4] = Rt, p.S[5] = S0_t_2;
+    for (int i = 0; i < 6; i++) p.A[i] = 0.;
+}
+
+// The rest of sample_inner_approx after the paths and sample_inner_unbiased's
+// weight beta^(-e L) / (1 - beta^e).
+template <class RT>
+__device__ __forceinline__ double unbiased_end(const PortfolioDevice& pf, const OptionRec& o, RT R_tau,
+                                               const UnbiasedPath& p) {
+    const double e = pf.dt_prob_expo, x = (double)pf.dt_beta;
+    const uint32_t L = p.L;
+    const uint32_t si = o.stock_ind;
+    const StockRec st = pf.stocks[si];
     // An Asian option's payoffs take the paths' averages over (tau, T].
-    const double* V = asian ? A : S;
+    const double* V = is_asian(o.type) ? p.A : p.S;
     double val_f = 0., val_c = 0., val_d_f = 0., val_d_c = 0.;
     const double m = pf.use_antithetic_risk ? 0.5 : 1.;
     const double diff = R_tau[si] - st.S0;
@@ -244,14 +311,34 @@ __device__ __forceinline__ double inner_unbiased_d(const PortfolioDevice& pf, co
     for (int i = 0; i < (pf.use_antithetic_risk ? 2 : 1); i++) {
         const int ic = 2 * i;
         val_f += m * diff_from_stock_d(o, V[ic], V[1], pf.r);
-        if (do_d_f) val_d_f += m * derv_from_stock_d(o, S[ic], st.S0, pf.r);
+        if (do_d_f) val_d_f += m * derv_from_stock_d(o, p.S[ic], st.S0, pf.r);
         if (L == 0) continue;
         val_c += m * diff_from_stock_d(o, V[ic + 3], V[4], pf.r);
-        if (do_d_c) val_d_c += m * derv_from_stock_d(o, S[ic + 3], st.S0, pf.r);
+        if (do_d_c) val_d_c += m * derv_from_stock_d(o, p.S[ic + 3], st.S0, pf.r);
     }
     const double ret = (val_f - val_c) + diff * (val_d_f - val_d_c);
     const double L_prob_1 = pow(x, -e * L) / (1. - pow(x, e));
     return ret * L_prob_1;
+}
+
+// sample_inner_unbiased (521-537) + sample_inner_approx (539-604), the paths in
+// this lane.
+template <class RT>
+__device__ __forceinline__ double inner_unbiased_d(const PortfolioDevice& pf, const OptionRec& o, RT R_tau,
+                                                   DStream& sync, DStream& inner, double& work) {
+    UnbiasedPath p;
+    unbiased_begin(pf, o, R_tau, sync, inner, work, p);
+    const StockRec st = pf.stocks[o.stock_ind];
+    const double T_tau = o.maturity - pf.tau;
+    const bool asian = is_asian(o.type);
+    if (p.L) {
+        if (asian) simulate_dt_d<true>(inner, p.S, p.A, p.N, pf.r, st.sigma, T_tau, true, pf.use_milstein, pf.dt_beta);
+        else simulate_dt_d<false>(inner, p.S, p.A, p.N, pf.r, st.sigma, T_tau, true, pf.use_milstein, pf.dt_beta);
+    } else {
+        if (asian) simulate_dt_d<true>(inner, p.S, p.A, p.N, pf.r, st.sigma, T_tau, false, pf.use_milstein, 2);
+        else simulate_dt_d<false>(inner, p.S, p.A, p.N, pf.r, st.sigma, T_tau, false, pf.use_milstein, 2);
+    }
+    return unbiased_end(pf, o, R_tau, p);
 }
 
 // portfolio::sampler::add_phi (portfolio_sampler.cpp:261-291).

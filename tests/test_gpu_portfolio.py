@@ -168,6 +168,38 @@ def test_asian_options_match_oracle(nm, oracle_f64, form, cores, cv):
     nm.set_form(nm.FORM_PLAIN)
 
 
+@pytest.mark.parametrize("form", [1, 2])
+def test_deep_unbiased_levels_match_oracle(nm, oracle_f64, form):
+    """Unbiased samples with level >= 2 are computed by a whole wave, each path
+    split into chunks of coarse steps whose factors are chained in order.  With
+    the level law P(L >= l) = 4^(-1.1 l) about one Unbiased sample in twenty is
+    that deep (paths of 128 to 10^5 steps).  The chained product rounds
+    differently from the oracle's one-lane recurrence, and a deep sample's
+    weight 4^(1.1 L) / p(option) magnifies that rounding: per-sample results and
+    level sums agree to 1e-7 relative (the mixed form's indicators exactly)."""
+    d = synthetic(Q=3, counts=(0, 10, 30), seed=41)
+    rs = np.random.RandomState(42)
+    t = d["type"].copy()
+    for j in range(10, 40):
+        if rs.rand() < 0.5:
+            t[j] = 3 if rs.rand() < 0.5 else -3
+    d["type"] = t
+    scal = dict(SCAL, dt_prob_expo=-1.1, max_loss=0.01, subsample=1, use_delta_cv=True,
+                use_antithetic_risk=True, phi_type=1 if form == 1 else 0, phi_size=1)
+    pf, o = _setup(nm, oracle_f64, d, 1, 8, 4, 0.0, scal, 23, form=form, rule=(1, 3))
+    for ell, M in ((0, 60), (1, 40), (2, 16)):
+        g = nm.sample(ell, M, outer_offset=0, trace=True)
+        r = o.sample(ell, M, keying=1, outer_offset=0, trace=True)
+        if form == 2:
+            same = (g["nf"] == r["nf"]) & (g["nc"] == r["nc"])
+            assert same.mean() > 0.9
+            assert np.array_equal(g["phi_f"][same], r["phi_f"][same])
+        else:
+            _cmp(g, r, rtol=1e-7)
+    nm.cleanup_nestedmc_sampler()
+    nm.set_form(nm.FORM_PLAIN)
+
+
 def test_asian_options_need_the_unbiased_method(nm):
     """Asian options have no exact terminal law and no Black-Scholes value:
     staged as Exact or BS options they are refused."""
