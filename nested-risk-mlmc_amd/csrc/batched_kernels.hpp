@@ -11,11 +11,22 @@
 
 namespace nmlmc {
 
-// Waves per SIMD the mixed-form test and final kernels are compiled for (their
-// inner loops wait on gathers, so occupancy matters more than a few spills).
+// Waves per SIMD the test and final kernels are compiled for.  Mixed form:
+// NMLMC_MIXED_WAVES (the inner loops wait on gathers, so occupancy matters
+// more than a few spills); FP64 forms: no cap.  A policy with P::kMixedWaves
+// false (the general portfolio loop, whose FP64 paths would spill heavily
+// under the mixed cap) takes NMLMC_GENERAL_WAVES in both.
 #ifndef NMLMC_MIXED_WAVES
 #define NMLMC_MIXED_WAVES 4
 #endif
+#ifndef NMLMC_GENERAL_WAVES
+#define NMLMC_GENERAL_WAVES 1
+#endif
+template <class P, bool MIXED>
+constexpr int mixed_waves() {
+    if (!P::kMixedWaves) return NMLMC_GENERAL_WAVES;
+    return MIXED ? NMLMC_MIXED_WAVES : 1;
+}
 
 __device__ __forceinline__ uint32_t class_of(uint64_t n, uint64_t base_min) {
     uint32_t c = 0;
@@ -100,7 +111,7 @@ __global__ __launch_bounds__(256) void k_b_init(typename P::Args A) {
 
 // Test round r of every outer sample still refining its fine or coarse count.
 template <class P, bool MIXED, int RULE>
-__global__ __launch_bounds__(256, MIXED ? NMLMC_MIXED_WAVES : 1) void k_b_test(typename P::Args A, uint32_t r) {
+__global__ __launch_bounds__(256, (mixed_waves<P, MIXED>())) void k_b_test(typename P::Args A, uint32_t r) {
     const BatchedCommon& c = A.c;
     const uint32_t nfe = aload_u32(&c.ctl->cnt_f[r]), nce = aload_u32(&c.ctl->cnt_c[r]);
     const uint64_t cur_f = c.cf.base << r, cur_c = c.cc.base << r;
@@ -532,7 +543,7 @@ __device__ __forceinline__ void list_reeval(const BatchedCommon& c, bool reeval,
 // FP32 bound of the Heaviside threshold (or met a payoff discontinuity) is not
 // finished here but listed for the FP64 redo.
 template <class P, bool MIXED>
-__global__ __launch_bounds__(256, MIXED ? NMLMC_MIXED_WAVES : 1) void k_b_final(typename P::Args A) {
+__global__ __launch_bounds__(256, (mixed_waves<P, MIXED>())) void k_b_final(typename P::Args A) {
     const BatchedCommon& c = A.c;
     BlockAcc<P::kMulti> acc;
     acc.template init<true>(c);
@@ -668,7 +679,7 @@ __global__ __launch_bounds__(256, MIXED ? NMLMC_MIXED_WAVES : 1) void k_b_final(
 // (MIXED) are written out.  Draws, decisions and counts are those of the
 // unfused sequence.
 template <class P, bool MIXED, int RULE>
-__global__ __launch_bounds__(256, MIXED ? NMLMC_MIXED_WAVES : 1) void k_b_fused(typename P::Args A) {
+__global__ __launch_bounds__(256, (mixed_waves<P, MIXED>())) void k_b_fused(typename P::Args A) {
     const BatchedCommon& c = A.c;
     BlockAcc<P::kMulti> acc;
     acc.template init<true>(c);

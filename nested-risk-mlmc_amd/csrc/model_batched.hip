@@ -21,6 +21,7 @@ struct ModelPolicy {
     static constexpr uint64_t spu = (VAR == MODEL_SHIFT) ? 2 : 1;
     static constexpr bool kMulti = false;
     static constexpr bool kFuse = true;  // k_b_fused is instantiated
+    static constexpr bool kMixedWaves = true;
 
     // init_inner_sampler does no work for the model (model_sampler.cpp:64).
     __device__ static double group_work(const Args&, const ItemState&) { return 0.; }

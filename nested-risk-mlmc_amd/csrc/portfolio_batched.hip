@@ -35,6 +35,9 @@ struct PortfolioPolicyT {
     // inner loop (BS, Exact, Unbiased) is too large to share a kernel with the
     // outer draw and the final at a useful occupancy.
     static constexpr bool kFuse = EXACT_ONLY;
+    // The general inner loop (FP64 Unbiased paths beside the FP32 Exact terms)
+    // runs without the mixed form's register cap.
+    static constexpr bool kMixedWaves = EXACT_ONLY;
 
     __device__ static const PortfolioDevice& pf_of(const Args& A, const ItemState& it) {
         return MULTI ? A.bt.inst[it.inst] : A.pf;
@@ -60,7 +63,10 @@ struct PortfolioPolicyT {
     // most 2^-18 of its magnitude (inner_exact_mix; for the other methods' FP32
     // terms, inner_exact_f, a term at a payoff discontinuity makes its segment's
     // magnitude infinite), hence the mean by at most 2^-18 of the mean
-    // magnitude.  start is FP64.
+    // magnitude.  The BS and Unbiased terms are FP64 in both forms; only the
+    // order of their sums differs, which moves a sum of n terms by at most
+    // n 2^-53 of their magnitudes: they enter the magnitude at 2^-16 (n up to
+    // 2^19).  start is FP64.
     __device__ static double mixed_bound(double mean_mag, double) { return 0x1p-18 * mean_mag; }
 
     __device__ static uint64_t key1(const Args& A, uint64_t m) {
@@ -320,15 +326,26 @@ struct PortfolioPolicyT {
                         const uint64_t first = st.rand(), second = st.rand();
                         st.nn += 2;
                         float x, y, mag;
-                        bool exact64;
                         boxmuller_f32(second, first, x, y);
                         work += o.W_est;
-                        const float ip = (float)(1. / prob);
-                        const float d = inner_exact_f(pf, o, (float)R[o.stock_ind], x, y, mag, exact64) * ip;
-                        redo64 |= exact64;
+                        float d;
+                        if (pf.subsample == 1) {
+                            // The option's FP32 record (its 1/p folded in): a jump
+                            // at the strike enters mag instead of asking for FP64.
+                            const MixRec q = pf.mix[idx];
+                            const uint32_t si = q.stype >> 8;
+                            d = inner_exact_mix(q, pf.mix_stocks[si], pf.use_antithetic_risk, pf.use_delta_cv,
+                                                (float)R[si], x, y, mag);
+                        } else {
+                            bool exact64;
+                            const float ip = (float)(1. / prob);
+                            d = inner_exact_f(pf, o, (float)R[o.stock_ind], x, y, mag, exact64) * ip;
+                            mag *= ip;
+                            redo64 |= exact64;
+                        }
                         b1.add(d);
                         b2.add(d * d);
-                        bab.add(mag * ip);
+                        bab.add(mag);
                     } else if (method == 0) {
                         add64(inner_bs_d(pf, o, R, work) / prob);
                     } else {
@@ -370,7 +387,7 @@ struct PortfolioPolicyT {
         }
         s.s1 += a1 + (F32 ? b1.value() : 0.);
         s.s2 += a2 + (F32 ? b2.value() : 0.);
-        s.sab += F32 ? (redo64 ? (double)INFINITY : ab + bab.value()) : 0.;
+        s.sab += F32 ? (redo64 ? (double)INFINITY : bab.value() + 0x1p-16 * ab) : 0.;
         s.work += work;
     }
 };
