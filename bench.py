@@ -315,8 +315,9 @@ def main():
     ap.add_argument("--batch-n", type=int, default=64, help="instances of the batched-risk leg (0: skip)")
     ap.add_argument("--batch-rel-eps", type=float, default=0.25)
     ap.add_argument("--batch-M0", type=int, default=1024)
-    ap.add_argument("--batch-asian-frac", type=float, default=0.0,
-                    help="share of Asian options in the batched-risk books")
+    ap.add_argument("--batch-asian-frac", type=float, default=0.2,
+                    help="share of Asian options in the books of the second batched-risk leg (0: skip it)")
+    ap.add_argument("--batch-asian-rel-eps", type=float, default=1.0)
     ap.add_argument("--no-autotune", action="store_true",
                     help="use the default launch shapes instead of autotuning each level first")
     ap.add_argument("--M0", type=float, default=1e7)
@@ -423,7 +424,10 @@ def main():
                                              args.pf_tte_M0, tune=not args.no_autotune)
     if args.batch_n > 0 and not args.no_tte:
         extra["batch"] = batch_bench(nm, args.batch_n, args.batch_rel_eps, args.batch_M0,
-                                     0 if args.no_plain else args.plain_cores, args.batch_asian_frac)
+                                     0 if args.no_plain else args.plain_cores)
+        if args.batch_asian_frac > 0:
+            extra["batch_asian"] = batch_bench(nm, args.batch_n, args.batch_asian_rel_eps, args.batch_M0, 0,
+                                               args.batch_asian_frac)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args.cpu_M0, args.levels)
