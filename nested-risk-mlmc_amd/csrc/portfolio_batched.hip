@@ -214,7 +214,7 @@ struct PortfolioPolicyT {
     }
 
     template <bool F32>
-    __device__ static void units(const Args& A, uint32_t m, const ItemState& it, uint32_t tag, uint64_t u0,
+    __device__ __forceinline__ static void units(const Args& A, uint32_t m, const ItemState& it, uint32_t tag, uint64_t u0,
                                  uint64_t u1, uint64_t, uint64_t, SegAcc& s) {
         const PortfolioDevice& pf = pf_of(A, it);
         const uint64_t seed = seed_of(A, it);
