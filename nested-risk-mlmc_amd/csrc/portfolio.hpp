@@ -44,6 +44,8 @@ struct MixStock {
 };
 
 constexpr uint32_t kGuideLo = 0x7fffffffu, kGuideMulti = 0x80000000u;
+// Unbiased levels whose weight beta^(-e L) / (1 - beta^e) is tabulated.
+constexpr uint32_t kUnbLevels = 32;
 
 struct PortfolioDevice {
     const OptionRec* options[3];
@@ -68,6 +70,11 @@ struct PortfolioDevice {
     double r, rho, s1_rho2, max_loss, tau, stau;
     uint32_t dt_h0inv, dt_beta;
     double dt_prob_expo;
+    // The Unbiased level law's constants, evaluated on the host as the
+    // reference evaluates them (sample_inner_unbiased, 521-537):
+    // ln(beta) e, and the weights beta^(-e L) / (1 - beta^e) of L < kUnbLevels.
+    double unb_lnbe;
+    double unb_weight[kUnbLevels];
     int phi_type;
     uint32_t phi_size;
     int subsample;

@@ -258,8 +258,7 @@ __device__ __forceinline__ uint64_t upow(uint64_t x, uint64_t n) {
 // 521-537): L = floor(ln u / (ln beta * e)) from one sync uniform, so that
 // P(L >= l) = beta^(e l).
 __device__ __forceinline__ uint32_t unbiased_level(const PortfolioDevice& pf, DStream& sync) {
-    const double e = pf.dt_prob_expo, x = (double)pf.dt_beta;
-    return (uint32_t)floor(log(sync.randu()) / (log(x) * e));
+    return (uint32_t)floor(log(sync.randu()) / pf.unb_lnbe);
 }
 
 // One Unbiased inner sample in three parts: unbiased_begin draws its level and
@@ -317,7 +316,7 @@ __device__ __forceinline__ double unbiased_end(const PortfolioDevice& pf, const 
         if (do_d_c) val_d_c += m * derv_from_stock_d(o, p.S[ic + 3], st.S0, pf.r);
     }
     const double ret = (val_f - val_c) + diff * (val_d_f - val_d_c);
-    const double L_prob_1 = pow(x, -e * L) / (1. - pow(x, e));
+    const double L_prob_1 = L < kUnbLevels ? pf.unb_weight[L] : pow(x, -e * L) / (1. - pow(x, e));
     return ret * L_prob_1;
 }
 

@@ -168,6 +168,10 @@ int Engine::stage_portfolio(const portfolio_data* data) {
     P.dt_h0inv = data->dt_h0inv;
     P.dt_beta = data->dt_beta;
     P.dt_prob_expo = data->dt_prob_expo;
+    P.unb_lnbe = std::log((double)data->dt_beta) * data->dt_prob_expo;
+    for (uint32_t L = 0; L < kUnbLevels; L++)
+        P.unb_weight[L] = std::pow((double)data->dt_beta, -data->dt_prob_expo * L) /
+                          (1. - std::pow((double)data->dt_beta, data->dt_prob_expo));
     P.phi_type = data->phi_type;
     P.phi_size = ps;
     P.subsample = data->subsample;
