@@ -33,8 +33,11 @@ constexpr int kMaxRounds = 30;
 constexpr int kMaxClasses = 32;
 
 // Relative bound (times the mean magnitude of the inner terms) on the FP32
-// inner mean's error; 16x above the worst case of 2^-22-accurate normals and
-// compensated sums.
+// inner mean's error.  FP32 normals lie within about 2^-20 of the FP64 ones and
+// the compensated sums add 2^-23, except for the pairs (about 2^-25 of them)
+// whose radius word rounds to 1 in FP32: those normals can be off by up to
+// 2^-12 (measured 2.4e-4 over 2^28 such pairs), which moves a mean over N >= 32
+// inner samples by at most 2^-17, below this bound.
 constexpr double kMixedBound = 0x1p-16;
 // FP64 redo of the mixed form: the first kRedoSlots listed outer samples with at
 // most kRedoGroups final groups are split over kRedoParts blocks each.
