@@ -20,7 +20,7 @@ namespace nmlmc {
 #define NMLMC_MIXED_WAVES 4
 #endif
 #ifndef NMLMC_GENERAL_WAVES
-#define NMLMC_GENERAL_WAVES 1
+#define NMLMC_GENERAL_WAVES 2
 #endif
 template <class P, bool MIXED>
 constexpr int mixed_waves() {
