@@ -322,7 +322,11 @@ struct PortfolioPolicyT {
                         continue;
                     }
                     const OptionRec o = pf.options[method][idx];
-                    if (F32 && method == 1) {
+                    // Mixed form: the Exact terms in FP32, except in books with
+                    // Unbiased options, whose FP64 paths dominate the work: there
+                    // FP32 Exact terms save little and their error bound sends more
+                    // outer samples to the FP64 redo than it saves.
+                    if (F32 && method == 1 && pf.nopt[2] == 0) {
                         const uint64_t first = st.rand(), second = st.rand();
                         st.nn += 2;
                         float x, y, mag;
