@@ -1128,6 +1128,29 @@ void orc_sample(orc_ctx* c, int keying, uint32_t ell, uint64_t M, uint64_t outer
     free(acc);
 }
 
+/* The outer draws of a level call under the batched keying, for tests of the
+ * outer draw: R_out[m * Q + i] = R_tau[i] of outer sample offset + m (the
+ * state sample_outer, portfolio_sampler.cpp:293-311, leaves behind), and the
+ * constant (init_inner_sampler's offset before any option term) in
+ * const_out[m] when const_out is not NULL. */
+void orc_outer_states(orc_ctx* c, uint32_t ell, uint64_t M, uint64_t outer_offset, double* R_out,
+                      double* const_out) {
+    orc_ensure_workers(c);
+    orc_sampler* s = &c->workers[0];
+    const uint32_t Q = c->pf.Q;
+    for (uint64_t m = 0; m < M; m++) {
+        orc_rs outer;
+        rs_seed(&outer, c->seed, ((uint64_t)ell << 46) | (outer_offset + m));
+        s->sync = &outer;
+        s->inner = &outer;
+        sampler_outer(c, s);
+        for (uint32_t i = 0; i < Q; i++) R_out[m * Q + i] = (double)s->R_tau[i];
+        if (const_out) const_out[m] = (double)s->constant;
+        s->sync = &s->own_sync;
+        s->inner = &s->own_inner;
+    }
+}
+
 /* kernel_sample_delta + portfolio_sample_delta_unbiased (portfolio_sampler.cpp:
  * 702-746, 758-781): T virtual threads, each its libgomp static share of M
  * delta samples of Unbiased option `option_index` from its persistent inner

@@ -69,6 +69,12 @@ void orc_sample(orc_ctx* c, int keying, uint32_t ell, uint64_t M, uint64_t outer
                 double* fsums, double* diffsums, double* work, uint64_t* stats,
                 double* tr_phi_f, double* tr_phi_c, uint64_t* tr_nf, uint64_t* tr_nc);
 
+/* Portfolio outer draws of outer samples [offset, offset + M) at level ell
+ * under the batched keying: R_out[m * Q + i] = R_tau[i]; const_out[m] (may be
+ * NULL) = the sampler's constant. */
+void orc_outer_states(orc_ctx* c, uint32_t ell, uint64_t M, uint64_t outer_offset, double* R_out,
+                      double* const_out);
+
 /* Building blocks exposed for known-answer tests. */
 void orc_threefry2x64(const uint64_t* ctr, const uint64_t* key, uint64_t* out);
 double orc_u01_f64(uint64_t u);

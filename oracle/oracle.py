@@ -178,6 +178,7 @@ class Oracle(_Common):
         lib.orc_sample_delta_unbiased.argtypes = [C.c_void_p, C.c_uint32, C.c_uint64, C.c_uint32,
                                                   _dp, _dp]
         lib.orc_samples_count.restype = C.c_uint64
+        lib.orc_outer_states.argtypes = [C.c_void_p, C.c_uint32, C.c_uint64, C.c_uint64, _dp, _dp]
         lib.orc_real_bytes.restype = C.c_int
         self.real = real
         self.ctx = None
@@ -241,6 +242,15 @@ class Oracle(_Common):
         self.lib.orc_sample_delta_unbiased(self.ctx, option_index, C.c_uint64(M), moments,
                                            _ptr(sums, _dp), _ptr(work, _dp))
         return sums, work[0]
+
+    def outer_states(self, ell, M, outer_offset=0):
+        """Portfolio outer draws under the batched keying: (R_tau [M x Q], the
+        sampler's constant [M])."""
+        Q = len(self._pf_keep["S0"])
+        R = np.zeros((M, Q))
+        const = np.zeros(M)
+        self.lib.orc_outer_states(self.ctx, ell, M, outer_offset, _ptr(R, _dp), _ptr(const, _dp))
+        return R, const
 
     def samples_count(self, ell):
         return (self.lib.orc_samples_count(self.ctx, ell, 0),
