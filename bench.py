@@ -61,10 +61,10 @@ def sweep(nm, M0, levels, rec=None):
         o = nm.sample(ell, M)
         dt = time.perf_counter() - t0
         if rec is not None:
-            reev, flips = nm.mixed_stats()
+            reev, flips, rck = nm.mixed_stats()
             rec.append(dict(ell=ell, M=M, wall=dt, kernel_ms=nm.last_kernel_ms(),
                             inner=int(o["stats"][0]), G=nm.last_group_lanes(), reev=reev,
-                            flips=flips, fmean=o["fsums"][0] / M,
+                            flips=flips, rechecks=rck, fmean=o["fsums"][0] / M,
                             dmean=(o["diffsums"][0] / M) if ell else 0.))
 
 

@@ -205,24 +205,33 @@ int nmlmc_sample_levels(uint32_t n, const uint32_t* ells, const uint64_t* Ms, sa
  * batch); nmlmc_batch_stage(i, data, seed, chol, d) builds instance i's tables
  * as portfolio_update_data does, with its own seed and optional Cholesky
  * factor (this replaces the engine's own staged problem, which must be staged
- * again before it is sampled).  nmlmc_batch_sample(ell, M, outs) is one level
- * call over all instances in one launch chain per chunk (batched and mixed
- * forms): M[i] outer samples of instance i, continuing that instance's level
- * ids, results into outs[i] - equal to those of instance i staged alone with
- * portfolio_init(data, seed) and sampled with the same calls.  The instances
- * share the sampler parameters and need one phi type (Indicator or Max). */
+ * again before it is sampled).  nmlmc_batch_sample(ell, n, M, outs) is one
+ * level call over all instances in one launch chain per chunk (batched and
+ * mixed forms): M[i] outer samples of instance i, continuing that instance's
+ * level ids, results into outs[i] - equal to those of instance i staged alone
+ * with portfolio_init(data, seed) and sampled with the same calls.  n is the
+ * length of M and outs and must equal the batch's instance count
+ * (NMLMC_ERR_ARG otherwise).  The instances share the sampler parameters and
+ * need one phi type (Indicator or Max). */
 int nmlmc_batch_begin(uint32_t n);
 int nmlmc_batch_stage(uint32_t i, const portfolio_data* data, uint64_t seed, const double* chol, uint32_t d);
-int nmlmc_batch_sample(uint32_t ell, const uint64_t* M, sample_output* outs);
+int nmlmc_batch_sample(uint32_t ell, uint32_t n, const uint64_t* M, sample_output* outs);
 
 /* Mixed form diagnostics of the last call: outer samples re-evaluated in FP64,
- * and indicator flips (FP32 decision differing from the FP64 one). */
-int nmlmc_mixed_stats(uint64_t* reevaluated, uint64_t* flips);
+ * indicator flips (FP32 decision differing from the FP64 one), and refinement
+ * decisions rechecked in FP64 (their FP32 test statistic lay within its error
+ * bound of the threshold).  Any pointer may be NULL. */
+int nmlmc_mixed_stats(uint64_t* reevaluated, uint64_t* flips, uint64_t* rechecked);
 
 /* Device time (ms) of the last sample call's kernels, from HIP events on the
  * library's stream, and the name of its dominant kernel. */
 double nmlmc_last_kernel_ms(void);
 const char* nmlmc_last_kernel_name(void);
+
+/* First 16 hex digits of the sha256 of the library's sources (csrc/ and this
+ * header, concatenated in path order) as built; loaders compare it with the
+ * sources beside them to refuse a stale build.  No counterpart in the reference. */
+const char* nmlmc_source_digest(void);
 
 /* Lanes per outer sample chosen for the last call (autotuned or overridden). */
 uint32_t nmlmc_last_group_lanes(void);
@@ -277,8 +286,10 @@ int nmlmc_mlmc_run_cb(nmlmc_sample_level_fn fn, void* user, double targetTOL, ui
  * level requests of all running instances are sampled by one batch level call
  * per level.  Per instance: its levels at levels_out[i * max_levels ...], the
  * level count at n_levels[i], the summary at summary_out[i * 5 ...] as
- * nmlmc_mlmc_run gives them, with the whole batch's wall time. */
-int nmlmc_batch_mlmc_run(const double* targetTOL, uint64_t M0, double theta, double Ca,
+ * nmlmc_mlmc_run gives them, with the whole batch's wall time.  n is the
+ * length of targetTOL, n_levels and summary_out (levels_out: n * max_levels)
+ * and must equal the batch's instance count (NMLMC_ERR_ARG otherwise). */
+int nmlmc_batch_mlmc_run(uint32_t n, const double* targetTOL, uint64_t M0, double theta, double Ca,
                          double starting_level_coeff, uint32_t max_levels, nmlmc_level_stats* levels_out,
                          uint32_t* n_levels, double* summary_out);
 

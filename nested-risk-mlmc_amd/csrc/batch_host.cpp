@@ -250,7 +250,7 @@ int Engine::batch_sample(uint32_t ell, const uint64_t* Ms, sample_output* outs) 
     }
     if (nc && (e = hipMemcpy(h_inst_out, d_inst_out, sizeof(Partial) * nc * n, hipMemcpyDeviceToHost)) != hipSuccess)
         return hip_fail(e, "hipMemcpy batch accumulators");
-    uint64_t reev = 0, flips = 0;
+    uint64_t reev = 0, flips = 0, rck = 0;
     for (uint32_t i = 0; i < n; i++) {
         Partial t;
         std::memset(&t, 0, sizeof(t));
@@ -268,12 +268,14 @@ int Engine::batch_sample(uint32_t ell, const uint64_t* Ms, sample_output* outs) 
         copy_out(ell, &outs[i]);
         reev += t.stats[6];
         flips += t.stats[7];
+        rck += t.stats[8];
         batch[i].next_outer[ell] += Ms[i];
     }
     last_ms = ms_total;
     last_kernel = "k_b_final";
     mixed_reeval = reev;
     mixed_flips = flips;
+    mixed_rechecks = rck;
     return set_error(NMLMC_OK, "");
 }
 
@@ -290,7 +292,10 @@ int nmlmc_batch_stage(uint32_t i, const portfolio_data* data, uint64_t seed, con
     return g_engine.batch_stage(i, data, seed, chol, d);
 }
 
-int nmlmc_batch_sample(uint32_t ell, const uint64_t* M, sample_output* outs) {
+int nmlmc_batch_sample(uint32_t ell, uint32_t n, const uint64_t* M, sample_output* outs) {
+    if (n != g_engine.batch.size())
+        return set_error(NMLMC_ERR_ARG, "batch_sample: %u arrays entries for a batch of %zu instances", n,
+                         g_engine.batch.size());
     return g_engine.batch_sample(ell, M, outs);
 }
 

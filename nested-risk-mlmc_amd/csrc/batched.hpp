@@ -33,11 +33,9 @@ constexpr int kMaxRounds = 30;
 constexpr int kMaxClasses = 32;
 
 // Relative bound (times the mean magnitude of the inner terms) on the FP32
-// inner mean's error.  FP32 normals lie within about 2^-20 of the FP64 ones and
-// the compensated sums add 2^-23, except for the pairs (about 2^-25 of them)
-// whose radius word rounds to 1 in FP32: those normals can be off by up to
-// 2^-12 (measured 2.4e-4 over 2^28 such pairs), which moves a mean over N >= 32
-// inner samples by at most 2^-17, below this bound.
+// inner mean's error.  FP32 normals lie within about 2^-20 of the FP64 ones
+// (boxmuller_f32 takes the radius of the words whose float rounds to 1 from
+// the complement word) and the compensated sums add 2^-23.
 constexpr double kMixedBound = 0x1p-16;
 // FP64 redo of the mixed form: the first kRedoSlots listed outer samples with at
 // most kRedoGroups final groups are split over kRedoParts blocks each.
@@ -59,7 +57,7 @@ struct ItemState {
 };
 
 struct TestEntry {
-    double s1, s2, sab;
+    double s1, s2, sab, smm;  // sab, smm: mixed form's error scales (SegAcc)
     uint32_t item, done;
 };
 
@@ -77,12 +75,14 @@ struct BatchedCtl {
     uint32_t nreeval;
     uint32_t overflow;
     uint32_t n_pend;  // fused first pass: outer samples left for the final phase
+    uint32_t rc_f[kMaxRounds + 1], rc_c[kMaxRounds + 1];  // mixed: decisions of round r left to the FP64 recheck
     Partial acc;  // level sums, work and stats (atomics)
 };
 
 // Per-lane accumulation of one segment.
 struct SegAcc {
     double s1 = 0., s2 = 0., sab = 0.;
+    double smm = 0.;  // mixed: sum of the FP32 terms' squared error scales (refinement decisions)
     double work = 0.;
     uint64_t nu = 0, nn = 0;  // uniforms and normals drawn (inner_stats[4], [5])
 };
@@ -269,6 +269,7 @@ struct BatchedCommon {
     ItemState* items;
     TestEntry* tf[2];
     TestEntry* tc[2];
+    TestEntry* rq[2];    // mixed: fine / coarse test groups whose decision is rechecked in FP64
     FinalEntry* fpool;
     uint32_t fcap;
     uint32_t* reeval;

@@ -103,17 +103,56 @@ __global__ __launch_bounds__(256) void k_b_init(typename P::Args A) {
         }
         it.nf = nf;
         it.nc = nc;
-        if (acc_f) c.tf[0][m] = TestEntry{0., 0., 0., (uint32_t)m, 0};
-        if (acc_c) c.tc[0][m] = TestEntry{0., 0., 0., (uint32_t)m, 0};
+        if (acc_f) c.tf[0][m] = TestEntry{0., 0., 0., 0., (uint32_t)m, 0};
+        if (acc_c) c.tc[0][m] = TestEntry{0., 0., 0., 0., (uint32_t)m, 0};
     }
     acc.template flush<false>(c);
 }
 
+// Mixed form: true when the refinement decision taken from FP32-accumulated
+// sums may differ from the one the FP64 sums of the same draws give (the
+// batched FP64 form's).  With |s1 - s1_64| <= e1 and |s2 - s2_64| <= e2
+// (P::decision_bounds, plus 2^-21 s2 for the compensated sums' own rounding),
+// |dist - dist_64| <= e1 / n and |var - var_64| <= ev; both are doubled to
+// leave room for the FP64 side's rounding.  delta^2_64 then lies in [qlo, qhi]
+// and the FP64 count new_64 (decreasing in delta^2) in [nlo, nhi]; the
+// decision new_64 <= cur is certain unless that interval straddles cur.  The
+// uncertain ones go to the FP64 recheck (k_b_test with RECHECK).
+template <class P, int RULE>
+__device__ __forceinline__ bool decision_uncertain(const BatchedCommon& c, const LevelCounts& lc, double start,
+                                                   uint64_t cur, double s1, double s2, double sab, double smm) {
+    const double n = (double)cur;
+    double e1, e2;
+    P::decision_bounds(n, start, sab, smm, e1, e2);
+    e2 += 0x1p-21 * fabs(s2);
+    const double ad = fabs(start + s1 / n);
+    const double var = s2 / (n - 1.) - (s1 * s1) / (n * (n - 1.));
+    const double ed = 2. * e1 / n;
+    const double ev = 2. * (e2 + (2. * fabs(s1) + e1) * e1 / n) / (n - 1.);
+    if (!(ad > ed) || !(var > ev)) return true;  // also NaN and infinite bounds
+    const double qlo = (ad - ed) * (ad - ed) / (var + ev), qhi = (ad + ed) * (ad + ed) / (var - ev);
+    double nlo, nhi;
+    if (RULE == 1) {
+        nlo = lc.tmp / qhi;
+        nhi = lc.tmp / qlo;
+    } else {
+        nlo = pow(qhi, -c.ep.adaptive_r / 2) * lc.tmp;
+        nhi = pow(qlo, -c.ep.adaptive_r / 2) * lc.tmp;
+    }
+    return nlo <= n && n < nhi;
+}
+
 // Test round r of every outer sample still refining its fine or coarse count.
-template <class P, bool MIXED, int RULE>
+// MIXED: inner sums in FP32; a decision within the FP32 error of its threshold
+// is not taken here but listed for the recheck.  RECHECK (after round r, mixed
+// form only): the listed groups' sums again in FP64 from the same draws, and
+// their decisions; their work and draw counts were taken in the FP32 pass.
+template <class P, bool MIXED, int RULE, bool RECHECK = false>
 __global__ __launch_bounds__(256, (mixed_waves<P, MIXED>())) void k_b_test(typename P::Args A, uint32_t r) {
+    static_assert(!(MIXED && RECHECK), "the recheck runs in FP64");
     const BatchedCommon& c = A.c;
-    const uint32_t nfe = aload_u32(&c.ctl->cnt_f[r]), nce = aload_u32(&c.ctl->cnt_c[r]);
+    const uint32_t nfe = aload_u32(RECHECK ? &c.ctl->rc_f[r] : &c.ctl->cnt_f[r]);
+    const uint32_t nce = aload_u32(RECHECK ? &c.ctl->rc_c[r] : &c.ctl->cnt_c[r]);
     const uint64_t cur_f = c.cf.base << r, cur_c = c.cc.base << r;
     const uint64_t uf = (cur_f + P::spu - 1) / P::spu, uc = (cur_c + P::spu - 1) / P::spu;
     const uint64_t spe_f = (uf + c.S - 1) / c.S, spe_c = (uc + c.S - 1) / c.S;
@@ -121,8 +160,8 @@ __global__ __launch_bounds__(256, (mixed_waves<P, MIXED>())) void k_b_test(typen
     if (total == 0) return;  // nothing refined into this round: nothing to flush
     BlockAcc<P::kMulti> acc;
     acc.template init<false>(c);
-    TestEntry* const lf = c.tf[r & 1];
-    TestEntry* const lc_ = c.tc[r & 1];
+    TestEntry* const lf = RECHECK ? c.rq[0] : c.tf[r & 1];
+    TestEntry* const lc_ = RECHECK ? c.rq[1] : c.tc[r & 1];
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < total; k += stride) {
         const bool fine = k < segs_f;
@@ -130,49 +169,69 @@ __global__ __launch_bounds__(256, (mixed_waves<P, MIXED>())) void k_b_test(typen
         const uint64_t spe = fine ? spe_f : spe_c;
         const uint64_t e = kk / spe, j = kk % spe;
         TestEntry& te = fine ? lf[e] : lc_[e];
-        const uint32_t m = r == 0 ? (uint32_t)e : te.item;
+        const uint32_t m = (r == 0 && !RECHECK) ? (uint32_t)e : te.item;
         const ItemState& it = c.items[m];
         const uint64_t cur = fine ? cur_f : cur_c, units = fine ? uf : uc;
         const uint64_t u0 = j * c.S, u1 = (u0 + c.S < units) ? u0 + c.S : units;
         SegAcc s;
         P::template units<MIXED>(A, m, it, (fine ? TAG_FINE_TEST : TAG_COARSE_TEST) + r, u0, u1, 0, cur, s);
         const uint32_t q = it.inst;
-        acc.work(q, s.work);
-        acc.st(q, 4, s.nu);
-        acc.st(q, 5, s.nn);
+        if (!RECHECK) {
+            acc.work(q, s.work);
+            acc.st(q, 4, s.nu);
+            acc.st(q, 5, s.nn);
+        }
         bool complete = spe == 1;
-        double s1 = s.s1, s2 = s.s2;
+        double s1 = s.s1, s2 = s.s2, sab = s.sab, smm = s.smm;
         if (__ballot(spe > 1)) {  // wave-uniform
             const RunSums rs = run_sums(fine ? e : (e | (1ull << 63)), s.s1, s.s2);
+            double rab = 0., rmm = 0.;
+            if (MIXED) {
+                const RunSums rm = run_sums(fine ? e : (e | (1ull << 63)), s.sab, s.smm);
+                rab = rm.a;
+                rmm = rm.b;
+            }
             if (!complete && rs.tail) {
                 aadd(&te.s1, rs.a);
                 aadd(&te.s2, rs.b);
+                if (MIXED) {
+                    aadd(&te.sab, rab);
+                    aadd(&te.smm, rmm);
+                }
                 wait_acks();
                 if (acount(&te.done, rs.n) == spe - rs.n) {
                     complete = true;
                     s1 = aread(&te.s1);
                     s2 = aread(&te.s2);
+                    if (MIXED) {
+                        sab = aread(&te.sab);
+                        smm = aread(&te.smm);
+                    }
                 }
             }
         }
-        bool append = false;
+        bool append = false, recheck = false;
         if (complete) {
-            acc.work(q, P::group_work(A, it));
+            if (!RECHECK) acc.work(q, P::group_work(A, it));
             const LevelCounts& lc = fine ? c.cf : c.cc;
-            bool more;
-            const uint64_t n = next_count<RULE>(c, lc, it.start, cur, s1, s2, more);
-            if (more) {
-                append = true;
-                acc.st(q, 0, n);
-            } else {
-                ItemState& w = c.items[m];
-                if (fine) {
-                    w.nf = n;
-                    acc.st(q, 1, 1);
-                    acc.st(q, 2, n == lc.max);
-                    acc.st(q, 3, n == lc.base);
+            if (MIXED) recheck = decision_uncertain<P, RULE>(c, lc, it.start, cur, s1, s2, sab, smm);
+            if (RECHECK) acc.st(q, 8, 1);
+            if (!recheck) {
+                bool more;
+                const uint64_t n = next_count<RULE>(c, lc, it.start, cur, s1, s2, more);
+                if (more) {
+                    append = true;
+                    acc.st(q, 0, n);
                 } else {
-                    w.nc = n;
+                    ItemState& w = c.items[m];
+                    if (fine) {
+                        w.nf = n;
+                        acc.st(q, 1, 1);
+                        acc.st(q, 2, n == lc.max);
+                        acc.st(q, 3, n == lc.base);
+                    } else {
+                        w.nc = n;
+                    }
                 }
             }
         }
@@ -180,8 +239,13 @@ __global__ __launch_bounds__(256, (mixed_waves<P, MIXED>())) void k_b_test(typen
         const uint32_t sf = wave_append(&c.ctl->cnt_f[r + 1], append && fine);
         const uint32_t sc = wave_append(&c.ctl->cnt_c[r + 1], append && !fine);
         if (append) {
-            if (fine) c.tf[(r + 1) & 1][sf] = TestEntry{0., 0., 0., m, 0};
-            else c.tc[(r + 1) & 1][sc] = TestEntry{0., 0., 0., m, 0};
+            if (fine) c.tf[(r + 1) & 1][sf] = TestEntry{0., 0., 0., 0., m, 0};
+            else c.tc[(r + 1) & 1][sc] = TestEntry{0., 0., 0., 0., m, 0};
+        }
+        if (MIXED) {
+            const uint32_t rf = wave_append(&c.ctl->rc_f[r], recheck && fine);
+            const uint32_t rc = wave_append(&c.ctl->rc_c[r], recheck && !fine);
+            if (recheck) c.rq[fine ? 0 : 1][fine ? rf : rc] = TestEntry{0., 0., 0., 0., m, 0};
         }
     }
     acc.template flush<false>(c);
@@ -691,7 +755,7 @@ __global__ __launch_bounds__(256, (mixed_waves<P, MIXED>())) void k_b_fused(type
         P::init_item(A, m, it, acc);
         const uint32_t q = it.inst;
         it.nf = it.nc = 0;
-        bool app_f = false, app_c = false;
+        bool app_f = false, app_c = false, rck_f = false, rck_c = false;
         if (!test_f) {
             it.nf = c.cf.max;
             acc.st(q, 1, 1);
@@ -703,14 +767,17 @@ __global__ __launch_bounds__(256, (mixed_waves<P, MIXED>())) void k_b_fused(type
             acc.work(q, t.work + P::group_work(A, it));
             acc.st(q, 4, t.nu);
             acc.st(q, 5, t.nn);
-            const uint64_t n = next_count<RULE>(c, c.cf, it.start, c.cf.base, t.s1, t.s2, app_f);
-            if (app_f) {
-                acc.st(q, 0, n);
-            } else {
-                it.nf = n;
-                acc.st(q, 1, 1);
-                acc.st(q, 2, (n == c.cf.max));
-                acc.st(q, 3, (n == c.cf.base));
+            if (MIXED) rck_f = decision_uncertain<P, RULE>(c, c.cf, it.start, c.cf.base, t.s1, t.s2, t.sab, t.smm);
+            if (!rck_f) {
+                const uint64_t n = next_count<RULE>(c, c.cf, it.start, c.cf.base, t.s1, t.s2, app_f);
+                if (app_f) {
+                    acc.st(q, 0, n);
+                } else {
+                    it.nf = n;
+                    acc.st(q, 1, 1);
+                    acc.st(q, 2, (n == c.cf.max));
+                    acc.st(q, 3, (n == c.cf.base));
+                }
             }
         }
         if (c.ell > 0) {
@@ -722,12 +789,17 @@ __global__ __launch_bounds__(256, (mixed_waves<P, MIXED>())) void k_b_fused(type
                 acc.work(q, t.work + P::group_work(A, it));
                 acc.st(q, 4, t.nu);
                 acc.st(q, 5, t.nn);
-                const uint64_t n = next_count<RULE>(c, c.cc, it.start, c.cc.base, t.s1, t.s2, app_c);
-                if (app_c) acc.st(q, 0, n);
-                else it.nc = n;
+                if (MIXED) rck_c = decision_uncertain<P, RULE>(c, c.cc, it.start, c.cc.base, t.s1, t.s2, t.sab, t.smm);
+                if (!rck_c) {
+                    const uint64_t n = next_count<RULE>(c, c.cc, it.start, c.cc.base, t.s1, t.s2, app_c);
+                    if (app_c) acc.st(q, 0, n);
+                    else it.nc = n;
+                }
             }
         }
-        bool pending = app_f || app_c, reeval = false;
+        // A decision left to the FP64 recheck (k_b_test<..., RECHECK> of round
+        // 0) sends the outer sample to the final phase like a refined one.
+        bool pending = app_f || app_c || rck_f || rck_c, reeval = false;
         if (!pending) {
             const FinalShape sh = final_shape(c, it);
             if (final_units<P>(c, it, sh) <= c.S) {
@@ -760,10 +832,16 @@ __global__ __launch_bounds__(256, (mixed_waves<P, MIXED>())) void k_b_fused(type
         const uint32_t sf = wave_append(&c.ctl->cnt_f[1], app_f);
         const uint32_t sc = wave_append(&c.ctl->cnt_c[1], app_c);
         const uint32_t sp = wave_append(&c.ctl->n_pend, pending);
-        if (app_f) c.tf[1][sf] = TestEntry{0., 0., 0., (uint32_t)m, 0};
-        if (app_c) c.tc[1][sc] = TestEntry{0., 0., 0., (uint32_t)m, 0};
+        if (app_f) c.tf[1][sf] = TestEntry{0., 0., 0., 0., (uint32_t)m, 0};
+        if (app_c) c.tc[1][sc] = TestEntry{0., 0., 0., 0., (uint32_t)m, 0};
         if (pending) c.pend[sp] = (uint32_t)m;
-        if (MIXED) list_reeval(c, reeval, (uint32_t)m);
+        if (MIXED) {
+            list_reeval(c, reeval, (uint32_t)m);
+            const uint32_t rf = wave_append(&c.ctl->rc_f[0], rck_f);
+            const uint32_t rc = wave_append(&c.ctl->rc_c[0], rck_c);
+            if (rck_f) c.rq[0][rf] = TestEntry{0., 0., 0., 0., (uint32_t)m, 0};
+            if (rck_c) c.rq[1][rc] = TestEntry{0., 0., 0., 0., (uint32_t)m, 0};
+        }
     }
     acc.template flush<true>(c);
 }
@@ -789,20 +867,30 @@ hipError_t launch_batched_level(const typename P::Args& A0, uint32_t blocks, hip
     hipLaunchKernelGGL(k_b_zero, dim3(1), dim3(256), 0, s, A.c.ctl, A.c.inst_out, A.c.n_inst);
     hipError_t e = P::launch_outer(A, s);
     if (e != hipSuccess) return e;
+    // Mixed form: the FP64 recheck of round r's uncertain decisions, before
+    // round r + 1 reads its queue.
+    auto recheck = [&](uint32_t r) {
+        if (!MIXED) return;
+        if (rule1) hipLaunchKernelGGL((k_b_test<P, false, 1, true>), dim3(blocks), dim3(256), 0, s, A, r);
+        else hipLaunchKernelGGL((k_b_test<P, false, 0, true>), dim3(blocks), dim3(256), 0, s, A, r);
+    };
     uint32_t r0 = 0;
     if constexpr (P::kFuse) {
         if (A.c.fused) {
             if (rule1) hipLaunchKernelGGL((k_b_fused<P, MIXED, 1>), dim3(blocks), dim3(256), 0, s, A);
             else hipLaunchKernelGGL((k_b_fused<P, MIXED, 0>), dim3(blocks), dim3(256), 0, s, A);
+            recheck(0);
             r0 = 1;
         }
     }
     if (!A.c.fused) {
         hipLaunchKernelGGL((k_b_init<P>), dim3(blocks), dim3(256), 0, s, A);
     }
-    for (uint32_t r = r0; r < A.c.rounds; r++)
+    for (uint32_t r = r0; r < A.c.rounds; r++) {
         if (rule1) hipLaunchKernelGGL((k_b_test<P, MIXED, 1>), dim3(blocks), dim3(256), 0, s, A, r);
         else hipLaunchKernelGGL((k_b_test<P, MIXED, 0>), dim3(blocks), dim3(256), 0, s, A, r);
+        recheck(r);
+    }
     hipLaunchKernelGGL((k_b_final_count<P>), dim3(blocks), dim3(256), 0, s, A);
     hipLaunchKernelGGL(k_b_final_alloc, dim3(1), dim3(64), 0, s, A.c.ctl, A.c.fcap);
     hipLaunchKernelGGL((k_b_final_fill<P>), dim3(blocks), dim3(256), 0, s, A);

@@ -111,8 +111,8 @@ void Engine::free_samplers() {
 // Buffers of the batched forms (kept between level calls, released here).
 void Engine::free_batched() {
     for (void* p : {(void*)bat_items, (void*)bat_tf[0], (void*)bat_tf[1], (void*)bat_tc[0], (void*)bat_tc[1],
-                    (void*)bat_reeval, (void*)bat_pend, (void*)bat_fpool, (void*)bat_redo_acc, (void*)bat_redo_cnt,
-                    (void*)bat_ctl, (void*)pf_R})
+                    (void*)bat_rq[0], (void*)bat_rq[1], (void*)bat_reeval, (void*)bat_pend, (void*)bat_fpool,
+                    (void*)bat_redo_acc, (void*)bat_redo_cnt, (void*)bat_ctl, (void*)pf_R})
         if (p) hipFree(p);
     if (bat_ctl_h) hipHostFree(bat_ctl_h);
     if (sweep_slots) hipHostFree(sweep_slots);
@@ -122,6 +122,7 @@ void Engine::free_batched() {
     sweep_exec = nullptr;
     bat_items = nullptr;
     bat_tf[0] = bat_tf[1] = bat_tc[0] = bat_tc[1] = nullptr;
+    bat_rq[0] = bat_rq[1] = nullptr;
     bat_reeval = nullptr;
     bat_pend = nullptr;
     bat_fpool = nullptr;
@@ -318,10 +319,11 @@ int Engine::prepare_batched(uint64_t M, BatchedCommon& c) {
         uint64_t cap = bat_cap ? bat_cap : 1024;
         while (cap < M) cap *= 2;
         for (void* p : {(void*)bat_items, (void*)bat_tf[0], (void*)bat_tf[1], (void*)bat_tc[0],
-                        (void*)bat_tc[1], (void*)bat_reeval, (void*)bat_pend})
+                        (void*)bat_tc[1], (void*)bat_rq[0], (void*)bat_rq[1], (void*)bat_reeval, (void*)bat_pend})
             if (p) hipFree(p);
         bat_items = nullptr;
         bat_tf[0] = bat_tf[1] = bat_tc[0] = bat_tc[1] = nullptr;
+        bat_rq[0] = bat_rq[1] = nullptr;
         bat_reeval = nullptr;
         bat_pend = nullptr;
         bat_cap = 0;
@@ -329,6 +331,7 @@ int Engine::prepare_batched(uint64_t M, BatchedCommon& c) {
         for (int i = 0; i < 2; i++) {
             if ((e = hipMalloc(&bat_tf[i], sizeof(TestEntry) * cap)) != hipSuccess) return hip_fail(e, "hipMalloc queue");
             if ((e = hipMalloc(&bat_tc[i], sizeof(TestEntry) * cap)) != hipSuccess) return hip_fail(e, "hipMalloc queue");
+            if ((e = hipMalloc(&bat_rq[i], sizeof(TestEntry) * cap)) != hipSuccess) return hip_fail(e, "hipMalloc queue");
         }
         if ((e = hipMalloc(&bat_reeval, sizeof(uint32_t) * cap)) != hipSuccess) return hip_fail(e, "hipMalloc reeval");
         if ((e = hipMalloc(&bat_pend, sizeof(uint32_t) * cap)) != hipSuccess) return hip_fail(e, "hipMalloc pending");
@@ -357,6 +360,8 @@ int Engine::prepare_batched(uint64_t M, BatchedCommon& c) {
     c.tf[1] = bat_tf[1];
     c.tc[0] = bat_tc[0];
     c.tc[1] = bat_tc[1];
+    c.rq[0] = bat_rq[0];
+    c.rq[1] = bat_rq[1];
     c.fpool = bat_fpool;
     c.fcap = (uint32_t)std::min<uint64_t>(bat_fcap, 0xffffffffull);
     c.reeval = bat_reeval;
@@ -494,6 +499,7 @@ int Engine::run_batched(uint32_t ell, uint64_t M, BatchedCommon& c, sample_outpu
     copy_out(ell, out);
     mixed_reeval = h_out->stats[6];
     mixed_flips = h_out->stats[7];
+    mixed_rechecks = h_out->stats[8];
     rc = trace_finish(M, tb, tr_phi_f, tr_phi_c, tr_nf, tr_nc);
     if (rc) return rc;
     return set_error(NMLMC_OK, "");
@@ -628,7 +634,7 @@ int Engine::sample_levels(uint32_t n, const uint32_t* ells, const uint64_t* Ms, 
         t.work += b.acc.work;
         for (int j = 0; j < kStatsExt; j++) t.stats[j] += b.acc.stats[j];
     }
-    uint64_t reev = 0, flips = 0;
+    uint64_t reev = 0, flips = 0, rck = 0;
     for (uint32_t i = 0; i < n; i++) {
         outs[i].M = Ms[i];
         if (redo[i]) {
@@ -637,11 +643,13 @@ int Engine::sample_levels(uint32_t n, const uint32_t* ells, const uint64_t* Ms, 
             ms_total += last_ms;
             reev += mixed_reeval;
             flips += mixed_flips;
+            rck += mixed_rechecks;
         } else {
             std::memcpy(h_out, &tot[i], sizeof(Partial));
             copy_out(ells[i], &outs[i]);
             reev += tot[i].stats[6];
             flips += tot[i].stats[7];
+            rck += tot[i].stats[8];
         }
         if (ells[i] >= next_outer.size()) next_outer.resize(ells[i] + 1, 0);
         next_outer[ells[i]] = off[i] + Ms[i];
@@ -649,6 +657,7 @@ int Engine::sample_levels(uint32_t n, const uint32_t* ells, const uint64_t* Ms, 
     last_ms = (float)ms_total;
     mixed_reeval = reev;
     mixed_flips = flips;
+    mixed_rechecks = rck;
     return set_error(NMLMC_OK, "");
 }
 
@@ -906,14 +915,20 @@ int nmlmc_sample(uint32_t ell, uint64_t M, uint64_t outer_offset, sample_output*
     return g_engine.sample(ell, M, outer_offset, 0, out, tr_phi_f, tr_phi_c, tr_nf, tr_nc);
 }
 
-int nmlmc_mixed_stats(uint64_t* reevaluated, uint64_t* flips) {
+int nmlmc_mixed_stats(uint64_t* reevaluated, uint64_t* flips, uint64_t* rechecked) {
     if (reevaluated) *reevaluated = g_engine.mixed_reeval;
     if (flips) *flips = g_engine.mixed_flips;
+    if (rechecked) *rechecked = g_engine.mixed_rechecks;
     return set_error(NMLMC_OK, "");
 }
 
 double nmlmc_last_kernel_ms(void) { return g_engine.last_ms; }
 const char* nmlmc_last_kernel_name(void) { return g_engine.last_kernel; }
+
+#ifndef NMLMC_SOURCE_DIGEST
+#define NMLMC_SOURCE_DIGEST "unknown"
+#endif
+const char* nmlmc_source_digest(void) { return NMLMC_SOURCE_DIGEST; }
 uint32_t nmlmc_last_group_lanes(void) { return g_engine.last_G; }
 
 int nmlmc_sample_levels(uint32_t n, const uint32_t* ells, const uint64_t* Ms, sample_output* outs) {

@@ -10,7 +10,7 @@ namespace nmlmc {
 constexpr int kMaxPhi = 4;
 constexpr int kMaxMoments = 4;
 constexpr int kStats = 6;       // inner_stats reported through the ABI
-constexpr int kStatsExt = 8;    // + mixed-form diagnostics: FP64 re-evaluations, flips
+constexpr int kStatsExt = 9;    // + mixed-form diagnostics: FP64 re-evaluations, flips, rechecked decisions
 
 // One record of per-group partial results, reduced in a fixed order by
 // k_reduce_partials: the per-level block of the reference (nested_sampler.hpp:587).

@@ -62,6 +62,7 @@ struct Engine {
     ItemState* bat_items = nullptr;
     TestEntry* bat_tf[2] = {nullptr, nullptr};
     TestEntry* bat_tc[2] = {nullptr, nullptr};
+    TestEntry* bat_rq[2] = {nullptr, nullptr};  // mixed: decisions left to the FP64 recheck
     FinalEntry* bat_fpool = nullptr;
     uint32_t* bat_reeval = nullptr;
     uint32_t* bat_pend = nullptr;
@@ -70,7 +71,7 @@ struct Engine {
     BatchedCtl* bat_ctl = nullptr;
     BatchedCtl* bat_ctl_h = nullptr;
     uint64_t bat_cap = 0, bat_fcap = 0;
-    uint64_t mixed_reeval = 0, mixed_flips = 0;
+    uint64_t mixed_reeval = 0, mixed_flips = 0, mixed_rechecks = 0;
 
     int ensure_device();
     int stage_portfolio(const portfolio_data* data);

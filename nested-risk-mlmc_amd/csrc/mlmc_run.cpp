@@ -258,11 +258,13 @@ int nmlmc_mlmc_run_cb(nmlmc_sample_level_fn fn, void* user, double targetTOL, ui
 // levels at levels_out[i * max_levels ...], the level count at n_levels[i] and
 // the summary at summary_out[i * 5 ...] as nmlmc_mlmc_run gives them (wall =
 // the whole batch's time).
-int nmlmc_batch_mlmc_run(const double* targetTOL, uint64_t M0, double theta, double Ca,
+int nmlmc_batch_mlmc_run(uint32_t n_arg, const double* targetTOL, uint64_t M0, double theta, double Ca,
                          double starting_level_coeff, uint32_t max_levels, nmlmc_level_stats* levels_out,
                          uint32_t* n_levels, double* summary_out) {
     const uint32_t n = (uint32_t)g_engine.batch.size();
     if (n == 0) return set_error(NMLMC_ERR_STATE, "no batch (nmlmc_batch_begin)");
+    if (n_arg != n)
+        return set_error(NMLMC_ERR_ARG, "batch_mlmc_run: %u array entries for a batch of %u instances", n_arg, n);
     if (!targetTOL) return set_error(NMLMC_ERR_ARG, "targetTOL is NULL");
     for (uint32_t i = 0; i < n; i++)
         if (!(targetTOL[i] > 0)) return set_error(NMLMC_ERR_ARG, "targetTOL[%u] must be positive", i);

@@ -32,6 +32,18 @@ struct ModelPolicy {
         return kMixedBound * (mean_mag + fabs(start));
     }
 
+    // Mixed form, refinement decisions: bounds on |s1 - s1_64| and |s2 - s2_64|
+    // of n FP32 terms (their error scales sum to sab, squared to smm) against
+    // the FP64 sums of the same draws.  Each FP32 term is within
+    // e_i = 2^-19 (mag_i + 1) of its FP64 value (normals within 2^-20.5, float
+    // rounding of Y and of the products 2^-23 of mag), so |s1 - s1_64| is at
+    // most 2^-19 (sab + n), inside the threshold bound plus 2^-19 n, and
+    // |s2 - s2_64| <= sum(2 |d_i| e_i + e_i^2) with |d_i| <= mag_i.
+    __device__ static void decision_bounds(double n, double start, double sab, double smm, double& e1, double& e2) {
+        e1 = n * mixed_bound(sab / n, start) + 0x1p-19 * n;
+        e2 = 0x1p-17 * (smm + sab) + 0x1p-36 * n;
+    }
+
     __device__ static uint64_t key1(const Args& A, uint64_t m) {
         return ((uint64_t)A.c.ell << 46) | (A.c.offset + m);
     }
@@ -94,6 +106,7 @@ struct ModelPolicy {
             const float Yf = (float)Y;
             const float sa = (float)A.mp.stau, sb = (float)A.mp.s1tau;
             Kahan b1, b2, bab;
+            float m2 = 0.f;
             for (uint64_t u = u0; u < u1; u++) {
                 const u64x2 w = threefry2x64_20(0, ((uint64_t)tag << 40) | u, A.c.seed, k1);
                 float x, y;
@@ -103,22 +116,26 @@ struct ModelPolicy {
                     const float d = p * p - q * q;
                     b1.add(d);
                     b2.add(d * d);
-                    bab.add(p * p + q * q);
+                    const float mg = p * p + q * q;
+                    bab.add(mg);
+                    m2 += mg * mg;
                     cnt++;
                 } else {
                     const uint64_t sm = 2 * u;
                     if (sm >= i0) {
-                        const float d = Yf + x;
+                        const float d = Yf + x, mg = fabsf(Yf) + fabsf(x);
                         b1.add(d);
                         b2.add(d * d);
-                        bab.add(fabsf(Yf) + fabsf(x));
+                        bab.add(mg);
+                        m2 += mg * mg;
                         cnt++;
                     }
                     if (sm + 1 < i1) {
-                        const float d = Yf + y;
+                        const float d = Yf + y, mg = fabsf(Yf) + fabsf(y);
                         b1.add(d);
                         b2.add(d * d);
-                        bab.add(fabsf(Yf) + fabsf(y));
+                        bab.add(mg);
+                        m2 += mg * mg;
                         cnt++;
                     }
                 }
@@ -126,6 +143,7 @@ struct ModelPolicy {
             s.s1 += b1.value();
             s.s2 += b2.value();
             s.sab += bab.value();
+            s.smm += m2;
         }
         s.work += (double)cnt;
         s.nn += cnt * (VAR == MODEL_REF ? 2 : 1);

@@ -69,6 +69,16 @@ struct PortfolioPolicyT {
     // 2^19).  start is FP64.
     __device__ static double mixed_bound(double mean_mag, double) { return 0x1p-18 * mean_mag; }
 
+    // Mixed form, refinement decisions: bounds on |s1 - s1_64| and |s2 - s2_64|
+    // of n terms with error scales summing to sab (squares: smm).  An FP32 term
+    // d_i is within e_i = 2^-18 mag_i of its FP64 value and |d_i| <= mag_i, so
+    // |d_i^2 - d64_i^2| <= (2 + 2^-18) 2^-18 mag_i^2; the FP64 terms' sums differ
+    // only in order (covered by sab's 2^-16 share and by the caller's 2^-21 s2).
+    __device__ static void decision_bounds(double n, double, double sab, double smm, double& e1, double& e2) {
+        e1 = 0x1p-18 * sab;
+        e2 = 0x1p-16 * smm;
+    }
+
     __device__ static uint64_t key1(const Args& A, uint64_t m) {
         if (MULTI) {
             const uint32_t k = segment(A, m);
@@ -222,6 +232,7 @@ struct PortfolioPolicyT {
         const uint64_t k1 = key1(A, m);
         double a1 = 0., a2 = 0., ab = 0., work = 0.;
         Kahan b1, b2, bab;
+        float mm = 0.f;
         bool redo64 = false;  // a term at a payoff discontinuity (mixed form)
         if (EXACT_ONLY && F32) {
             // Two inner samples per iteration: both option searches and record
@@ -229,6 +240,7 @@ struct PortfolioPolicyT {
             // computed, so one sample's gathers overlap the other's arithmetic.
             const double cp = pf.compute_prob[1];
             const double wc = pf.w_est_const;
+            float m2 = 0.f;
             auto term = [&](const MixRec& q, float Rq, const u64x2& blk1, uint32_t idx) {
                 float x, y, mag;
                 boxmuller_f32(blk1.a, blk1.b, x, y);  // second = word 3 (element a), first = word 2
@@ -238,6 +250,7 @@ struct PortfolioPolicyT {
                 b1.add(d);
                 b2.add(d * d);
                 bab.add(mag);
+                m2 += mag * mag;
             };
             uint64_t u = u0;
             for (; u + 1 < u1; u += 2) {
@@ -265,6 +278,7 @@ struct PortfolioPolicyT {
             s.s1 += b1.value();
             s.s2 += b2.value();
             s.sab += bab.value();
+            s.smm += m2;
             s.work += work;
             return;
         }
@@ -350,6 +364,7 @@ struct PortfolioPolicyT {
                         b1.add(d);
                         b2.add(d * d);
                         bab.add(mag);
+                        mm += mag * mag;
                     } else if (method == 0) {
                         add64(inner_bs_d(pf, o, R, work) / prob);
                     } else {
@@ -392,6 +407,7 @@ struct PortfolioPolicyT {
         s.s1 += a1 + (F32 ? b1.value() : 0.);
         s.s2 += a2 + (F32 ? b2.value() : 0.);
         s.sab += F32 ? (redo64 ? (double)INFINITY : bab.value() + 0x1p-16 * ab) : 0.;
+        s.smm += F32 ? (double)mm : 0.;
         s.work += work;
     }
 };

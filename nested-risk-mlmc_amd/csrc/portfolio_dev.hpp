@@ -554,10 +554,6 @@ __device__ __forceinline__ float inner_exact_f(const PortfolioDevice& pf, const 
 // indicator S < K, whose stock level lies within 2^-17 K of the strike may
 // jump under that rounding; the jump's size (|discp|, or the indicator's
 // factor) enters mag with the weight 2^18, so that the bound covers it whole.
-// The rare normals whose radius word rounds to 1 in FP32 (about 2^-25 of the
-// pairs; the FP64 radius is then up to 2^-12 while the FP32 one is 0) can be
-// off by up to 2^-12 (measured 2.4e-4): such a term exceeds its own share, but
-// over a mean of N >= 12 inner samples the bound on the mean still covers it.
 __device__ __forceinline__ float inner_exact_mix(const MixRec& q, const MixStock& st, bool anti, bool cv_on,
                                                  float R, float z0, float z1, float& mag) {
     const int type = (int)(q.stype & 0xffu) - 2;

@@ -83,12 +83,27 @@ __device__ __forceinline__ void boxmuller_f64(uint64_t u0, uint64_t u1, double& 
 
 // FP32 Box-Muller for the mixed form: the same words, u1 -> (0,1] in float and
 // u0 -> [-1,1], sin/cos(pi x) and log in hardware-accelerated single precision.
+// The float of u1 carries a relative error up to 2^-24, i.e. an absolute error
+// of about 2^-24 in ln u1; the radius r = sqrt(-2 ln u1) then has an error of
+// about 2^-24 / r, which for u1 near 1 (small r) would reach 100% (a word of
+// u1 >= 2^64 - 2^39 rounds to 1: radius 0 against an FP64 radius up to 2^-12).
+// Words with u1 > 1 - 2^-8 (top byte all ones, 2^-8 of them) therefore take
+// the radius from q = 1 - u01(u1) = ~u1 2^-64 + 2^-65, whose float is accurate
+// to 2^-24 relative: -ln(1 - q) = q (1 + q/2 + q^2/3 + q^3/4), truncation below
+// 2^-33 relative for q <= 2^-8.  Elsewhere r >= 2^-3.5 and the error stays
+// below about 2^-20.5.
 __device__ __forceinline__ void boxmuller_f32(uint64_t u0, uint64_t u1, float& x, float& y) {
     const float a = (float)(int64_t)u0 * 0x1p-63f + 0x1p-64f;
-    const float b = (float)u1 * 0x1p-64f + 0x1p-65f;
     float s, c;
     __sincosf(3.14159265358979f * a, &s, &c);
-    const float r = __fsqrt_rn(-2.f * __logf(b));
+    float r2;
+    if ((uint32_t)(u1 >> 32) < 0xFF000000u) {
+        r2 = -2.f * __logf((float)u1 * 0x1p-64f + 0x1p-65f);
+    } else {
+        const float q = (float)(~u1) * 0x1p-64f + 0x1p-65f;
+        r2 = 2.f * q * (1.f + q * (0.5f + q * (0x1.555556p-2f + q * 0.25f)));
+    }
+    const float r = __fsqrt_rn(r2);
     x = s * r;
     y = c * r;
 }

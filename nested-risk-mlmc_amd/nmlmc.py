@@ -87,6 +87,7 @@ EXPORTS = [
     "portfolio_estimate_delta_for_unbiased", "portfolio_gen_data", "portfolio_free_data",
     "nmlmc_mlmc_last_table", "nmlmc_autotune_level", "nmlmc_sample_levels",
     "nmlmc_batch_begin", "nmlmc_batch_stage", "nmlmc_batch_sample", "nmlmc_batch_mlmc_run",
+    "nmlmc_source_digest",
 ]
 
 _lib = None
@@ -96,14 +97,39 @@ class NmlmcError(RuntimeError):
     pass
 
 
+def source_digest():
+    """sha256 (first 16 hex digits) of the library's sources as the Makefile
+    computes it: csrc/*.hip, *.cpp, *.hpp and include/nmlmc.h, concatenated in
+    path order; None when the sources are not beside the package."""
+    import glob
+    import hashlib
+    files = [p for ext in ("hip", "cpp", "hpp") for p in glob.glob(os.path.join(HERE, "csrc", "*." + ext))]
+    files.append(os.path.join(os.path.dirname(HERE), "include", "nmlmc.h"))
+    files = sorted(os.path.abspath(p) for p in files)
+    if not all(os.path.exists(p) for p in files):
+        return None
+    h = hashlib.sha256()
+    for p in files:
+        with open(p, "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
+
+
 def load(path=LIB_PATH):
-    """Load libnmlmc.so (raises if it has not been built)."""
+    """Load libnmlmc.so (raises if it has not been built, or if the in-tree
+    build does not match the sources beside it)."""
     global _lib
     if _lib is not None:
         return _lib
     if not os.path.exists(path):
         raise NmlmcError("%s is missing: build it with `make -C nested-risk-mlmc_amd`" % path)
     lib = C.CDLL(path)
+    lib.nmlmc_source_digest.restype = C.c_char_p
+    built = lib.nmlmc_source_digest().decode()
+    want = source_digest()
+    if want is not None and built != want and not os.environ.get("NMLMC_LIBRARY"):
+        raise NmlmcError("%s was built from other sources (digest %s, sources %s): rebuild it with "
+                         "`make -C nested-risk-mlmc_amd`" % (path, built, want))
     u32, u64, dbl, i32 = C.c_uint32, C.c_uint64, C.c_double, C.c_int
     dp, u64p = C.POINTER(C.c_double), C.POINTER(C.c_uint64)
     sig = {
@@ -130,16 +156,17 @@ def load(path=LIB_PATH):
         "nmlmc_model_init": (i32, [i32, i32, dbl, dbl, u64]),
         "nmlmc_portfolio_set_correlation": (i32, [dp, u32]),
         "nmlmc_sample": (i32, [u32, u64, u64, C.POINTER(SampleOutput), dp, dp, u64p, u64p]),
-        "nmlmc_mixed_stats": (i32, [u64p, u64p]),
+        "nmlmc_mixed_stats": (i32, [u64p, u64p, u64p]),
         "nmlmc_last_kernel_ms": (dbl, []),
         "nmlmc_last_kernel_name": (C.c_char_p, []),
+        "nmlmc_source_digest": (C.c_char_p, []),
         "nmlmc_last_group_lanes": (u32, []),
         "nmlmc_autotune_level": (i32, [u32, u64, C.POINTER(u32), C.POINTER(u32), dp]),
         "nmlmc_sample_levels": (i32, [u32, C.POINTER(u32), C.POINTER(u64), C.POINTER(SampleOutput)]),
         "nmlmc_batch_begin": (i32, [u32]),
         "nmlmc_batch_stage": (i32, [u32, C.POINTER(PortfolioData), u64, dp, u32]),
-        "nmlmc_batch_sample": (i32, [u32, C.POINTER(u64), C.POINTER(SampleOutput)]),
-        "nmlmc_batch_mlmc_run": (i32, [dp, u64, dbl, dbl, dbl, u32, C.POINTER(LevelStats),
+        "nmlmc_batch_sample": (i32, [u32, u32, C.POINTER(u64), C.POINTER(SampleOutput)]),
+        "nmlmc_batch_mlmc_run": (i32, [u32, dp, u64, dbl, dbl, dbl, u32, C.POINTER(LevelStats),
                                        C.POINTER(u32), dp]),
         "nmlmc_mlmc_run": (i32, [dbl, u64, dbl, dbl, dbl, i32, u32, C.POINTER(LevelStats),
                                  C.POINTER(C.c_uint32), dp]),
@@ -255,9 +282,11 @@ def sample_levels(ells, Ms, moments=4, phi_size=1):
 
 
 def mixed_stats():
-    a, b = C.c_uint64(), C.c_uint64()
-    check(load().nmlmc_mixed_stats(C.byref(a), C.byref(b)))
-    return a.value, b.value
+    """Mixed form diagnostics of the last call: (outer samples re-evaluated in
+    FP64, indicator flips, refinement decisions rechecked in FP64)."""
+    a, b, r = C.c_uint64(), C.c_uint64(), C.c_uint64()
+    check(load().nmlmc_mixed_stats(C.byref(a), C.byref(b), C.byref(r)))
+    return a.value, b.value, r.value
 
 
 def last_kernel_ms():
@@ -460,7 +489,7 @@ def batch_mlmc_run(targetTOLs, M0, theta=0.5, Ca=3.0, starting_level_coeff=1.5, 
     levels = (LevelStats * (n * max_levels))()
     nl = (C.c_uint32 * n)()
     summ = (C.c_double * (5 * n))()
-    check(lib.nmlmc_batch_mlmc_run((C.c_double * n)(*targetTOLs), M0, theta, Ca, starting_level_coeff,
+    check(lib.nmlmc_batch_mlmc_run(n, (C.c_double * n)(*targetTOLs), M0, theta, Ca, starting_level_coeff,
                                    max_levels, levels, nl, summ))
     res = []
     for i in range(n):
@@ -491,7 +520,7 @@ def batch_sample(ell, Ms, moments=4, phi_size=1):
     n = len(Ms)
     outs = [Output(moments, phi_size) for _ in range(n)]
     arr = (SampleOutput * n)(*[o.c for o in outs])
-    check(lib.nmlmc_batch_sample(ell, (C.c_uint64 * n)(*Ms), arr))
+    check(lib.nmlmc_batch_sample(ell, n, (C.c_uint64 * n)(*Ms), arr))
     res = []
     for i, o in enumerate(outs):
         o.c = arr[i]

@@ -280,6 +280,8 @@ class Ref(_Common):
                                          C.c_uint32, C.c_uint32]
         lib.refp_model_init.argtypes = [C.c_int, C.c_double, C.c_double, C.c_uint64]
         lib.refp_model_sample.argtypes = [C.c_uint32, C.c_uint64, C.c_uint32, _dp, _dp, _dp, _u64p]
+        lib.refp_shift_init.argtypes = [C.c_double, C.c_uint64]
+        lib.refp_shift_sample.argtypes = [C.c_uint32, C.c_uint64, C.c_uint32, _dp, _dp, _dp, _u64p]
         lib.refp_portfolio_set.argtypes = (
             [C.c_uint32, _dp, _dp, _dp, _u32p, _dp, _dp, _dp, _u32p, _ip, _dp, _dp, C.c_double,
              C.c_double, C.c_double, C.c_double, C.c_uint32, C.c_uint32, C.c_double, C.c_int,
@@ -324,6 +326,20 @@ class Ref(_Common):
         stats = np.zeros(6, np.uint64)
         self.lib.refp_model_sample(ell, M, self.moments, _ptr(fs, _dp), _ptr(ds, _dp),
                                    _ptr(work, _dp), _ptr(stats, _u64p))
+        return dict(fsums=fs, diffsums=ds, work=work[0], stats=stats)
+
+    def shift_init(self, max_loss, seed):
+        """The shift model X = Y + Z (Indicator phi) on the reference's engine."""
+        self.phi_size = 1
+        self.lib.refp_shift_init(max_loss, seed)
+
+    def shift_sample(self, ell, M):
+        fs = np.zeros(self.moments)
+        ds = np.zeros(self.moments)
+        work = np.zeros(1)
+        stats = np.zeros(6, np.uint64)
+        self.lib.refp_shift_sample(ell, M, self.moments, _ptr(fs, _dp), _ptr(ds, _dp), _ptr(work, _dp),
+                                   _ptr(stats, _u64p))
         return dict(fsums=fs, diffsums=ds, work=work[0], stats=stats)
 
     def portfolio_set(self, pf):

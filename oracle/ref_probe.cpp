@@ -180,6 +180,49 @@ void refp_model_sample(uint32_t ell, uint64_t M, uint32_t moments, double* fsums
     for (int i = 0; i < c_inner_stats_count; i++) stats[i] = out.inner_stats[i];
 }
 
+}  // extern "C"
+
+// The shift model of the benchmark workloads (outer Y ~ N(0,1), inner
+// X = Y + Z, threshold max_loss, Indicator phi) as a sampler class of the
+// reference's compile-time SAMPLER concept (the members model::sampler has),
+// so that the reference's own engine (sample<>, inner_sampler, random_state;
+// nested_sampler.hpp) runs that workload: the CPU baseline of the benchmark.
+namespace refp_shift {
+static real_t g_max_loss = 2;
+class sampler : public inner_sampler_base {
+    real_t Y;
+public:
+    static uint32_t GetWorkspaceSize(const model::data&) { return 0; }
+    sampler(void*) : Y(NAN) {}
+    void sample_outer(uint64_t, work_t&) { Y = randn_sync(); }
+    void copy_outer(const sampler& o) { Y = o.Y; }
+    real_t init_inner_sampler(uint64_t&, work_t&) { return -g_max_loss; }
+    real_t sample_inner(uint64_t, uint64_t, work_t& w) {
+        w++;
+        return Y + randn();
+    }
+    static void add_phi(real_t avg, real_t coeff, real_t* phi) { *phi += coeff * (avg >= 0.f); }
+};
+}  // namespace refp_shift
+
+extern "C" {
+
+void refp_shift_init(double max_loss, uint64_t seed) {
+    refp_shift::g_max_loss = static_cast<real_t>(max_loss);
+    gc_phi_size = 1;
+    init_samplers<refp_shift::sampler>(seed, 0);
+}
+
+void refp_shift_sample(uint32_t ell, uint64_t M, uint32_t moments, double* fsums, double* diffsums,
+                       double* work, uint64_t* stats) {
+    sample_output out(moments, moments, 1);
+    out.M = M;
+    sample<refp_shift::sampler>(ell, 0, &out.M, out.fsums, out.diffsums, &out.total_work, out.inner_stats);
+    for (uint32_t i = 0; i < moments; i++) { fsums[i] = out.fsums[i]; diffsums[i] = out.diffsums[i]; }
+    *work = out.total_work;
+    for (int i = 0; i < c_inner_stats_count; i++) stats[i] = out.inner_stats[i];
+}
+
 // Portfolio from flat arrays.  Options are given per compute method, already in the
 // order the caller wants them staged (descending prob, as portfolio_update_data asserts).
 static portfolio::data g_probe_pdata;

@@ -69,6 +69,18 @@ def test_errors_without_a_gpu(nm):
     assert lib.nmlmc_set_adaptive_rule(1, 3) == 1  # needs init first
     lib.init_nestedmc_sampler(0, None)
     assert lib.nmlmc_last_error() == 2
+    # Batch calls check their array length against the staged batch (none here).
+    Ms = (C.c_uint64 * 2)(10, 10)
+    outs = (nm.SampleOutput * 2)()
+    assert lib.nmlmc_batch_sample(0, 2, Ms, outs) == 2
+    assert b"batch of 0" in lib.nmlmc_last_error_string()
+    with pytest.raises(nm.NmlmcError):
+        nm.batch_sample(0, [10, 10])
+
+
+def test_library_matches_sources(nm):
+    """The loaded library was built from the sources beside it."""
+    assert nm.load().nmlmc_source_digest().decode() == nm.source_digest()
 
 
 def _two_point(mean, var, M, real=float):

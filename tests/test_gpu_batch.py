@@ -7,7 +7,7 @@ import importlib
 import numpy as np
 import pytest
 
-from portfolio_util import synthetic, to_library
+from portfolio_util import synthetic, to_library, to_oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -124,6 +124,50 @@ def test_batch_with_asian_book_matches_alone(nm, form):
     nm.set_form(nm.FORM_PLAIN)
 
 
+@pytest.mark.parametrize("form", [1, 2])
+def test_batch_books_match_oracle(nm, oracle_f64, form):
+    """Six books of the batched-risk workload (d in {10, 50, 100}, P
+    log-uniform in [10^3, 10^5], 20% Asians by the Unbiased method, 20%
+    digitals, factor correlation, pilot-quantile thresholds) sampled together,
+    each instance's level sums, work and counts against the oracle run on that
+    book alone with the batched keying (its seed, continuing outer ids): the
+    batch is checked against the reference algorithm itself, not only against
+    the library's own lone runs."""
+    cfg = importlib.import_module(nm.__name__.rsplit(".", 1)[0] + ".configs")
+    books = cfg.batch_books(6, seed=7, n_pilot=100, asian_frac=0.2)
+    calls = [(0, [160, 90, 200, 120, 60, 100]), (1, [80, 50, 60, 40, 70, 30]),
+             (2, [24, 30, 16, 20, 12, 28]), (1, [20, 10, 30, 10, 20, 10])]
+    nm.init_nestedmc_sampler(1, 64, 2, 0.0, True)
+    nm.set_adaptive_rule(1, 4)
+    nm.set_form(form)
+    nm.batch_begin(len(books))
+    keep, orcs = [], []
+    for i, (a, L, sc, q) in enumerate(books):
+        pf, start, delta = to_library(nm, a, **sc)
+        keep.append(pf)
+        nm.batch_stage(i, pf, 300 + i, L)
+        o = oracle_f64.__class__("f64").new(1, 64, 2, 0.0, True, adapt_rule=1, cap_log2=4)
+        o.portfolio_init(to_oracle(a, start, delta, **sc), 300 + i, L)
+        orcs.append(o)
+    offsets = {}
+    for ell, Ms in calls:
+        print("batch level", ell, Ms, flush=True)
+        got = nm.batch_sample(ell, Ms)
+        for i, o in enumerate(orcs):
+            off = offsets.get((i, ell), 0)
+            r = o.sample(ell, Ms[i], keying=1, outer_offset=off)
+            offsets[(i, ell)] = off + Ms[i]
+            g = got[i]
+            assert np.array_equal(g["stats"], r["stats"]), (i, ell, g["stats"], r["stats"])
+            np.testing.assert_allclose(g["work"], r["work"], rtol=1e-12)
+            np.testing.assert_allclose(g["fsums"], r["fsums"], rtol=1e-9, atol=1e-300)
+            np.testing.assert_allclose(g["diffsums"], r["diffsums"], rtol=1e-9, atol=1e-300)
+        if form == 2:
+            print("mixed stats (redone, flips, rechecked)", nm.mixed_stats(), flush=True)
+    nm.cleanup_nestedmc_sampler()
+    nm.set_form(nm.FORM_PLAIN)
+
+
 def test_batch_errors(nm):
     """A batch needs staged instances and a batched form; at most 64 instances."""
     nm.init_nestedmc_sampler(1, 16, 2, 0.0, True)
@@ -137,6 +181,10 @@ def test_batch_errors(nm):
     with pytest.raises(nm.NmlmcError):
         nm.batch_sample(0, [10, 10])  # instance 1 not staged
     nm.batch_stage(1, pf, seed + 1)
+    with pytest.raises(nm.NmlmcError):
+        nm.batch_sample(0, [10])  # one entry for a batch of two
+    with pytest.raises(nm.NmlmcError):
+        nm.batch_mlmc_run([1e-2, 1e-2, 1e-2], 100)  # three tolerances for a batch of two
     nm.set_form(nm.FORM_PLAIN)
     with pytest.raises(nm.NmlmcError):
         nm.batch_sample(0, [10, 10])

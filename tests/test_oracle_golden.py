@@ -185,3 +185,24 @@ def test_live_against_reference_build(real):
             _cmp_call(O.sample(ell, M), {k: (v.tolist() if hasattr(v, "tolist") else v)
                                          for k, v in R.model_sample(ell, M).items()},
                       exact=cores == 1)
+
+
+@pytest.mark.skipif(not ref_available(), reason="reference build (oracle/_ref) not present")
+def test_shift_model_against_reference_engine():
+    """The benchmark's shift model (X = Y + Z, threshold 2, Indicator) as a
+    sampler class on the reference's own engine, against the oracle's shift
+    model under the reference keying: the non-adaptive sweep of levels 0..8
+    with N = 32 2^l, antithetic, seed 1, 8 threads (the CPU-path workload at a
+    reduced M0).  Work and counts exact; sums to 1e-12 (the reference adds its
+    threads' sums in completion order)."""
+    O, R = Oracle("f64"), Ref("f64")
+    O.new(8, 32, 2, 0.0, True)
+    R.init(8, 32, 2, 0.0, True)
+    O.model_init(1, 0, 0.5, 2.0, 1)
+    R.shift_init(2.0, 1)
+    for ell in range(9):
+        M = int(np.ceil(4000 * 2.0 ** (-1.5 * ell)))
+        o, r = O.sample(ell, M), R.shift_sample(ell, M)
+        assert o["work"] == r["work"] and np.array_equal(o["stats"], r["stats"])
+        np.testing.assert_allclose(o["fsums"], r["fsums"], rtol=1e-12, atol=1e-300)
+        np.testing.assert_allclose(o["diffsums"], r["diffsums"], rtol=1e-12, atol=1e-300)
