@@ -29,6 +29,17 @@ scaled figures are kept in profiles/hbm_traffic.json and read from there.
 
 cpu_baseline: the oracle (C restatement, FP64, same keying and workload) on the
 host cores with OpenMP, over a bounded sample of the sweep (M_l scaled down).
+Its calibration against the reference's own OpenMP engine (oracle/_ref, where
+present) on the workload both run (the non-adaptive sweep of config1) is
+reported beside it.
+
+Also reported: per-level comparisons of the mixed form with the batched FP64
+form on the same outer samples (count flips, indicator flips, FP64 redos and
+rechecked decisions, level-mean differences), for the model and for the large
+book; the portfolio legs' rooflines (VALU lane-ops of the Exact inner samples,
+opcount.json); config1 (the non-adaptive model sweep of the CPU path, levels
+0..8, M_l = ceil(1e6 2^-1.5l)) on the GPU forms and on the reference's own
+OpenMP engine; config3 (the small book) beside the large one.
 """
 import argparse
 import json
@@ -36,6 +47,8 @@ import math
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -66,6 +79,39 @@ def sweep(nm, M0, levels, rec=None):
                             inner=int(o["stats"][0]), G=nm.last_group_lanes(), reev=reev,
                             flips=flips, rechecks=rck, fmean=o["fsums"][0] / M,
                             dmean=(o["diffsums"][0] / M) if ell else 0.))
+
+
+def compare_forms(nm, setup_fn, M0, levels):
+    """The mixed and the batched FP64 forms on the same outer samples (outer ids
+    0..M_l - 1 of every level): per level the refinement-count flips (N_f or
+    N_c differing), indicator flips (phi differing), the mixed form's FP64
+    redos, flips of its FP32 indicators before the redo and decisions
+    rechecked in FP64, and the difference of the level means (fine mean at
+    l = 0, difference mean above)."""
+    runs = {}
+    for form in (nm.FORM_MIXED, nm.FORM_BATCHED):
+        setup_fn(nm, form, 1)
+        rs = []
+        for ell in range(levels + 1):
+            M = level_M(M0, ell)
+            o = nm.sample(ell, M, outer_offset=0, trace=True)
+            o["mixed_stats"] = nm.mixed_stats() if form == nm.FORM_MIXED else (0, 0, 0)
+            rs.append(o)
+        nm.cleanup_nestedmc_sampler()
+        runs[form] = rs
+    out = []
+    for ell, (a, b) in enumerate(zip(runs[nm.FORM_MIXED], runs[nm.FORM_BATCHED])):
+        M = level_M(M0, ell)
+        ma = (a["fsums"][0] if ell == 0 else a["diffsums"][0]) / M
+        mb = (b["fsums"][0] if ell == 0 else b["diffsums"][0]) / M
+        reev, flips, rck = a["mixed_stats"]
+        out.append({"ell": ell, "M": M,
+                    "count_flips": int(np.sum((a["nf"] != b["nf"]) | (a["nc"] != b["nc"]))),
+                    "indicator_flips": int(np.sum((a["phi_f"] != b["phi_f"]) | (a["phi_c"] != b["phi_c"]))),
+                    "redone_fp64": int(reev), "fp32_indicator_flips_before_redo": int(flips),
+                    "rechecked_decisions": int(rck), "level_mean_mixed": ma, "level_mean_fp64": mb,
+                    "level_mean_diff": abs(ma - mb)})
+    return out
 
 
 def autotune(nm, M0, levels):
@@ -123,6 +169,22 @@ def roofline(recs, form):
             "ops_per_inner_sample": oi}
 
 
+def portfolio_roofline(recs, form_key):
+    """Roofline of the large book's level calls: Exact inner samples x VALU
+    lane-ops per sample (opcount.json, portfolio_exact) over the level calls'
+    kernel time (the outer draws and the engine's bookkeeping are not counted
+    as algorithmic work)."""
+    with open(os.path.join(ROOT, "nested-risk-mlmc_amd", "opcount.json")) as f:
+        ops = json.load(f)["portfolio_exact"][form_key]["inner"]
+    inner = sum(r["inner"] for r in recs)
+    secs = sum(r["kernel_ms"] for r in recs) / 1e3
+    achieved = inner * ops / secs / 1e12
+    return {"bound": "valu", "achieved": round(achieved, 3), "peak": round(PEAK_VALU_TOPS, 1), "unit": "TOP/s",
+            "frac": round(achieved / PEAK_VALU_TOPS, 4), "ops_per_inner_sample": ops,
+            "inner_samples_per_s": round(inner / secs, 1), "launches": len(recs),
+            "avg_launch_ms": round(1e3 * secs / len(recs), 4)}
+
+
 def per_level(recs, levels):
     out = []
     for ell in range(levels + 1):
@@ -158,6 +220,64 @@ def cpu_baseline(M0_cpu, levels):
                       % (levels, M0_cpu, total, dt)}
 
 
+def cpu_config1(M0, levels=8):
+    """Config 1 on the host cores: the non-adaptive shift-model sweep (N = 32 2^l,
+    antithetic, seed 1, M_l = ceil(M0 2^-1.5l)) with the reference keying, on
+    the reference's own OpenMP engine (oracle/_ref, where it was built) and on
+    the oracle, with the same threads: outer samples/s of each and the ratio
+    (the calibration of the oracle as the CPU baseline)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import ctypes
+    from oracle import Oracle, Ref, ref_available  # the checker, timed as the CPU baseline
+    threads = min(16, os.cpu_count() or 1)
+    try:
+        ctypes.CDLL("libgomp.so.1").omp_set_num_threads(threads)
+    except OSError:
+        pass
+
+    def sweep_cpu(fn):
+        per, total = [], 0
+        t0 = time.perf_counter()
+        for ell in range(levels + 1):
+            M = level_M(M0, ell)
+            t1 = time.perf_counter()
+            fn(ell, M)
+            per.append(round(M / (time.perf_counter() - t1), 1))
+            total += M
+        return round(total / (time.perf_counter() - t0), 1), per
+
+    orc = Oracle("f64").new(threads, 32, 2, 0.0, True)
+    orc.model_init(1, 0, 0.5, 2.0, 1)
+    port, port_per = sweep_cpu(lambda l, M: orc.sample(l, M))
+    out = {"cores": threads, "M0": M0, "levels": levels, "port": port, "port_per_level": port_per}
+    if ref_available("f64"):
+        ref = Ref("f64").init(threads, 32, 2, 0.0, True)
+        ref.shift_init(2.0, 1)
+        rv, rper = sweep_cpu(lambda l, M: ref.shift_sample(l, M))
+        out.update(reference=rv, reference_per_level=rper, port_over_reference=round(port / rv, 3))
+    return out
+
+
+def gpu_config1(nm, M0, levels, plain_cores):
+    """Config 1's sweep on the GPU forms: per-level outer samples/s and the whole
+    sweep's rate (non-adaptive: the reference rule with r = 0)."""
+    out = {}
+    for form, cores, label in ((nm.FORM_PLAIN, plain_cores, "plain_fp64"), (nm.FORM_BATCHED, 1, "batched_fp64"),
+                               (nm.FORM_MIXED, 1, "mixed")):
+        nm.init_nestedmc_sampler(cores, 32, 2, 0.0, True)
+        nm.set_form(form)
+        nm.model_init_ex(nm.MODEL_SHIFT, nm.PHI_INDICATOR, 0.5, 2.0, 1)
+        sweep(nm, M0, levels)
+        recs = []
+        t0 = time.perf_counter()
+        sweep(nm, M0, levels, recs)
+        el = time.perf_counter() - t0
+        total = sum(level_M(M0, l) for l in range(levels + 1))
+        out[label] = {"outer_samples_per_s": round(total / el, 1), "per_level": per_level(recs, levels)}
+        nm.cleanup_nestedmc_sampler()
+    return out
+
+
 def time_to_eps(nm, form, cores, eps, M0, setup_fn=None):
     (setup_fn or setup)(nm, form, cores, seed=7)
     t0 = time.perf_counter()
@@ -177,10 +297,13 @@ def portfolio_configs(nm):
     return importlib.import_module(nm.__name__ + ".configs")
 
 
-def portfolio_bench(nm, which, levels, M0, rel_eps, tte, plain_cores, tte_M0, tune=True):
+def portfolio_bench(nm, which, levels, M0, rel_eps, tte, plain_cores, tte_M0, tune=True, rel_eps2=None):
     """Portfolio books: per-level outer samples/s of the mixed and plain forms
-    (N_base = 64 2^l, 3-sigma doubling capped at 2^4, M_l = ceil(M0 2^-1.5l))
-    and time to RMSE eps = rel_eps * (1 - q) (q the threshold's quantile)."""
+    (N_base = 64 2^l, 3-sigma doubling capped at 2^4, M_l = ceil(M0 2^-1.5l)),
+    their rooflines, the per-level comparison of the mixed form with the
+    batched FP64 form on the same outer samples, and time to RMSE
+    eps = rel_eps * (1 - q) (q the threshold's quantile), and with rel_eps2 at
+    that tighter tolerance too."""
     cfg = portfolio_configs(nm)
     t0 = time.perf_counter()
     a, L, sc = cfg.large_portfolio() if which == "large" else cfg.small_portfolio()
@@ -215,25 +338,35 @@ def portfolio_bench(nm, which, levels, M0, rel_eps, tte, plain_cores, tte_M0, tu
         total = sum(level_M(M0, l) for l in range(levels + 1))
         r = {"outer_samples_per_s": round(total / el, 1), "per_level": per_level(recs, levels),
              "kernel_ms": [round(x["kernel_ms"], 3) for x in recs],
-             "inner_samples": sum(x["inner"] for x in recs)}
+             "inner_samples": sum(x["inner"] for x in recs),
+             "roofline": portfolio_roofline(recs, "mixed" if form == nm.FORM_MIXED else "fp64")}
         if form == nm.FORM_MIXED:
             r["launch_shapes"] = shapes
             r["reevaluated"] = sum(x["reev"] for x in recs)
             r["flips"] = sum(x["flips"] for x in recs)
+            r["rechecked_decisions"] = sum(x["rechecks"] for x in recs)
         nm.cleanup_nestedmc_sampler()
         if tte:
             r["time_to_eps"] = time_to_eps(nm, form, cores, eps, tte_M0, setup_fn=pf_setup)
+            if rel_eps2:
+                r["time_to_eps_tight"] = time_to_eps(nm, form, cores, rel_eps2 * (1 - q), tte_M0,
+                                                     setup_fn=pf_setup)
         out[label] = r
         print("# portfolio %s %s: %.3e outer samples/s%s" % (
             which, label, r["outer_samples_per_s"],
             (", time to eps %.2f s" % r["time_to_eps"]["seconds"]) if tte else ""),
             file=sys.stderr, flush=True)
+    out["mixed_vs_fp64_per_level"] = compare_forms(nm, pf_setup, M0, levels)
     if tte:
-        m, p_ = out["mixed"]["time_to_eps"], out["plain_fp64"]["time_to_eps"]
-        out["eps"] = eps
-        out["speedup_time_to_eps"] = round(p_["seconds"] / m["seconds"], 2)
-        sig = math.hypot(m["stat_error"], p_["stat_error"])
-        out["estimates_within_3_sigma"] = bool(abs(m["estimate"] - p_["estimate"]) <= 3 * sig)
+        for key, e in (("time_to_eps", eps), ("time_to_eps_tight", rel_eps2 * (1 - q) if rel_eps2 else None)):
+            if e is None:
+                continue
+            m, p_ = out["mixed"][key], out["plain_fp64"][key]
+            # The driver's stat_error is sqrt(Ca sum V/M) with Ca = 3: one
+            # standard error is stat_error / sqrt(3).
+            sig = math.hypot(m["stat_error"], p_["stat_error"]) / math.sqrt(3.0)
+            out[key] = {"eps": e, "rel_eps": e / (1 - q), "speedup": round(p_["seconds"] / m["seconds"], 2),
+                        "estimates_within_3_sigma": bool(abs(m["estimate"] - p_["estimate"]) <= 3 * sig)}
     return out
 
 
@@ -334,6 +467,13 @@ def main():
     ap.add_argument("--pf-M0", type=float, default=65536)
     ap.add_argument("--pf-rel-eps", type=float, default=0.05)
     ap.add_argument("--pf-tte-M0", type=int, default=1024)
+    ap.add_argument("--pf-rel-eps2", type=float, default=0.025,
+                    help="a second, tighter large-book tolerance (x (1 - q)); 0: skip")
+    ap.add_argument("--cmp-M0", type=float, default=1e6,
+                    help="M0 of the per-level mixed-vs-FP64 comparison sweep of the model")
+    ap.add_argument("--config1-M0", type=float, default=1e6)
+    ap.add_argument("--no-config1", action="store_true")
+    ap.add_argument("--no-small", action="store_true", help="skip the small book (config 3)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (for scaling rocprofv3 byte counters)")
     args = ap.parse_args()
@@ -364,6 +504,7 @@ def main():
     value = outer_per_step * args.steps / elapsed
     reev = sum(r["reev"] for r in recs)
     flips = sum(r["flips"] for r in recs)
+    rechecks = sum(r["rechecks"] for r in recs)
     rl = roofline(recs, chosen)
     levels_mixed = per_level(recs, args.levels)
     print("# mixed: %.3e outer samples/s, %.1f ms/step, roofline %.3f of VALU peak"
@@ -393,25 +534,16 @@ def main():
         bel = time.perf_counter() - t0
         extra["batched_fp64"] = {"outer_samples_per_s": round(outer_per_step / bel, 1),
                                  "per_level": per_level(brecs, args.levels), "launch_shapes": bshapes}
-        # The mixed form's first timed sweep and this sweep draw the same outer
-        # samples (same keys and offsets after one warm-up sweep each): per-level
-        # difference of the level means (fine mean at l = 0, difference mean
-        # above) and the mixed form's FP64 re-evaluations and indicator flips.
-        if args.warmup == 1:
-            first = recs[:args.levels + 1]
-            extra["mixed_vs_fp64_per_level"] = [
-                {"ell": a["ell"], "mean_diff": abs((a["dmean"] if a["ell"] else a["fmean"]) -
-                                                   (b["dmean"] if b["ell"] else b["fmean"])),
-                 "reevaluated": a["reev"], "flips": a["flips"]}
-                for a, b in zip(first, brecs)]
         nm.cleanup_nestedmc_sampler()
+    extra["mixed_vs_fp64_per_level"] = compare_forms(nm, setup, args.cmp_M0, args.levels)
     if not args.no_tte:
         tm = time_to_eps(nm, chosen, 1, args.eps, args.tte_M0)
         print("# time to eps (mixed): %s" % tm, file=sys.stderr, flush=True)
         tp = time_to_eps(nm, nm.FORM_PLAIN, args.plain_cores, args.eps, args.tte_M0)
         print("# time to eps (plain): %s" % tp, file=sys.stderr, flush=True)
         exact = 0.5 * math.erfc(2 / math.sqrt(2))
-        sig = math.hypot(tm["stat_error"], tp["stat_error"])
+        # stat_error = sqrt(Ca sum V/M) with Ca = 3: one standard error is stat_error / sqrt(3).
+        sig = math.hypot(tm["stat_error"], tp["stat_error"]) / math.sqrt(3.0)
         extra["time_to_eps"] = {"eps": args.eps, "mixed": tm, "plain_fp64": tp,
                                 "speedup": round(tp["seconds"] / tm["seconds"], 2),
                                 "exact": exact,
@@ -421,7 +553,18 @@ def main():
     if args.portfolio != "none":
         extra["portfolio"] = portfolio_bench(nm, args.portfolio, args.pf_levels, args.pf_M0,
                                              args.pf_rel_eps, not args.no_tte, args.plain_cores,
-                                             args.pf_tte_M0, tune=not args.no_autotune)
+                                             args.pf_tte_M0, tune=not args.no_autotune,
+                                             rel_eps2=args.pf_rel_eps2 or None)
+    if not args.no_small:
+        extra["config3_small_book"] = portfolio_bench(nm, "small", args.pf_levels, args.pf_M0, args.pf_rel_eps,
+                                                      not args.no_tte, args.plain_cores, args.pf_tte_M0,
+                                                      tune=not args.no_autotune)
+    if not args.no_config1:
+        extra["config1"] = {"workload": "shift model, adaptive refinement off (N = 32 2^l), antithetic, seed 1, "
+                                        "levels 0..8, M_l = ceil(%g 2^-1.5l)" % args.config1_M0,
+                            "gpu": gpu_config1(nm, args.config1_M0, 8, args.plain_cores)}
+        if not args.no_cpu:
+            extra["config1"]["cpu"] = cpu_config1(args.config1_M0)
     if args.batch_n > 0 and not args.no_tte:
         extra["batch"] = batch_bench(nm, args.batch_n, args.batch_rel_eps, args.batch_M0,
                                      0 if args.no_plain else args.plain_cores)
@@ -431,6 +574,12 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args.cpu_M0, args.levels)
+        c1 = extra.get("config1", {}).get("cpu")
+        if c1 and "reference" in c1:
+            cpu["calibration"] = {"workload": "config1 sweep (non-adaptive shift model), same threads",
+                                  "cores": c1["cores"], "port_outer_samples_per_s": c1["port"],
+                                  "reference_outer_samples_per_s": c1["reference"],
+                                  "port_over_reference": c1["port_over_reference"]}
 
     line = {
         "metric": "outer samples/s per level; time to RMSE eps: batched FP32+FP64 vs plain FP64",
@@ -453,6 +602,7 @@ def main():
         "per_level_outer_samples_per_s": levels_mixed,
         "mixed_reevaluated": reev,
         "mixed_flips": flips,
+        "mixed_rechecked_decisions": rechecks,
         "roofline": rl,
         "cpu_baseline": cpu,
     }
