@@ -235,17 +235,24 @@ def cpu_config1(M0, levels=8):
     except OSError:
         pass
 
-    def sweep_cpu(fn):
-        per, total = [], 0
-        t0 = time.perf_counter()
-        for ell in range(levels + 1):
-            M = level_M(M0, ell)
-            t1 = time.perf_counter()
-            fn(ell, M)
-            per.append(round(M / (time.perf_counter() - t1), 1))
-            total += M
-        return round(total / (time.perf_counter() - t0), 1), per
+    def sweep_cpu(fn, reps=2):
+        # The best of reps sweeps (per level and overall).
+        best, best_per = 0., None
+        for _ in range(reps):
+            per, total = [], 0
+            t0 = time.perf_counter()
+            for ell in range(levels + 1):
+                M = level_M(M0, ell)
+                t1 = time.perf_counter()
+                fn(ell, M)
+                per.append(round(M / (time.perf_counter() - t1), 1))
+                total += M
+            rate = round(total / (time.perf_counter() - t0), 1)
+            best_per = per if best_per is None else [max(a, b) for a, b in zip(best_per, per)]
+            best = max(best, rate)
+        return best, best_per
 
+    # Each sweep continues the streams of the one before (the same work).
     orc = Oracle("f64").new(threads, 32, 2, 0.0, True)
     orc.model_init(1, 0, 0.5, 2.0, 1)
     port, port_per = sweep_cpu(lambda l, M: orc.sample(l, M))

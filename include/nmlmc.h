@@ -182,6 +182,19 @@ int nmlmc_model_init(int variant, int phi_type, double tau, double max_loss, uin
  * one-factor draw).  Call after portfolio_init / portfolio_update_data. */
 int nmlmc_portfolio_set_correlation(const double* chol, uint32_t d);
 
+/* Level-coupled inner paths for the options of method 2 (Unbiased) of the
+ * portfolios staged after this call (portfolio_init, nmlmc_batch_stage; no
+ * counterpart in the reference): instead of the Unbiased method's randomised
+ * level, such an option's inner sample at MLMC level l runs the Milstein paths
+ * of sample_inner_approx (portfolio_sampler.cpp:539-604) with
+ * dt_h0inv 2^floor(l/2) steps over (tau, T] - the batched-risk workload's
+ * arithmetic-average Asians.  An antithetic final inner sample of a level
+ * l > 0 gives the fine estimator its value on those paths and the coarse
+ * estimator its value on dt_h0inv 2^floor((l-1)/2) steps of the same
+ * increments; refinement tests use their level's step count.  Batched and
+ * mixed forms; sub-sampling mode 1. */
+int nmlmc_set_level_coupled_paths(int on);
+
 /* One level call with a status.  outer_offset (batched forms) selects the outer
  * sample ids [outer_offset, outer_offset + M); pass UINT64_MAX to continue after
  * the previous call of this level.  Optional per-outer-sample trace arrays of
@@ -216,6 +229,18 @@ int nmlmc_sample_levels(uint32_t n, const uint32_t* ells, const uint64_t* Ms, sa
 int nmlmc_batch_begin(uint32_t n);
 int nmlmc_batch_stage(uint32_t i, const portfolio_data* data, uint64_t seed, const double* chol, uint32_t d);
 int nmlmc_batch_sample(uint32_t ell, uint32_t n, const uint64_t* M, sample_output* outs);
+
+/* model::sampler (model_sampler.cpp:42-92) written as a class of the
+ * reference's sampler concept and run through the C++ template entry points
+ * nmlmc::init_samplers<SAMPLER> / nmlmc::sample<SAMPLER> of
+ * include/nmlmc_sampler.hpp (the replacement of the reference's
+ * init_samplers / sample<SAMPLER>, nested_sampler.hpp:618-694): the same
+ * arguments as model_init / model_sample (model_sampler.cpp:100-114), the
+ * batched FP64 form's keying and results.  Staging it replaces the engine's
+ * problem; nmlmc_sample and model_sample then refuse until a problem is
+ * staged again. */
+int nmlmc_sampler_model_init(const model_data* data, uint64_t seed);
+int nmlmc_sampler_model_sample(uint32_t ell, uint64_t M, sample_output* out);
 
 /* Mixed form diagnostics of the last call: outer samples re-evaluated in FP64,
  * indicator flips (FP32 decision differing from the FP64 one), and refinement

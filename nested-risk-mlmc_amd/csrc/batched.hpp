@@ -16,6 +16,7 @@
 //
 // Policy interface (see model_batched.hip, portfolio_batched.hip):
 //   P::spu                      inner samples per unit
+//   P::kTwoValued               final draws of levels > 0 give SegAcc::s1c too
 //   P::init_item(A, m, item)    outer draw, start value (init_inner_sampler)
 //   P::units<F32>(A, item_idx, item, tag, u0, u1, i0, i1, acc)
 //                               accumulate units [u0, u1) of draw group
@@ -62,7 +63,7 @@ struct TestEntry {
 };
 
 struct FinalEntry {
-    double s1, sab;
+    double s1, sab, s1c;  // s1c: coarse-discretisation sum (two-valued policies)
     uint32_t item, group, done, kind;  // kind: 0 antithetic / fine-only, 1 plain fine, 2 plain coarse
 };
 
@@ -82,7 +83,12 @@ struct BatchedCtl {
 // Per-lane accumulation of one segment.
 struct SegAcc {
     double s1 = 0., s2 = 0., sab = 0.;
-    double smm = 0.;  // mixed: sum of the FP32 terms' squared error scales (refinement decisions)
+    double smm = 0.;  // mixed: second-moment error scales of the FP32 terms (P::decision_bounds)
+    // Two-valued policies (P::kTwoValued): the final draws' coarse values.  With
+    // level-coupled inner paths an antithetic final inner sample has a value on
+    // the level's discretisation (summed in s1) and one on the level below's,
+    // from the same increments (summed here); the coarse estimator takes these.
+    double s1c = 0.;
     double work = 0.;
     uint64_t nu = 0, nn = 0;  // uniforms and normals drawn (inner_stats[4], [5])
 };

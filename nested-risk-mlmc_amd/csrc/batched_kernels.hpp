@@ -114,10 +114,13 @@ __global__ __launch_bounds__(256) void k_b_init(typename P::Args A) {
 // batched FP64 form's).  With |s1 - s1_64| <= e1 and |s2 - s2_64| <= e2
 // (P::decision_bounds, plus 2^-21 s2 for the compensated sums' own rounding),
 // |dist - dist_64| <= e1 / n and |var - var_64| <= ev; both are doubled to
-// leave room for the FP64 side's rounding.  delta^2_64 then lies in [qlo, qhi]
-// and the FP64 count new_64 (decreasing in delta^2) in [nlo, nhi]; the
-// decision new_64 <= cur is certain unless that interval straddles cur.  The
-// uncertain ones go to the FP64 recheck (k_b_test with RECHECK).
+// leave room for the FP64 side's rounding.  Bounds above a tenth of dist or
+// var leave the decision to the recheck.  Otherwise delta^2_64 is within 40%
+// of delta^2, so the FP64 count new_64 (decreasing in delta^2, as its power -1
+// or -r/2, r < 2) is within a factor 1.7 of new: a decision with new at least
+// a factor 2 from cur is certain.  Nearer ones take the exact interval
+// [qlo, qhi] of delta^2_64 and are uncertain when [nlo, nhi] straddles cur.
+// The uncertain ones go to the FP64 recheck (k_b_test with RECHECK).
 template <class P, int RULE>
 __device__ __forceinline__ bool decision_uncertain(const BatchedCommon& c, const LevelCounts& lc, double start,
                                                    uint64_t cur, double s1, double s2, double sab, double smm) {
@@ -125,11 +128,15 @@ __device__ __forceinline__ bool decision_uncertain(const BatchedCommon& c, const
     double e1, e2;
     P::decision_bounds(n, start, sab, smm, e1, e2);
     e2 += 0x1p-21 * fabs(s2);
-    const double ad = fabs(start + s1 / n);
     const double var = s2 / (n - 1.) - (s1 * s1) / (n * (n - 1.));
+    const double dist = start + s1 / n;
+    const double ad = fabs(dist);
     const double ed = 2. * e1 / n;
     const double ev = 2. * (e2 + (2. * fabs(s1) + e1) * e1 / n) / (n - 1.);
-    if (!(ad > ed) || !(var > ev)) return true;  // also NaN and infinite bounds
+    if (!(ed < 0.1 * ad) || !(ev < 0.1 * var)) return true;  // also NaN and infinite bounds
+    const double q = dist * dist / var;
+    const double nc = (RULE == 1) ? lc.tmp / q : pow(q, -c.ep.adaptive_r / 2) * lc.tmp;
+    if (nc >= 2. * n || nc <= 0.5 * n) return false;
     const double qlo = (ad - ed) * (ad - ed) / (var + ev), qhi = (ad + ed) * (ad + ed) / (var - ev);
     double nlo, nhi;
     if (RULE == 1) {
@@ -286,6 +293,22 @@ __device__ __forceinline__ uint64_t final_units(const BatchedCommon& c, const It
     return (uint64_t)s.groups * ((s.n + P::spu - 1) / P::spu);
 }
 
+// The sums of an antithetic final group that its two estimators take: the
+// big side's (all draws of the larger count) and the small side's (groups of
+// the smaller count).  A two-valued policy's group gives its fine values in
+// s1 and its coarse values in s1c; the fine side is the big one when
+// N_f > N_c (and at level 0).  Other policies have one value per sample.
+template <class P>
+__device__ __forceinline__ void side_sums(const BatchedCommon& c, const FinalShape& sh, double s1, double s1c,
+                                          double& s_big, double& s_small) {
+    if (!P::kTwoValued || c.ell == 0) {
+        s_big = s_small = s1;
+        return;
+    }
+    s_big = sh.f_big ? s1 : s1c;
+    s_small = sh.f_big ? s1c : s1;
+}
+
 template <class P>
 __global__ __launch_bounds__(256) void k_b_final_count(typename P::Args A) {
     const BatchedCommon& c = A.c;
@@ -368,13 +391,13 @@ __global__ __launch_bounds__(256) void k_b_final_fill(typename P::Args A) {
         __syncthreads();
         if (valid) {
             if (whole) {
-                c.fpool[whole_off + base[kMaxClasses] + rank_a] = FinalEntry{0., 0., (uint32_t)m, 0, 0, 3};
+                c.fpool[whole_off + base[kMaxClasses] + rank_a] = FinalEntry{0., 0., 0., (uint32_t)m, 0, 0, 3};
             } else if (plain2) {
-                c.fpool[c.ctl->class_off[cls_a] + base[cls_a] + rank_a] = FinalEntry{0., 0., (uint32_t)m, 0, 0, 1};
-                c.fpool[c.ctl->class_off[cls_b] + base[cls_b] + rank_b] = FinalEntry{0., 0., (uint32_t)m, 0, 0, 2};
+                c.fpool[c.ctl->class_off[cls_a] + base[cls_a] + rank_a] = FinalEntry{0., 0., 0., (uint32_t)m, 0, 0, 1};
+                c.fpool[c.ctl->class_off[cls_b] + base[cls_b] + rank_b] = FinalEntry{0., 0., 0., (uint32_t)m, 0, 0, 2};
             } else {
                 FinalEntry* dst = c.fpool + c.ctl->class_off[cls_a] + base[cls_a] + rank_a;
-                for (uint32_t g = 0; g < s.groups; g++) dst[g] = FinalEntry{0., 0., (uint32_t)m, g, 0, 0};
+                for (uint32_t g = 0; g < s.groups; g++) dst[g] = FinalEntry{0., 0., 0., (uint32_t)m, g, 0, 0};
             }
         }
         __syncthreads();
@@ -394,15 +417,16 @@ __device__ __forceinline__ void finish_outer(const BatchedCommon& c, uint64_t m,
 
 // phi of an outer sample from its finished final groups (sample_phi /
 // sample_phi_antithetic, nested_sampler.hpp:164-216).
+template <class P>
 __device__ __forceinline__ void outer_phi(const BatchedCommon& c, const ItemState& it, double s1_big,
                                           double phi_acc, double phi_f_na, double phi_c_na,
                                           double& phi_f, double& phi_c) {
     if (c.ell == 0) {
-        phi_f = phi_scalar(c.phi_type, it.start + s1_big / (double)it.nf, 1., 0.);
+        phi_f = P::phi(c, it.start + s1_big / (double)it.nf, 1., 0.);
         phi_c = 0.;
     } else if (c.ep.antithetic) {
         const FinalShape s = final_shape(c, it);
-        const double phi_big = phi_scalar(c.phi_type, it.start + s1_big / (double)s.big, 1., 0.);
+        const double phi_big = P::phi(c, it.start + s1_big / (double)s.big, 1., 0.);
         phi_f = s.f_big ? phi_big : phi_acc;
         phi_c = s.f_big ? phi_acc : phi_big;
     } else {
@@ -454,8 +478,11 @@ __device__ __forceinline__ void redo_finish(const BatchedCommon& c, uint64_t m, 
 // kRedoParts blocks of its grid column: lanes add their groups' FP64 sums into
 // the slot, and the block completing the slot's count forms phi.  The others
 // are summed by one block each in k_b_redo_seq, group after group.
-__device__ __forceinline__ bool redo_split(const BatchedCommon& c, uint32_t w, const FinalShape& sh, uint64_t spu) {
-    return w < kRedoSlots && !(c.ell > 0 && !c.ep.antithetic) && sh.groups <= kRedoGroups && sh.n % spu == 0;
+// Two-valued policies' groups have two sums: they take the sequential redo.
+template <class P>
+__device__ __forceinline__ bool redo_split(const BatchedCommon& c, uint32_t w, const FinalShape& sh) {
+    return !P::kTwoValued && w < kRedoSlots && !(c.ell > 0 && !c.ep.antithetic) && sh.groups <= kRedoGroups &&
+           sh.n % P::spu == 0;
 }
 
 template <class P>
@@ -468,7 +495,7 @@ __global__ __launch_bounds__(256) void k_b_redo_split(typename P::Args A) {
         const uint32_t m = c.reeval[w];
         const ItemState& it = c.items[m];
         const FinalShape sh = final_shape(c, it);
-        if (!redo_split(c, w, sh, P::spu)) continue;
+        if (!redo_split<P>(c, w, sh)) continue;
         const uint64_t gu = sh.n / P::spu;  // units per group
         const uint64_t U = (uint64_t)sh.groups * gu;
         const uint64_t p0 = U * part / parts, p1 = U * (part + 1) / parts;
@@ -497,11 +524,11 @@ __global__ __launch_bounds__(256) void k_b_redo_split(typename P::Args A) {
         double s1_big = 0., phi_small = 0.;
         for (uint32_t g = 0; g < sh.groups; g++) {
             const double s1 = aread(&slot[g]);
-            phi_small = phi_scalar(c.phi_type, it.start + s1 / (double)sh.n, coeff, phi_small);
+            phi_small = P::phi(c, it.start + s1 / (double)sh.n, coeff, phi_small);
             s1_big += s1;
         }
         double phi_f, phi_c;
-        outer_phi(c, it, s1_big, phi_small, 0., 0., phi_f, phi_c);
+        outer_phi<P>(c, it, s1_big, phi_small, 0., 0., phi_f, phi_c);
         redo_finish(c, m, it, phi_f, phi_c);
     }
 }
@@ -509,38 +536,46 @@ __global__ __launch_bounds__(256) void k_b_redo_split(typename P::Args A) {
 template <class P>
 __global__ __launch_bounds__(256) void k_b_redo_seq(typename P::Args A) {
     const BatchedCommon& c = A.c;
-    __shared__ double red[4];
+    __shared__ double red[8];
     const uint32_t n = aload_u32(&c.ctl->nreeval);
     for (uint32_t w = blockIdx.x; w < n; w += gridDim.x) {
         const uint32_t m = c.reeval[w];
         const ItemState& it = c.items[m];
         const FinalShape sh = final_shape(c, it);
-        if (redo_split(c, w, sh, P::spu)) continue;
-        auto group_sum64 = [&](uint32_t tag, uint64_t i0, uint64_t i1) {
+        if (redo_split<P>(c, w, sh)) continue;
+        // FP64 sums of a draw group, block-wide: s1 and (two-valued) s1c.
+        auto group_sum64 = [&](uint32_t tag, uint64_t i0, uint64_t i1, double* s1c_out) {
             SegAcc s;
             const uint64_t ub = i0 / P::spu, ue = (i1 + P::spu - 1) / P::spu;
             for (uint64_t u = ub + threadIdx.x; u < ue; u += blockDim.x)
                 P::template units<false>(A, m, it, tag, u, u + 1, i0, i1, s);
             const double v = wave_sum(s.s1);
-            if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+            const double vc = P::kTwoValued ? wave_sum(s.s1c) : 0.;
+            if ((threadIdx.x & 63) == 0) {
+                red[threadIdx.x >> 6] = v;
+                red[4 + (threadIdx.x >> 6)] = vc;
+            }
             __syncthreads();
             const double t = ((red[0] + red[1]) + red[2]) + red[3];
+            if (s1c_out) *s1c_out = P::kTwoValued ? ((red[4] + red[5]) + red[6]) + red[7] : t;
             __syncthreads();
             return t;
         };
         double phi_f = 0., phi_c = 0.;
         if (c.ell > 0 && !c.ep.antithetic) {
-            phi_f = phi_scalar(c.phi_type, it.start + group_sum64(TAG_FINAL, 0, it.nf) / (double)it.nf, 1., 0.);
-            phi_c = phi_scalar(c.phi_type, it.start + group_sum64(TAG_FINAL_COARSE, 0, it.nc) / (double)it.nc, 1., 0.);
+            phi_f = P::phi(c, it.start + group_sum64(TAG_FINAL, 0, it.nf, nullptr) / (double)it.nf, 1., 0.);
+            phi_c = P::phi(c, it.start + group_sum64(TAG_FINAL_COARSE, 0, it.nc, nullptr) / (double)it.nc, 1., 0.);
         } else {
             const double coeff = (double)(1.f / (float)sh.groups);
             double s1_big = 0., phi_small = 0.;
             for (uint32_t g = 0; g < sh.groups; g++) {
-                const double s1 = group_sum64(TAG_FINAL, (uint64_t)g * sh.n, (uint64_t)(g + 1) * sh.n);
-                phi_small = phi_scalar(c.phi_type, it.start + s1 / (double)sh.n, coeff, phi_small);
-                s1_big += s1;
+                double s1c, sb, ss;
+                const double s1 = group_sum64(TAG_FINAL, (uint64_t)g * sh.n, (uint64_t)(g + 1) * sh.n, &s1c);
+                side_sums<P>(c, sh, s1, s1c, sb, ss);
+                phi_small = P::phi(c, it.start + ss / (double)sh.n, coeff, phi_small);
+                s1_big += sb;
             }
-            outer_phi(c, it, s1_big, phi_small, 0., 0., phi_f, phi_c);
+            outer_phi<P>(c, it, s1_big, phi_small, 0., 0., phi_f, phi_c);
         }
         if (threadIdx.x == 0) redo_finish(c, m, it, phi_f, phi_c);
     }
@@ -566,7 +601,7 @@ __device__ __forceinline__ bool whole_final(const typename P::Args& A, uint32_t 
         const double mf = it.start + a.s1 / (double)it.nf, mc = it.start + b.s1 / (double)it.nc;
         near = MIXED && (fabs(mf) <= P::mixed_bound(a.sab / (double)it.nf, it.start) ||
                          fabs(mc) <= P::mixed_bound(b.sab / (double)it.nc, it.start));
-        outer_phi(c, it, 0., 0., phi_scalar(c.phi_type, mf, 1., 0.), phi_scalar(c.phi_type, mc, 1., 0.), phi_f,
+        outer_phi<P>(c, it, 0., 0., P::phi(c, mf, 1., 0.), P::phi(c, mc, 1., 0.), phi_f,
                   phi_c);
     } else {
         const double coeff = (double)(1.f / (float)sh.groups);
@@ -576,15 +611,17 @@ __device__ __forceinline__ bool whole_final(const typename P::Args& A, uint32_t 
             acc.work(it.inst, a.work + P::group_work(A, it));
             acc.st(it.inst, 4, a.nu);
             acc.st(it.inst, 5, a.nn);
-            const double mean = it.start + a.s1 / (double)sh.n;
+            double sb, ss;
+            side_sums<P>(c, sh, a.s1, a.s1c, sb, ss);
+            const double mean = it.start + ss / (double)sh.n;
             if (MIXED) near |= fabs(mean) <= P::mixed_bound(a.sab / (double)sh.n, it.start);
-            phi_small = phi_scalar(c.phi_type, mean, coeff, phi_small);
-            s1_big += a.s1;
+            phi_small = P::phi(c, mean, coeff, phi_small);
+            s1_big += sb;
             sab_big += a.sab;
         }
         if (MIXED)
             near |= fabs(it.start + s1_big / (double)sh.big) <= P::mixed_bound(sab_big / (double)sh.big, it.start);
-        outer_phi(c, it, s1_big, phi_small, 0., 0., phi_f, phi_c);
+        outer_phi<P>(c, it, s1_big, phi_small, 0., 0., phi_f, phi_c);
     }
     return MIXED && near && c.phi_type == PHI_INDICATOR;
 }
@@ -673,18 +710,22 @@ __global__ __launch_bounds__(256, (mixed_waves<P, MIXED>())) void k_b_final(type
             acc.st(it.inst, 5, s.nn);
         }
         bool complete = split && spe == 1;
-        double s1 = s.s1, sab = s.sab;
+        double s1 = s.s1, sab = s.sab, s1c = s.s1c;
         if (__ballot(split && spe > 1)) {  // wave-uniform
             FinalEntry& fe = c.fpool[fidx];
-            const RunSums rs = run_sums(split ? fidx : ((1ull << 63) | k), s.s1, s.sab);
+            const unsigned long long key = split ? fidx : ((1ull << 63) | k);
+            const RunSums rs = run_sums(key, s.s1, s.sab);
+            const double rc = P::kTwoValued ? run_sums(key, s.s1c, 0.).a : 0.;
             if (split && spe > 1 && rs.tail) {
                 aadd(&fe.s1, rs.a);
                 if (MIXED) aadd(&fe.sab, rs.b);
+                if (P::kTwoValued) aadd(&fe.s1c, rc);
                 wait_acks();
                 if (acount(&fe.done, rs.n) == spe - rs.n) {
                     complete = true;
                     s1 = aread(&fe.s1);
                     sab = MIXED ? aread(&fe.sab) : 0.;
+                    s1c = P::kTwoValued ? aread(&fe.s1c) : 0.;
                 }
             }
         }
@@ -693,28 +734,29 @@ __global__ __launch_bounds__(256, (mixed_waves<P, MIXED>())) void k_b_final(type
             const uint32_t m = fe.item;
             ItemState& it = c.items[m];
             acc.work(it.inst, P::group_work(A, it));
-            const double mean = it.start + s1 / (double)n_cls;
+            const FinalShape sh = final_shape(c, it);
+            double s_big = s1, s_small = s1;
+            if (fe.kind == 0) side_sums<P>(c, sh, s1, s1c, s_big, s_small);
+            const double mean = it.start + s_small / (double)n_cls;
             const bool near = MIXED && fabs(mean) <= P::mixed_bound(sab / (double)n_cls, it.start);
             if (near) __hip_atomic_fetch_or(&it.near, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (fe.kind == 0) {
-                const FinalShape sh = final_shape(c, it);
                 const double coeff = (double)(1.f / (float)sh.groups);
-                if (c.ell > 0) aadd(&it.phi_acc, phi_scalar(c.phi_type, mean, coeff, 0.));
-                aadd(&it.s1_big, s1);
+                if (c.ell > 0) aadd(&it.phi_acc, P::phi(c, mean, coeff, 0.));
+                aadd(&it.s1_big, s_big);
                 if (MIXED) aadd(&it.sab_big, sab);
             } else if (fe.kind == 1) {
-                astore(&it.phi_f, phi_scalar(c.phi_type, mean, 1., 0.));
+                astore(&it.phi_f, P::phi(c, mean, 1., 0.));
             } else {
-                astore(&it.phi_c, phi_scalar(c.phi_type, mean, 1., 0.));
+                astore(&it.phi_c, P::phi(c, mean, 1., 0.));
             }
             wait_acks();
             if (__hip_atomic_fetch_sub(&it.left, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 1u) {
                 const double s1b = aread(&it.s1_big), pacc = aread(&it.phi_acc);
                 double phi_f, phi_c;
-                outer_phi(c, it, s1b, pacc, aread(&it.phi_f), aread(&it.phi_c), phi_f, phi_c);
+                outer_phi<P>(c, it, s1b, pacc, aread(&it.phi_f), aread(&it.phi_c), phi_f, phi_c);
                 bool is_near = MIXED && aread_u32(&it.near) != 0;
                 if (MIXED && fe.kind == 0) {
-                    const FinalShape sh = final_shape(c, it);
                     const double mb = it.start + s1b / (double)sh.big;
                     is_near |= fabs(mb) <= P::mixed_bound(aread(&it.sab_big) / (double)sh.big, it.start);
                 }

@@ -198,6 +198,8 @@ int Engine::sample(uint32_t ell, uint64_t M, uint64_t outer_offset, uint32_t lan
                    uint64_t* tr_nc) {
     if (!inited) return set_error(NMLMC_ERR_STATE, "init_nestedmc_sampler has not been called");
     if (problem < 0) return set_error(NMLMC_ERR_STATE, "no problem staged (model_init / portfolio_init)");
+    if (problem == PROBLEM_SAMPLER_CLASS)
+        return set_error(NMLMC_ERR_STATE, "a sampler class is staged: sample it with nmlmc::sample<SAMPLER>");
     if (!out) return set_error(NMLMC_ERR_ARG, "sample_output is NULL");
     if (ell > 40) return set_error(NMLMC_ERR_ARG, "level %u out of range", ell);
     out->M = M;
@@ -248,8 +250,16 @@ int Engine::trace_finish(uint64_t M, TraceBufs& tb, double* tr_phi_f, double* tr
     return NMLMC_OK;
 }
 
-int Engine::finish_call(uint32_t ell, sample_output* out) {
-    hipError_t e = hipMemcpyAsync(h_out, d_out, sizeof(Partial), hipMemcpyDeviceToHost, stream);
+int Engine::plain_buffers(size_t n, Partial*& partials, Partial*& result) {
+    const int rc = ensure_partials(n);
+    if (rc) return rc;
+    partials = d_partials;
+    result = d_out;
+    return NMLMC_OK;
+}
+
+int Engine::finish_call(uint32_t ell, sample_output* out, const Partial* result) {
+    hipError_t e = hipMemcpyAsync(h_out, result, sizeof(Partial), hipMemcpyDeviceToHost, stream);
     if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync result");
     e = hipStreamSynchronize(stream);
     if (e != hipSuccess) return hip_fail(e, "kernel execution");
@@ -278,10 +288,11 @@ int Engine::sample_model_parity(uint32_t ell, uint64_t M, uint32_t lanes_overrid
     if (const char* s = getenv("NMLMC_GROUP_LANES")) H.G = pow2_ceil_u32(strtoul(s, nullptr, 10));
     last_G = H.G;
     const uint64_t nblocks = parity_blocks(cores, H.G);
-    int rc = ensure_partials(nblocks);
+    Partial *parts, *res;
+    int rc = plain_buffers(nblocks, parts, res);
     if (rc) return rc;
     H.states = d_states;
-    H.partials = d_partials;
+    H.partials = parts;
     TraceBufs tb;
     rc = trace_alloc(M, tr_phi_f || tr_phi_c || tr_nf || tr_nc, tb);
     if (rc) return rc;
@@ -294,10 +305,10 @@ int Engine::sample_model_parity(uint32_t ell, uint64_t M, uint32_t lanes_overrid
     e = launch_model_parity(H, stream);
     if (e != hipSuccess) return hip_fail(e, "launch k_model_parity");
     if ((e = hipEventRecord(ev1, stream)) != hipSuccess) return hip_fail(e, "hipEventRecord");
-    e = launch_reduce_partials(d_partials, (uint32_t)nblocks, d_out, stream);
+    e = launch_reduce_partials(parts, (uint32_t)nblocks, res, stream);
     if (e != hipSuccess) return hip_fail(e, "launch k_reduce_partials");
     last_kernel = "k_model_parity";
-    rc = finish_call(ell, out);
+    rc = finish_call(ell, out, res);
     if (rc) return rc;
     rc = trace_finish(M, tb, tr_phi_f, tr_phi_c, tr_nf, tr_nc);
     if (rc) return rc;
@@ -825,6 +836,7 @@ void init_nestedmc_sampler(uint32_t cores, const nested_sampler_params* params) 
     g_engine.ep.adaptive_r = params->adaptive_r;
     g_engine.ep.adapt_rule = 0;
     g_engine.ep.cap_log2 = 0;
+    g_engine.level_coupled = false;
     g_engine.problem = -1;
     g_engine.inited = true;
     int rc = g_engine.ensure_device();

@@ -17,7 +17,9 @@
 
 namespace nmlmc {
 
-enum { PROBLEM_MODEL = 0, PROBLEM_PORTFOLIO = 1 };
+// PROBLEM_SAMPLER_CLASS: a sampler class of the reference's concept staged by
+// init_samplers<S> (include/nmlmc_sampler.hpp), sampled only through sample<S>.
+enum { PROBLEM_MODEL = 0, PROBLEM_PORTFOLIO = 1, PROBLEM_SAMPLER_CLASS = 2 };
 
 struct TraceBufs {
     double* phi_f = nullptr;
@@ -86,7 +88,9 @@ struct Engine {
     int trace_alloc(uint64_t M, bool want, TraceBufs& tb);
     int trace_finish(uint64_t M, TraceBufs& tb, double* tr_phi_f, double* tr_phi_c,
                      uint64_t* tr_nf, uint64_t* tr_nc);
-    int finish_call(uint32_t ell, sample_output* out);
+    int finish_call(uint32_t ell, sample_output* out, const Partial* result);
+    // The plain form's partial records for n blocks and the reduced record.
+    int plain_buffers(size_t n, Partial*& partials, Partial*& result);
     int sample(uint32_t ell, uint64_t M, uint64_t outer_offset, uint32_t lanes_override,
                sample_output* out, double* tr_phi_f, double* tr_phi_c, uint64_t* tr_nf,
                uint64_t* tr_nc);
@@ -106,6 +110,9 @@ struct Engine {
         uint32_t S = 0, bpc = 0;
     };
     std::map<uint64_t, Shape> tuned;  // autotune_level's choices, key (form << 32) | level
+    // Level-coupled paths for the method-2 options of portfolios staged from
+    // now on (nmlmc_set_level_coupled_paths).
+    bool level_coupled = false;
     Shape force{};                    // the candidate autotune_level is timing
     Shape batched_shape(uint32_t ell, uint64_t M, uint64_t spu, const BatchedCommon& c, uint32_t lanes_override);
     int autotune_level(uint32_t ell, uint64_t M, Shape& best, double& best_ms);

@@ -22,9 +22,15 @@ struct ModelPolicy {
     static constexpr bool kMulti = false;
     static constexpr bool kFuse = true;  // k_b_fused is instantiated
     static constexpr bool kMixedWaves = true;
+    static constexpr bool kTwoValued = false;
 
     // init_inner_sampler does no work for the model (model_sampler.cpp:64).
     __device__ static double group_work(const Args&, const ItemState&) { return 0.; }
+
+    // add_phi (model_sampler.cpp:81-91): coeff times the Indicator or Max of u, added to acc.
+    __device__ static double phi(const BatchedCommon& c, double u, double coeff, double acc) {
+        return phi_scalar(c.phi_type, u, coeff, acc);
+    }
 
     // Mixed form: an FP32 mean within this distance of the threshold is
     // re-evaluated in FP64 (mean_mag: mean of the inner terms' magnitudes).

@@ -75,6 +75,11 @@ struct PortfolioDevice {
     // ln(beta) e, and the weights beta^(-e L) / (1 - beta^e) of L < kUnbLevels.
     double unb_lnbe;
     double unb_weight[kUnbLevels];
+    // Level-coupled paths (batched forms, the batched-risk workload's Asians):
+    // the options of method 2 are priced on Milstein paths of
+    // dt_h0inv 2^floor(l / 2) steps at level l, fine and coarse from the same
+    // increments, instead of the Unbiased method's randomised level.
+    uint32_t coupled;
     int phi_type;
     uint32_t phi_size;
     int subsample;

@@ -38,6 +38,9 @@ struct PortfolioPolicyT {
     // The general inner loop (FP64 Unbiased paths beside the FP32 Exact terms)
     // runs without the mixed form's register cap.
     static constexpr bool kMixedWaves = EXACT_ONLY;
+    // The general books' level-coupled inner paths (PortfolioDevice::coupled)
+    // give the final draws a coarse value too.
+    static constexpr bool kTwoValued = !EXACT_ONLY;
 
     __device__ static const PortfolioDevice& pf_of(const Args& A, const ItemState& it) {
         return MULTI ? A.bt.inst[it.inst] : A.pf;
@@ -47,6 +50,11 @@ struct PortfolioPolicyT {
     }
     __device__ static double group_work(const Args& A, const ItemState& it) {
         return MULTI ? A.bt.inst_bs_work[it.inst] : A.bs_work;
+    }
+    // add_phi (portfolio_sampler.cpp:261-291) for the scalar phi types of the
+    // batched forms (Indicator, Max).
+    __device__ static double phi(const BatchedCommon& c, double u, double coeff, double acc) {
+        return phi_scalar(c.phi_type, u, coeff, acc);
     }
     // MULTI: the segment holding item m (largest k with seg_item0[k] <= m).
     __device__ static uint32_t segment(const Args& A, uint64_t m) {
@@ -70,13 +78,16 @@ struct PortfolioPolicyT {
     __device__ static double mixed_bound(double mean_mag, double) { return 0x1p-18 * mean_mag; }
 
     // Mixed form, refinement decisions: bounds on |s1 - s1_64| and |s2 - s2_64|
-    // of n terms with error scales summing to sab (squares: smm).  An FP32 term
-    // d_i is within e_i = 2^-18 mag_i of its FP64 value and |d_i| <= mag_i, so
-    // |d_i^2 - d64_i^2| <= (2 + 2^-18) 2^-18 mag_i^2; the FP64 terms' sums differ
-    // only in order (covered by sab's 2^-16 share and by the caller's 2^-21 s2).
+    // of n terms with error scales summing to sab; smm sums mag_i (|d_i| +
+    // 2^-19 mag_i).  An FP32 term d_i is within e_i = 2^-18 mag_i of its FP64
+    // value, so |d_i^2 - d64_i^2| <= 2 e_i |d_i| + e_i^2
+    // = 2^-17 mag_i (|d_i| + 2^-19 mag_i).  (mag_i, the scale of the stock
+    // levels behind the term, is some hundred times |d_i|: the bound on s2 uses
+    // |d_i| to stay relative to the variance.)  The FP64 terms' sums differ
+    // only in order (covered by sab's 2^-16 share and the caller's 2^-21 s2).
     __device__ static void decision_bounds(double n, double, double sab, double smm, double& e1, double& e2) {
         e1 = 0x1p-18 * sab;
-        e2 = 0x1p-16 * smm;
+        e2 = 0x1p-17 * smm;
     }
 
     __device__ static uint64_t key1(const Args& A, uint64_t m) {
@@ -231,6 +242,7 @@ struct PortfolioPolicyT {
         const StridedR R = stocks_at_tau(A, m);
         const uint64_t k1 = key1(A, m);
         double a1 = 0., a2 = 0., ab = 0., work = 0.;
+        double dc = 0.;  // level-coupled finals: sum of (coarse - fine) values
         Kahan b1, b2, bab;
         float mm = 0.f;
         bool redo64 = false;  // a term at a payoff discontinuity (mixed form)
@@ -250,7 +262,7 @@ struct PortfolioPolicyT {
                 b1.add(d);
                 b2.add(d * d);
                 bab.add(mag);
-                m2 += mag * mag;
+                m2 += mag * (fabsf(d) + 0x1p-19f * mag);
             };
             uint64_t u = u0;
             for (; u + 1 < u1; u += 2) {
@@ -319,6 +331,16 @@ struct PortfolioPolicyT {
                 a2 += v * v;
                 ab += fabs(v);
             };
+            // Level-coupled paths (pf.coupled): the draw group's discretisation.
+            // Fine tests use the level's step count, coarse tests and the
+            // plain estimator's coarse final the level below's; the antithetic
+            // final of a level > 0 takes both, on the same increments, and its
+            // coarse values go to s1c (dc: their difference from the fine ones).
+            const uint32_t ell_g = (tag >= TAG_COARSE_TEST && tag < TAG_FINAL) || tag == TAG_FINAL_COARSE
+                                       ? (A.c.ell ? A.c.ell - 1 : 0) : A.c.ell;
+            const uint64_t n_cpl = coupled_steps(pf, ell_g);
+            const bool two = tag == TAG_FINAL && A.c.ell > 0 && A.c.ep.antithetic;
+            const uint32_t cs = two ? (uint32_t)(n_cpl / coupled_steps(pf, A.c.ell - 1)) : 1u;
             for (uint64_t w0 = u0; w0 < u1; w0 += 64) {
                 const uint32_t nw = (uint32_t)((u1 - w0) < 64 ? (u1 - w0) : 64);
                 uint64_t lv0 = 0, lv1 = 0, lvw = 0;
@@ -327,6 +349,10 @@ struct PortfolioPolicyT {
                     st.init(seed, k1, ((uint64_t)tag << 40) | (w0 + j), 0, 0., 0);
                     uint32_t method, idx;
                     const double prob = pick_option(pf, st, st, method, idx);
+                    if (method == 2 && pf.coupled) {
+                        lv0 |= 1ull << j;  // a pass of their own, below
+                        continue;
+                    }
                     if (method == 2) {
                         const uint32_t L = unbiased_level(pf, st);
                         const uint64_t bit = 1ull << j;
@@ -364,7 +390,7 @@ struct PortfolioPolicyT {
                         b1.add(d);
                         b2.add(d * d);
                         bab.add(mag);
-                        mm += mag * mag;
+                        mm += mag * (fabsf(d) + 0x1p-19f * mag);
                     } else if (method == 0) {
                         add64(inner_bs_d(pf, o, R, work) / prob);
                     } else {
@@ -375,7 +401,8 @@ struct PortfolioPolicyT {
                     s.nu += st.nu;
                     s.nn += st.nn;
                 }
-                // Unbiased samples of level 0, then level 1, in this lane.
+                // Unbiased samples of level 0, then level 1, in this lane (or
+                // the level-coupled samples, in one pass).
                 for (int pass = 0; pass < 2; pass++) {
                     uint64_t mask = pass == 0 ? lv0 : lv1;
                     while (mask) {
@@ -385,7 +412,17 @@ struct PortfolioPolicyT {
                         st.init(seed, k1, ((uint64_t)tag << 40) | (w0 + j), 0, 0., 0);
                         uint32_t method, idx;
                         const double prob = pick_option(pf, st, st, method, idx);
-                        add64(inner_unbiased_d(pf, pf.options[2][idx], R, st, st, work) / prob);
+                        if (pf.coupled) {
+                            double vc;
+                            const double vf =
+                                inner_coupled_d(pf, pf.options[2][idx], R, st, n_cpl, cs, two, work, vc) / prob;
+                            vc /= prob;
+                            add64(vf);
+                            dc += vc - vf;
+                            ab += fabs(vc - vf);
+                        } else {
+                            add64(inner_unbiased_d(pf, pf.options[2][idx], R, st, st, work) / prob);
+                        }
                         s.nu += st.nu;
                         s.nn += st.nn;
                     }
@@ -405,6 +442,7 @@ struct PortfolioPolicyT {
             }
         }
         s.s1 += a1 + (F32 ? b1.value() : 0.);
+        s.s1c += a1 + dc + (F32 ? b1.value() : 0.);
         s.s2 += a2 + (F32 ? b2.value() : 0.);
         s.sab += F32 ? (redo64 ? (double)INFINITY : bab.value() + 0x1p-16 * ab) : 0.;
         s.smm += F32 ? (double)mm : 0.;

@@ -13,69 +13,6 @@
 
 namespace nmlmc {
 
-// A random_state (par_omp.hpp:72-121) positioned on a keyed counter sequence:
-// block b of the stream is threefry({b, c1}, {k0, k1}); word q is element
-// 1 - (q & 1) of block q >> 1.  c1 = 0 for the reference keying's per-thread
-// streams; c1 = (tag << 40) | unit for a batched inner unit.
-struct DStream {
-    uint64_t k0, k1, c1, P;
-    double saved;
-    uint32_t is_saved;
-    uint64_t blk;
-    u64x2 cur;
-    uint64_t nu, nn;  // randu / randn calls (inner_stats[4], [5])
-
-    __device__ void init(uint64_t a, uint64_t b, uint64_t c, uint64_t p, double sv, uint32_t iss) {
-        k0 = a;
-        k1 = b;
-        c1 = c;
-        P = p;
-        saved = sv;
-        is_saved = iss;
-        blk = ~0ull;
-        nu = nn = 0;
-    }
-    __device__ uint64_t rand() {
-        const uint64_t q = P++;
-        const uint64_t b = q >> 1;
-        if (b != blk) {
-            cur = threefry2x64_20(b, c1, k0, k1);
-            blk = b;
-        }
-        return (q & 1) ? cur.a : cur.b;
-    }
-    __device__ double randn() {
-        nn++;
-        if (is_saved) {
-            is_saved = 0;
-            return saved;
-        }
-        const uint64_t first = rand();
-        const uint64_t second = rand();
-        double x, y;
-        boxmuller_f64(second, first, x, y);
-        is_saved = 1;
-        saved = y;
-        return x;
-    }
-    __device__ double randu() {
-        nu++;
-        return u01_f64(rand());
-    }
-    // Move past the next k normals of a normal-only run of draws (the cached
-    // one first, then Box-Muller pairs): the stream is then where k randn()
-    // calls leave it, its last pair computed when k ends inside one.
-    __device__ void skip_normals(uint64_t k) {
-        if (k == 0) return;
-        if (is_saved) {
-            is_saved = 0;
-            k--;
-        }
-        P += 2 * (k >> 1);
-        if (k & 1) (void)randn();
-    }
-};
-
 constexpr double kSqrt2 = 1.414213562373095;
 constexpr double kSqrt2Pi = 2.506628274631;
 
@@ -338,6 +275,58 @@ __device__ __forceinline__ double inner_unbiased_d(const PortfolioDevice& pf, co
         else simulate_dt_d<false>(inner, p.S, p.A, p.N, pf.r, st.sigma, T_tau, false, pf.use_milstein, 2);
     }
     return unbiased_end(pf, o, R_tau, p);
+}
+
+// Steps of the level-coupled paths at level l: dt_h0inv 2^floor(l / 2).
+__device__ __forceinline__ uint64_t coupled_steps(const PortfolioDevice& pf, uint32_t ell) {
+    return (uint64_t)pf.dt_h0inv << (ell / 2);
+}
+
+// A level-coupled inner sample of a method-2 option: sample_inner_approx's paths
+// (539-604) with a step count set by the level instead of drawn.  The S0 path
+// and its antithetic twin go exactly to tau (one normal), then the three paths
+// (S0, R_tau, antithetic S0) run n_f Milstein steps over (tau, T]; with `two`,
+// also coarse paths of n_f / cs steps on the same increments (cs = 1: the
+// coarse paths are the fine ones).  Each value: the antithetic mean of
+// w e^{-rT} (payoff(S0 path) - payoff(R_tau path)), on the paths' averages for
+// Asians, plus the delta control variate's pathwise term on its own path.
+// Returns the fine value, the coarse one in v_c (the fine one without `two`);
+// the work is the steps' share of W_est (n / dt_h0inv per path set), as the
+// Unbiased method counts it.
+template <class RT>
+__device__ __forceinline__ double inner_coupled_d(const PortfolioDevice& pf, const OptionRec& o, RT R_tau,
+                                                  DStream& inner, uint64_t n_f, uint32_t cs, bool two,
+                                                  double& work, double& v_c) {
+    const uint32_t si = o.stock_ind;
+    const StockRec st = pf.stocks[si];
+    const double dW_0_tau = pf.stau * inner.randn();
+    const double S0_t_1 = simulate_stock_d(st.S0, pf.r, st.sigma, pf.tau, dW_0_tau);
+    const double S0_t_2 = simulate_stock_d(st.S0, pf.r, st.sigma, pf.tau, -dW_0_tau);
+    const bool coarse = two && cs > 1;
+    work += (double)(n_f / pf.dt_h0inv) * o.W_est;
+    if (coarse) work += (double)(n_f / cs / pf.dt_h0inv) * o.W_est;
+    const double Rt = R_tau[si];
+    double S[6] = {S0_t_1, Rt, S0_t_2, S0_t_1, Rt, S0_t_2};
+    double A[6] = {0., 0., 0., 0., 0., 0.};
+    const double T_tau = o.maturity - pf.tau;
+    const bool asian = is_asian(o.type);
+    if (asian) simulate_dt_d<true>(inner, S, A, n_f, pf.r, st.sigma, T_tau, coarse, pf.use_milstein, cs);
+    else simulate_dt_d<false>(inner, S, A, n_f, pf.r, st.sigma, T_tau, coarse, pf.use_milstein, cs);
+    const double* V = asian ? A : S;
+    const double m = pf.use_antithetic_risk ? 0.5 : 1.;
+    const double diff = Rt - st.S0;
+    double val_f = 0., val_c = 0., val_d_f = 0., val_d_c = 0.;
+    for (int i = 0; i < (pf.use_antithetic_risk ? 2 : 1); i++) {
+        const int ic = 2 * i;
+        val_f += m * diff_from_stock_d(o, V[ic], V[1], pf.r);
+        if (pf.use_delta_cv) val_d_f += m * derv_from_stock_d(o, S[ic], st.S0, pf.r);
+        if (!coarse) continue;
+        val_c += m * diff_from_stock_d(o, V[ic + 3], V[4], pf.r);
+        if (pf.use_delta_cv) val_d_c += m * derv_from_stock_d(o, S[ic + 3], st.S0, pf.r);
+    }
+    const double vf = val_f + diff * val_d_f;
+    v_c = coarse ? val_c + diff * val_d_c : vf;
+    return vf;
 }
 
 // portfolio::sampler::add_phi (portfolio_sampler.cpp:261-291).

@@ -181,6 +181,9 @@ int Engine::stage_portfolio(const portfolio_data* data) {
     P.use_milstein = data->use_milstein;
     if (data->options[2].size && (data->dt_beta < 2 || !(data->dt_prob_expo < 0) || data->dt_h0inv == 0))
         return set_error(NMLMC_ERR_ARG, "Unbiased options need dt_beta >= 2, dt_prob_expo < 0, dt_h0inv > 0");
+    P.coupled = (level_coupled && data->options[2].size) ? 1u : 0u;
+    if (P.coupled && data->subsample != 1)
+        return set_error(NMLMC_ERR_ARG, "level-coupled paths need sub-sampling mode 1");
     // FP32 records of the Exact options and of the stocks for the mixed form.
     {
         const option_params* ex = (const option_params*)data->options[1].data;
@@ -236,6 +239,8 @@ int Engine::stage_portfolio(const portfolio_data* data) {
 
 int Engine::sample_portfolio_parity(uint32_t ell, uint64_t M, uint32_t, sample_output* out,
                                     double* tr_phi_f, double* tr_phi_c, uint64_t* tr_nf, uint64_t* tr_nc) {
+    if (pf.coupled)
+        return set_error(NMLMC_ERR_STATE, "level-coupled paths run in the batched forms (nmlmc_set_form)");
     PortfolioParityHost H;
     H.pf = pf;
     H.ep = ep;
@@ -246,10 +251,11 @@ int Engine::sample_portfolio_parity(uint32_t ell, uint64_t M, uint32_t, sample_o
     H.T = cores;
     H.ell = ell;
     H.bs_work = pf_bs_work;
-    int rc = ensure_partials(cores);
+    Partial *parts, *res;
+    int rc = plain_buffers(cores, parts, res);
     if (rc) return rc;
     H.states = d_states;
-    H.partials = d_partials;
+    H.partials = parts;
     TraceBufs tb;
     rc = trace_alloc(M, tr_phi_f || tr_phi_c || tr_nf || tr_nc, tb);
     if (rc) return rc;
@@ -262,10 +268,10 @@ int Engine::sample_portfolio_parity(uint32_t ell, uint64_t M, uint32_t, sample_o
     if (e != hipSuccess) return hip_fail(e, "hipEventRecord");
     if ((e = launch_portfolio_parity(H, stream)) != hipSuccess) return hip_fail(e, "launch k_portfolio_parity");
     if ((e = hipEventRecord(ev1, stream)) != hipSuccess) return hip_fail(e, "hipEventRecord");
-    if ((e = launch_reduce_partials(d_partials, cores, d_out, stream)) != hipSuccess)
+    if ((e = launch_reduce_partials(parts, cores, res, stream)) != hipSuccess)
         return hip_fail(e, "launch k_reduce_partials");
     last_kernel = "k_portfolio_parity";
-    rc = finish_call(ell, out);
+    rc = finish_call(ell, out, res);
     if (rc) return rc;
     rc = trace_finish(M, tb, tr_phi_f, tr_phi_c, tr_nf, tr_nc);
     if (rc) return rc;
@@ -461,6 +467,11 @@ void portfolio_estimate_delta_for_unbiased(portfolio_data* pdata, double TOL) {
         ops[j].delta_est += total[0] / total_M;
     }
     set_error(NMLMC_OK, "");
+}
+
+int nmlmc_set_level_coupled_paths(int on) {
+    g_engine.level_coupled = on != 0;
+    return set_error(NMLMC_OK, "");
 }
 
 int nmlmc_portfolio_set_correlation(const double* chol, uint32_t d) {

@@ -129,6 +129,69 @@ __device__ __forceinline__ void stream_bm_f64(uint64_t k0, uint64_t k1, uint64_t
     boxmuller_f64(second, first, x, y);
 }
 
+// A random_state (par_omp.hpp:72-121) positioned on a keyed counter sequence:
+// block b of the stream is threefry({b, c1}, {k0, k1}); word q is element
+// 1 - (q & 1) of block q >> 1.  c1 = 0 for the reference keying's per-thread
+// streams; c1 = (tag << 40) | unit for a batched inner unit.
+struct DStream {
+    uint64_t k0, k1, c1, P;
+    double saved;
+    uint32_t is_saved;
+    uint64_t blk;
+    u64x2 cur;
+    uint64_t nu, nn;  // randu / randn calls (inner_stats[4], [5])
+
+    __device__ void init(uint64_t a, uint64_t b, uint64_t c, uint64_t p, double sv, uint32_t iss) {
+        k0 = a;
+        k1 = b;
+        c1 = c;
+        P = p;
+        saved = sv;
+        is_saved = iss;
+        blk = ~0ull;
+        nu = nn = 0;
+    }
+    __device__ uint64_t rand() {
+        const uint64_t q = P++;
+        const uint64_t b = q >> 1;
+        if (b != blk) {
+            cur = threefry2x64_20(b, c1, k0, k1);
+            blk = b;
+        }
+        return (q & 1) ? cur.a : cur.b;
+    }
+    __device__ double randn() {
+        nn++;
+        if (is_saved) {
+            is_saved = 0;
+            return saved;
+        }
+        const uint64_t first = rand();
+        const uint64_t second = rand();
+        double x, y;
+        boxmuller_f64(second, first, x, y);
+        is_saved = 1;
+        saved = y;
+        return x;
+    }
+    __device__ double randu() {
+        nu++;
+        return u01_f64(rand());
+    }
+    // Move past the next k normals of a normal-only run of draws (the cached
+    // one first, then Box-Muller pairs): the stream is then where k randn()
+    // calls leave it, its last pair computed when k ends inside one.
+    __device__ void skip_normals(uint64_t k) {
+        if (k == 0) return;
+        if (is_saved) {
+            is_saved = 0;
+            k--;
+        }
+        P += 2 * (k >> 1);
+        if (k & 1) (void)randn();
+    }
+};
+
 // Persistent state of one reference stream between calls: P words handed out,
 // plus the cached second normal of the last Box-Muller pair.
 struct StreamState {

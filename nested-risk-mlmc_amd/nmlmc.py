@@ -87,7 +87,8 @@ EXPORTS = [
     "portfolio_estimate_delta_for_unbiased", "portfolio_gen_data", "portfolio_free_data",
     "nmlmc_mlmc_last_table", "nmlmc_autotune_level", "nmlmc_sample_levels",
     "nmlmc_batch_begin", "nmlmc_batch_stage", "nmlmc_batch_sample", "nmlmc_batch_mlmc_run",
-    "nmlmc_source_digest",
+    "nmlmc_source_digest", "nmlmc_sampler_model_init", "nmlmc_sampler_model_sample",
+    "nmlmc_set_level_coupled_paths",
 ]
 
 _lib = None
@@ -99,12 +100,15 @@ class NmlmcError(RuntimeError):
 
 def source_digest():
     """sha256 (first 16 hex digits) of the library's sources as the Makefile
-    computes it: csrc/*.hip, *.cpp, *.hpp and include/nmlmc.h, concatenated in
-    path order; None when the sources are not beside the package."""
+    computes it: csrc/*.hip, *.cpp, *.hpp and include/nmlmc.h, *.hpp,
+    concatenated in path order; None when the sources are not beside the
+    package."""
     import glob
     import hashlib
     files = [p for ext in ("hip", "cpp", "hpp") for p in glob.glob(os.path.join(HERE, "csrc", "*." + ext))]
-    files.append(os.path.join(os.path.dirname(HERE), "include", "nmlmc.h"))
+    inc = os.path.join(os.path.dirname(HERE), "include")
+    files.append(os.path.join(inc, "nmlmc.h"))
+    files += glob.glob(os.path.join(inc, "*.hpp"))
     files = sorted(os.path.abspath(p) for p in files)
     if not all(os.path.exists(p) for p in files):
         return None
@@ -160,6 +164,9 @@ def load(path=LIB_PATH):
         "nmlmc_last_kernel_ms": (dbl, []),
         "nmlmc_last_kernel_name": (C.c_char_p, []),
         "nmlmc_source_digest": (C.c_char_p, []),
+        "nmlmc_set_level_coupled_paths": (i32, [i32]),
+        "nmlmc_sampler_model_init": (i32, [C.POINTER(ModelData), u64]),
+        "nmlmc_sampler_model_sample": (i32, [u32, u64, C.POINTER(SampleOutput)]),
         "nmlmc_last_group_lanes": (u32, []),
         "nmlmc_autotune_level": (i32, [u32, u64, C.POINTER(u32), C.POINTER(u32), dp]),
         "nmlmc_sample_levels": (i32, [u32, C.POINTER(u32), C.POINTER(u64), C.POINTER(SampleOutput)]),
@@ -264,6 +271,20 @@ def sample(ell, M, outer_offset=None, trace=False, moments=4, phi_size=1):
     if trace:
         out.update(phi_f=tr[0], phi_c=tr[1], nf=tr[2], nc=tr[3])
     return out
+
+
+def sampler_model_init(phi_type, tau, max_loss, seed):
+    """model::sampler as a class of the reference's sampler concept, staged
+    through nmlmc::init_samplers<SAMPLER> (nmlmc_sampler_model_init)."""
+    d = ModelData(np.sqrt(tau), np.sqrt(1 - tau), max_loss, phi_type)
+    check(load().nmlmc_sampler_model_init(C.byref(d), seed))
+
+
+def sampler_model_sample(ell, M, moments=4):
+    """One level call of that class through nmlmc::sample<SAMPLER>."""
+    o = Output(moments)
+    check(load().nmlmc_sampler_model_sample(ell, M, C.byref(o.c)))
+    return o.as_dict()
 
 
 def sample_levels(ells, Ms, moments=4, phi_size=1):
@@ -450,6 +471,12 @@ def portfolio_sample(ell, M, override_dyn_inner=0, moments=4, phi_size=1):
     load().portfolio_sample(ell, override_dyn_inner, M, C.byref(o.c))
     check()
     return o.as_dict()
+
+
+def set_level_coupled_paths(on):
+    """Level-coupled inner paths for the method-2 options of portfolios staged
+    from now on (nmlmc_set_level_coupled_paths)."""
+    check(load().nmlmc_set_level_coupled_paths(int(bool(on))))
 
 
 def portfolio_set_correlation(chol):

@@ -109,6 +109,17 @@ void orc_boxmuller(uint64_t u0, uint64_t u1, double* out) {
     out[1] = c * r;
 }
 
+/* Zeroed memory on 128-byte boundaries in whole 128-byte lines: arrays of
+ * per-thread records written by OpenMP threads must not share cache lines. */
+static void* orc_calloc_lines(size_t n, size_t size) {
+    size_t bytes = (n ? n : 1) * size;
+    bytes = (bytes + 127) & ~(size_t)127;
+    void* p = aligned_alloc(128, bytes);
+    if (p) memset(p, 0, bytes);
+    return p;
+}
+static size_t orc_line_reals(size_t n) { return (n + 15) & ~(size_t)15; }
+
 /* random_state (par_omp.hpp:72-121).  rand() hands out cur[1] then cur[0]
  * (par_omp.hpp:92-99); ctr.incr() is a carrying little-endian increment
  * (Random123/array.h:133-163).  randn() calls boxmuller(rand(), rand()): the
@@ -458,6 +469,7 @@ typedef struct {
     real_t dt_prob_expo;
     int subsample;
     int use_delta_cv, use_coarse_delta_cv, use_antithetic_risk, use_milstein;
+    int coupled; /* extension: method-2 options on level-coupled paths */
     prob_t compute_prob[3];
     real_t* delta_coalesce;
     uint32_t options_count;
@@ -474,7 +486,14 @@ typedef struct {
     real_t Y;
     real_t *W_tau, *R_tau;
     real_t constant;
-} orc_sampler;
+    /* Level-coupled paths (extension): the current draw group's steps, whether
+     * it is a two-valued final (and the fine-to-coarse step ratio), and the
+     * coarse value of the last inner sample (its fine value otherwise). */
+    uint64_t cpl_steps;
+    int cpl_two;
+    uint32_t cpl_cs;
+    real_t last_vc;
+} __attribute__((aligned(128))) orc_sampler; /* one cache-line pair each: no false sharing between threads */
 
 struct orc_ctx {
     uint32_t cores;
@@ -668,6 +687,49 @@ static real_t sample_inner_unbiased(orc_ctx* c, orc_sampler* s, const orc_option
     return (real_t)((prob_t)ret * L_prob_1);
 }
 
+/* Level-coupled inner sample of a method-2 option (extension; the batched-risk
+ * workload's Asians): sample_inner_approx's paths (539-604) with the step count
+ * of the draw group's level, dt_h0inv 2^floor(l/2), instead of a drawn level:
+ * the S0 path and its antithetic twin exactly to tau (one normal), then
+ * s->cpl_steps Milstein steps of the three paths over (tau, T]; in a two-valued
+ * final also the coarse paths of cpl_steps / cpl_cs steps on the same
+ * increments.  Fine value returned, coarse value (the fine one when there are
+ * no coarse paths) left in s->last_vc; each value is the antithetic mean of
+ * the option differences plus the delta control variate on its own paths. */
+static real_t sample_inner_coupled(orc_ctx* c, orc_sampler* s, const orc_option* o, work_t* tw) {
+    const orc_portfolio* pf = &c->pf;
+    uint32_t si = o->stock_ind;
+    const orc_stock* st = &pf->stocks[si];
+    real_t dW_0_tau = pf->stau * s_randn(s, s->inner);
+    real_t S0_t_1_i = simulate_stock_r(st->S0, pf->r, st->sigma, pf->tau, dW_0_tau);
+    real_t S0_t_2_i = simulate_stock_r(st->S0, pf->r, st->sigma, pf->tau, -dW_0_tau);
+    const uint64_t N = s->cpl_steps;
+    const int coarse = s->cpl_two && s->cpl_cs > 1;
+    *tw += (real_t)((N / pf->dt_h0inv) * o->W_est);
+    if (coarse) *tw += (real_t)((N / s->cpl_cs / pf->dt_h0inv) * o->W_est);
+    sum_t S[6] = {S0_t_1_i, s->R_tau[si], S0_t_2_i, S0_t_1_i, s->R_tau[si], S0_t_2_i};
+    sum_t A[6] = {0, 0, 0, 0, 0, 0};
+    sum_t* A_p = is_asian(o->type) ? A : NULL;
+    sampler_randn_ctx rc = {s};
+    simulate_stock_dt_r(coarse, sampler_inner_randn, &rc, S, A_p, N, pf->r, st->sigma, o->maturity - pf->tau, 3,
+                        pf->use_milstein, coarse ? s->cpl_cs : 2);
+    const sum_t* V = A_p ? A : S;
+    real_t val_f = 0, val_c = 0, val_d_f = 0, val_d_c = 0;
+    const real_t m = pf->use_antithetic_risk ? 0.5 : 1.;
+    const real_t diff = delta_factor(pf, s, si);
+    for (int i = 0; i < (pf->use_antithetic_risk ? 2 : 1); i++) {
+        int ic = 2 * i;
+        val_f += m * option_diff_from_stock(o, (real_t)V[ic], (real_t)V[1], pf->r);
+        if (pf->use_delta_cv) val_d_f += m * option_derv_from_stock(o, (real_t)S[ic], st->S0, pf->r);
+        if (!coarse) continue;
+        val_c += m * option_diff_from_stock(o, (real_t)V[ic + 3], (real_t)V[4], pf->r);
+        if (pf->use_delta_cv) val_d_c += m * option_derv_from_stock(o, (real_t)S[ic + 3], st->S0, pf->r);
+    }
+    real_t vf = val_f + diff * val_d_f;
+    s->last_vc = coarse ? val_c + diff * val_d_c : vf;
+    return vf;
+}
+
 /* sampler::sample_delta_unbiased (portfolio_sampler.cpp:397-444): pathwise
  * delta of Unbiased option `o` from an antithetic pair of paths, the Milstein
  * paths (dt_h0inv steps, sigma and -sigma) against the exact GBM paths driven by
@@ -711,7 +773,7 @@ static real_t sample_inner_option(orc_ctx* c, orc_sampler* s, work_t* tw, int me
     switch (method) {
     case 0: return sample_inner_bs(c, s, o, tw);
     case 1: return sample_inner_exact(c, s, o, tw);
-    case 2: return sample_inner_unbiased(c, s, o, tw);
+    case 2: return c->pf.coupled ? sample_inner_coupled(c, s, o, tw) : sample_inner_unbiased(c, s, o, tw);
     default: abort();
     }
 }
@@ -734,7 +796,11 @@ static real_t sample_inner_random_option(orc_ctx* c, orc_sampler* s, work_t* tw)
         prob = pf->options[mi][idx].prob / prob_norm;
     }
     orc_option o = pf->options[mi][idx];
-    return (real_t)((prob_t)sample_inner_option(c, s, tw, (int)mi, &o) / prob);
+    s->last_vc = NAN;
+    real_t v = (real_t)((prob_t)sample_inner_option(c, s, tw, (int)mi, &o) / prob);
+    /* Only a coupled sample has a coarse value of its own. */
+    s->last_vc = (mi == 2 && pf->coupled) ? (real_t)((prob_t)s->last_vc / prob) : v;
+    return v;
 }
 
 /* sample_inner_all_options (336-344). */
@@ -811,6 +877,7 @@ typedef struct {
     orc_sampler* s;
     uint64_t key[2]; /* batched: the outer sample's key */
     int spu;         /* batched: inner samples per keyed unit (2 for the shift model) */
+    uint32_t ell;    /* the level of the call */
 } draw_env;
 
 #define TAG_FINE_TEST 1
@@ -825,15 +892,26 @@ typedef struct {
  * sample's key, both the sync-role and inner-role draws of that sample coming
  * from that stream. */
 static void inner_loop(orc_ctx* c, draw_env* e, uint32_t tag, uint64_t first, uint64_t count,
-                       sum_t* out_s1, sum_t* out_s2, work_t* out_tw) {
-    sum_t s1 = 0, s2 = 0;
+                       sum_t* out_s1, sum_t* out_s2, work_t* out_tw, sum_t* out_s1c) {
+    sum_t s1 = 0, s2 = 0, s1c = 0;
     work_t tw = 0.;
     orc_sampler* s = e->s;
+    /* Level-coupled paths: coarse tests and the plain coarse final take the
+     * level below's steps; the antithetic final of a level > 0 is two-valued. */
+    if (c->problem == ORC_PORTFOLIO && c->pf.coupled) {
+        const int below = (tag >= TAG_COARSE_TEST && tag < TAG_FINAL) || tag == TAG_FINAL_COARSE;
+        const uint32_t lv = below ? (e->ell ? e->ell - 1 : 0) : e->ell;
+        s->cpl_steps = (uint64_t)c->pf.dt_h0inv << (lv / 2);
+        s->cpl_two = tag == TAG_FINAL && e->ell > 0 && c->p.antithetic;
+        s->cpl_cs = s->cpl_two ? (uint32_t)(s->cpl_steps / ((uint64_t)c->pf.dt_h0inv << ((e->ell - 1) / 2))) : 1;
+    }
     if (e->keying == ORC_KEY_PARITY) {
         for (uint64_t i = 0; i < count; i++) {
+            s->last_vc = NAN;
             real_t d = sampler_inner(c, s, &tw);
             s1 += d;
             if (out_s2) s2 += d * d;
+            if (out_s1c) s1c += (c->problem == ORC_PORTFOLIO && c->pf.coupled) ? s->last_vc : d;
         }
     } else {
         orc_rs unit;
@@ -847,30 +925,33 @@ static void inner_loop(orc_ctx* c, draw_env* e, uint32_t tag, uint64_t first, ui
                 unit.ctr[1] = ((uint64_t)tag << 40) | (i / e->spu);
                 if (i % e->spu) (void)rs_randn(&unit); /* skip the pair's first normal */
             }
+            s->last_vc = NAN;
             real_t d = sampler_inner(c, s, &tw);
             s1 += d;
             if (out_s2) s2 += d * d;
+            if (out_s1c) s1c += (c->problem == ORC_PORTFOLIO && c->pf.coupled) ? s->last_vc : d;
         }
         s->inner = save_in;
         s->sync = save_sync;
     }
     *out_s1 += s1;
     if (out_s2) *out_s2 += s2;
+    if (out_s1c) *out_s1c += s1c;
     *out_tw += tw;
 }
 
 /* sample_inner_sum (105-111). */
 static void sample_inner_sum(orc_ctx* c, draw_env* e, uint32_t tag, uint64_t first, uint64_t count,
-                             real_t* start, sum_t* out_s1, work_t* tw) {
+                             real_t* start, sum_t* out_s1, work_t* tw, sum_t* out_s1c) {
     *start = sampler_init_inner(c, e->s, tw);
-    inner_loop(c, e, tag, first, count, out_s1, NULL, tw);
+    inner_loop(c, e, tag, first, count, out_s1, NULL, tw, out_s1c);
 }
 
 /* sample_inner_mean (112-119). */
 static real_t sample_inner_mean(orc_ctx* c, draw_env* e, uint32_t tag, uint64_t count, work_t* tw) {
     real_t start = sampler_init_inner(c, e->s, tw);
     sum_t s1 = 0;
-    inner_loop(c, e, tag, 0, count, &s1, NULL, tw);
+    inner_loop(c, e, tag, 0, count, &s1, NULL, tw, NULL);
     return (real_t)(start + s1 / count);
 }
 
@@ -878,7 +959,7 @@ static real_t sample_inner_mean(orc_ctx* c, draw_env* e, uint32_t tag, uint64_t 
 static real_t sample_inner_delta2(orc_ctx* c, draw_env* e, uint32_t tag, uint64_t count, work_t* tw) {
     real_t start = sampler_init_inner(c, e->s, tw);
     sum_t s1 = 0, s2 = 0;
-    inner_loop(c, e, tag, 0, count, &s1, &s2, tw);
+    inner_loop(c, e, tag, 0, count, &s1, &s2, tw, NULL);
     real_t out_var = (real_t)(s2 / (count - 1) - (s1 * s1) / (double)(count * (count - 1)));
     real_t dist = start + (real_t)(s1 / count);
     return dist * dist / out_var;
@@ -911,12 +992,17 @@ static uint64_t sample_phi_antithetic(orc_ctx* c, draw_env* e, uint64_t count_f,
     real_t coeff_small = 1.f / groups_small;
     real_t start_small = 0;
     sum_t s1_big = 0;
+    /* Two-valued finals (level-coupled paths): the fine estimator takes the
+     * fine values, the coarse one the coarse values; the big side is the fine
+     * one when count_f > count_c.  Otherwise both sums are the same. */
+    const int fine_big = count_f > count_c;
     for (uint64_t i = 0; i < groups_small; i++) {
-        sum_t s1_small = 0;
-        sample_inner_sum(c, e, TAG_FINAL, i * count_small, count_small, &start_small, &s1_small, tw);
+        sum_t s1_f = 0, s1_c = 0;
+        sample_inner_sum(c, e, TAG_FINAL, i * count_small, count_small, &start_small, &s1_f, tw, &s1_c);
+        sum_t s1_small = fine_big ? s1_c : s1_f;
         real_t mean_small = (real_t)(start_small + s1_small / count_small);
         sampler_add_phi(c, mean_small, coeff_small, out_small);
-        s1_big += s1_small;
+        s1_big += fine_big ? s1_f : s1_c;
     }
     real_t mean_big = (real_t)(start_small + s1_big / count_big);
     sampler_add_phi(c, mean_big, 1.f, out_big);
@@ -1034,7 +1120,7 @@ typedef struct {
     sum_t f[MAX_PHI * MAX_MOM], d[MAX_PHI * MAX_MOM];
     work_t tw;
     uint64_t st[NSTATS];
-} orc_acc;
+} __attribute__((aligned(128))) orc_acc;
 
 /* Outer samples [s0, e0) on sampler s: the m-loop of kernel_sample. */
 static void run_range(orc_ctx* c, orc_sampler* s, int keying, uint32_t ell, uint64_t s0, uint64_t e0,
@@ -1048,6 +1134,7 @@ static void run_range(orc_ctx* c, orc_sampler* s, int keying, uint32_t ell, uint
         e.keying = keying;
         e.s = s;
         e.spu = (c->problem == ORC_MODEL_SHIFT) ? 2 : 1;
+        e.ell = ell;
         orc_rs outer;
         if (keying == ORC_KEY_BATCHED) {
             e.key[0] = c->seed;
@@ -1095,7 +1182,7 @@ void orc_sample(orc_ctx* c, int keying, uint32_t ell, uint64_t M, uint64_t outer
     sc_init(c, fine_only ? 0 : ell - 1, &sc_c);
 
     const uint64_t T = (keying == ORC_KEY_PARITY) ? c->cores : (M + ORC_BATCH_CHUNK - 1) / ORC_BATCH_CHUNK;
-    orc_acc* acc = (orc_acc*)calloc(T ? T : 1, sizeof(orc_acc));
+    orc_acc* acc = (orc_acc*)orc_calloc_lines(T, sizeof(orc_acc));
     if (keying == ORC_KEY_BATCHED) orc_ensure_workers(c);
 #pragma omp parallel for schedule(dynamic, 1)
     for (uint64_t t = 0; t < T; t++) {
@@ -1128,6 +1215,10 @@ void orc_sample(orc_ctx* c, int keying, uint32_t ell, uint64_t M, uint64_t outer
     free(acc);
 }
 
+/* Level-coupled paths for the staged portfolio's method-2 options (extension;
+ * sub-sampling mode 1). */
+void orc_set_coupled(orc_ctx* c, int on) { c->pf.coupled = on && c->pf.subsample == 1 && c->pf.nopt[2] > 0; }
+
 /* The outer draws of a level call under the batched keying, for tests of the
  * outer draw: R_out[m * Q + i] = R_tau[i] of outer sample offset + m (the
  * state sample_outer, portfolio_sampler.cpp:293-311, leaves behind), and the
@@ -1158,7 +1249,7 @@ void orc_outer_states(orc_ctx* c, uint32_t ell, uint64_t M, uint64_t outer_offse
 void orc_sample_delta_unbiased(orc_ctx* c, uint32_t option_index, uint64_t M, uint32_t moments,
                                double* sums, double* work) {
     const uint32_t T = c->cores;
-    orc_acc* acc = (orc_acc*)calloc(T ? T : 1, sizeof(orc_acc));
+    orc_acc* acc = (orc_acc*)orc_calloc_lines(T, sizeof(orc_acc));
     const orc_option* o = &c->pf.options[2][option_index];
 #pragma omp parallel for schedule(dynamic, 1)
     for (uint32_t t = 0; t < T; t++) {
@@ -1196,7 +1287,7 @@ orc_ctx* orc_new(uint32_t cores, const orc_params* p) {
     c->cores = cores;
     c->p = *p;
     c->phi_size = 1;
-    c->samplers = (orc_sampler*)calloc(cores, sizeof(orc_sampler));
+    c->samplers = (orc_sampler*)orc_calloc_lines(cores, sizeof(orc_sampler));
     return c;
 }
 
@@ -1230,14 +1321,14 @@ static void orc_ensure_workers(orc_ctx* c) {
     if (c->workers && c->nworkers >= n) return;
     free(c->workers);
     free(c->worker_ws);
-    c->workers = (orc_sampler*)calloc(n, sizeof(orc_sampler));
-    c->worker_ws = c->ws_reals ? (real_t*)calloc((size_t)c->ws_reals * n, sizeof(real_t)) : NULL;
+    c->workers = (orc_sampler*)orc_calloc_lines(n, sizeof(orc_sampler));
+    c->worker_ws = c->ws_reals ? (real_t*)orc_calloc_lines(orc_line_reals(c->ws_reals) * n, sizeof(real_t)) : NULL;
     for (int i = 0; i < n; i++) {
         orc_sampler* s = &c->workers[i];
         s->sync = &s->own_sync;
         s->inner = &s->own_inner;
         if (c->ws_reals) {
-            s->W_tau = c->worker_ws + (size_t)i * c->ws_reals;
+            s->W_tau = c->worker_ws + (size_t)i * orc_line_reals(c->ws_reals);
             s->R_tau = s->W_tau + c->ws_reals / 2;
         }
     }
@@ -1253,7 +1344,7 @@ static void seed_samplers(orc_ctx* c, uint64_t seed, uint32_t ws_reals) {
     c->worker_ws = NULL;
     c->nworkers = 0;
     free(c->workspace);
-    c->workspace = ws_reals ? (real_t*)calloc((size_t)ws_reals * c->cores, sizeof(real_t)) : NULL;
+    c->workspace = ws_reals ? (real_t*)orc_calloc_lines(orc_line_reals(ws_reals) * c->cores, sizeof(real_t)) : NULL;
     for (uint32_t t = 0; t < c->cores; t++) {
         orc_sampler* s = &c->samplers[t];
         memset(s, 0, sizeof(*s));
@@ -1263,7 +1354,7 @@ static void seed_samplers(orc_ctx* c, uint64_t seed, uint32_t ws_reals) {
         s->inner = &s->own_inner;
         s->Y = NAN;
         if (ws_reals) {
-            s->W_tau = c->workspace + (size_t)t * ws_reals;
+            s->W_tau = c->workspace + (size_t)t * orc_line_reals(ws_reals);
             s->R_tau = s->W_tau + ws_reals / 2;
         }
     }

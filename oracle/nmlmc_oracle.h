@@ -69,6 +69,13 @@ void orc_sample(orc_ctx* c, int keying, uint32_t ell, uint64_t M, uint64_t outer
                 double* fsums, double* diffsums, double* work, uint64_t* stats,
                 double* tr_phi_f, double* tr_phi_c, uint64_t* tr_nf, uint64_t* tr_nc);
 
+/* Level-coupled paths for the staged portfolio's method-2 options (the
+ * batched-risk workload's Asians; needs sub-sampling mode 1 and method-2
+ * options, else off): at level l such an option's inner sample runs
+ * dt_h0inv 2^floor(l/2) Milstein steps, fine and (antithetic finals of l > 0)
+ * coarse values from the same increments. */
+void orc_set_coupled(orc_ctx* c, int on);
+
 /* Portfolio outer draws of outer samples [offset, offset + M) at level ell
  * under the batched keying: R_out[m * Q + i] = R_tau[i]; const_out[m] (may be
  * NULL) = the sampler's constant. */

@@ -179,6 +179,7 @@ class Oracle(_Common):
                                                   _dp, _dp]
         lib.orc_samples_count.restype = C.c_uint64
         lib.orc_outer_states.argtypes = [C.c_void_p, C.c_uint32, C.c_uint64, C.c_uint64, _dp, _dp]
+        lib.orc_set_coupled.argtypes = [C.c_void_p, C.c_int]
         lib.orc_real_bytes.restype = C.c_int
         self.real = real
         self.ctx = None
@@ -242,6 +243,10 @@ class Oracle(_Common):
         self.lib.orc_sample_delta_unbiased(self.ctx, option_index, C.c_uint64(M), moments,
                                            _ptr(sums, _dp), _ptr(work, _dp))
         return sums, work[0]
+
+    def set_coupled(self, on):
+        """Level-coupled paths for the staged portfolio's method-2 options."""
+        self.lib.orc_set_coupled(self.ctx, int(bool(on)))
 
     def outer_states(self, ell, M, outer_offset=0):
         """Portfolio outer draws under the batched keying: (R_tau [M x Q], the

@@ -252,3 +252,43 @@ def test_portfolio_level_sweep_and_autotune(nm, form):
     nm.set_form(nm.FORM_PLAIN)
     for g, r in zip(got, ref):
         assert _key(g) == _key(r)
+
+
+@pytest.mark.parametrize("form", [1, 2])
+def test_batch_level_coupled_books_match_oracle(nm, oracle_f64, form):
+    """The batched-risk workload as specified: books with 20% arithmetic Asians
+    on level-coupled Milstein paths, sampled together; each instance's level
+    sums, work and counts against the oracle on that book alone."""
+    cfg = importlib.import_module(nm.__name__.rsplit(".", 1)[0] + ".configs")
+    books = cfg.batch_books(4, seed=9, n_pilot=100, asian_frac=0.2)
+    calls = [(0, [120, 90, 60, 100]), (1, [60, 50, 40, 30]), (2, [20, 24, 16, 12]), (3, [10, 12, 8, 6])]
+    nm.init_nestedmc_sampler(1, 64, 2, 0.0, True)
+    nm.set_adaptive_rule(1, 4)
+    nm.set_form(form)
+    nm.set_level_coupled_paths(True)
+    nm.batch_begin(len(books))
+    keep, orcs = [], []
+    for i, (a, L, sc, q) in enumerate(books):
+        pf, start, delta = to_library(nm, a, **sc)
+        keep.append(pf)
+        nm.batch_stage(i, pf, 500 + i, L)
+        o = oracle_f64.__class__("f64").new(1, 64, 2, 0.0, True, adapt_rule=1, cap_log2=4)
+        o.portfolio_init(to_oracle(a, start, delta, **sc), 500 + i, L)
+        o.set_coupled(True)
+        orcs.append(o)
+    offsets = {}
+    for ell, Ms in calls:
+        print("batch level", ell, Ms, flush=True)
+        got = nm.batch_sample(ell, Ms)
+        for i, o in enumerate(orcs):
+            off = offsets.get((i, ell), 0)
+            r = o.sample(ell, Ms[i], keying=1, outer_offset=off)
+            offsets[(i, ell)] = off + Ms[i]
+            g = got[i]
+            assert np.array_equal(g["stats"], r["stats"]), (i, ell, g["stats"], r["stats"])
+            np.testing.assert_allclose(g["work"], r["work"], rtol=1e-12)
+            np.testing.assert_allclose(g["fsums"], r["fsums"], rtol=1e-9, atol=1e-300)
+            np.testing.assert_allclose(g["diffsums"], r["diffsums"], rtol=1e-9, atol=1e-300)
+    nm.set_level_coupled_paths(False)
+    nm.cleanup_nestedmc_sampler()
+    nm.set_form(nm.FORM_PLAIN)

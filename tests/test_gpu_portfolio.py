@@ -368,3 +368,55 @@ def test_option_search_with_ties_and_tiny_probabilities(nm, oracle_f64, form):
             _cmp(nm.sample(ell, M, trace=True), o.sample(ell, M, trace=True))
     nm.cleanup_nestedmc_sampler()
     nm.set_form(nm.FORM_PLAIN)
+
+
+@pytest.mark.parametrize("form,segment", [(1, 0), (2, 0), (1, 3), (2, 3)])
+def test_level_coupled_paths_match_oracle(nm, oracle_f64, form, segment, monkeypatch):
+    """Level-coupled Milstein paths for the method-2 options (Asians among them)
+    beside Exact options: at level l an Asian inner sample runs
+    8 2^floor(l/2) steps, and an antithetic final's coarse estimator takes the
+    values on the level below's steps from the same increments.  Batched FP64
+    and mixed forms against the oracle's batched keying over levels 0..4
+    (with segment = 3, the final draw groups split over several lanes and
+    their fine and coarse sums go through the cross-lane hand-off)."""
+    if segment:
+        monkeypatch.setenv("NMLMC_SEGMENT", str(segment))
+    d = _with_asians(51)
+    scal = dict(SCAL, max_loss=0.01, subsample=1, use_delta_cv=True, use_antithetic_risk=True,
+                phi_type=1 if form == 1 else 0, phi_size=1)
+    nm.init_nestedmc_sampler(1, 8, 2, 0.0, True)
+    nm.set_adaptive_rule(1, 3)
+    nm.set_form(form)
+    nm.set_level_coupled_paths(True)
+    pf, start, delta = to_library(nm, d, **scal)
+    nm.portfolio_init(pf, 29)
+    o = oracle_f64.new(1, 8, 2, 0.0, True, adapt_rule=1, cap_log2=3)
+    o.portfolio_init(to_oracle(d, start, delta, **scal), 29)
+    o.set_coupled(True)
+    for ell, M in ((0, 40), (1, 30), (2, 24), (3, 16), (4, 12)):
+        g = nm.sample(ell, M, outer_offset=0, trace=True)
+        r = o.sample(ell, M, keying=1, outer_offset=0, trace=True)
+        print("level", ell, flush=True)
+        _cmp_mixed(g, r, rtol=RTOL_UNBIASED) if form == 2 else _cmp(g, r, rtol=RTOL_UNBIASED)
+    nm.set_level_coupled_paths(False)
+    nm.cleanup_nestedmc_sampler()
+    nm.set_form(nm.FORM_PLAIN)
+
+
+def test_level_coupled_paths_need_batched_form(nm):
+    """The plain form (the reference keying) has no level-coupled paths, and
+    they need sub-sampling mode 1: both refused with a status."""
+    d = _with_asians(52)
+    nm.init_nestedmc_sampler(1, 8, 2, 0.0, True)
+    nm.set_level_coupled_paths(True)
+    pf, _, _ = to_library(nm, d, **dict(SCAL, max_loss=0.01, subsample=1, use_delta_cv=True,
+                                        use_antithetic_risk=True, phi_type=0, phi_size=1))
+    nm.portfolio_init(pf, 3)
+    with pytest.raises(nm.NmlmcError):
+        nm.sample(0, 10)
+    pf0, _, _ = to_library(nm, d, **dict(SCAL, max_loss=0.01, subsample=0, use_delta_cv=True,
+                                         use_antithetic_risk=True, phi_type=0, phi_size=1))
+    with pytest.raises(nm.NmlmcError):
+        nm.portfolio_init(pf0, 3)
+    nm.set_level_coupled_paths(False)
+    nm.cleanup_nestedmc_sampler()
