@@ -94,6 +94,9 @@ int Engine::ensure_device() {
     if ((e = hipEventCreate(&ev1)) != hipSuccess) return hip_fail(e, "hipEventCreate");
     if ((e = hipMalloc(&d_out, sizeof(Partial))) != hipSuccess) return hip_fail(e, "hipMalloc");
     if ((e = hipHostMalloc(&h_out, sizeof(Partial))) != hipSuccess) return hip_fail(e, "hipHostMalloc");
+    // The no-allocation entry point's scratch address; if it cannot be taken,
+    // that entry point reports it and everything else works on.
+    noalloc_setup(device);
     return NMLMC_OK;
 }
 
@@ -251,6 +254,14 @@ int Engine::trace_finish(uint64_t M, TraceBufs& tb, double* tr_phi_f, double* tr
 }
 
 int Engine::plain_buffers(size_t n, Partial*& partials, Partial*& result) {
+    if (scr_partials) {
+        if (n > scr_cap)
+            return set_error(NMLMC_ERR_ARG, "the level call needs %zu partial records, the scratch holds %zu "
+                                            "(fewer virtual threads or lanes)", n, scr_cap);
+        partials = scr_partials;
+        result = scr_result;
+        return NMLMC_OK;
+    }
     const int rc = ensure_partials(n);
     if (rc) return rc;
     partials = d_partials;

@@ -55,6 +55,13 @@ struct Engine {
     size_t ws_cap = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // Scratch of a plain-form level call: the per-block partial records (room
+    // for scr_cap) and the reduced record.  NULL: the engine's own buffers,
+    // grown on demand; the no-allocation entry point points them at its
+    // static device array for the length of its call.
+    Partial* scr_partials = nullptr;
+    size_t scr_cap = 0;
+    Partial* scr_result = nullptr;
     float last_ms = 0;
     const char* last_kernel = "";
     uint32_t last_G = 0;
@@ -89,7 +96,9 @@ struct Engine {
     int trace_finish(uint64_t M, TraceBufs& tb, double* tr_phi_f, double* tr_phi_c,
                      uint64_t* tr_nf, uint64_t* tr_nc);
     int finish_call(uint32_t ell, sample_output* out, const Partial* result);
-    // The plain form's partial records for n blocks and the reduced record.
+    // The plain form's partial records for n blocks and the reduced record:
+    // the scratch override when set (an error if n exceeds it), else the
+    // engine's own buffers.
     int plain_buffers(size_t n, Partial*& partials, Partial*& result);
     int sample(uint32_t ell, uint64_t M, uint64_t outer_offset, uint32_t lanes_override,
                sample_output* out, double* tr_phi_f, double* tr_phi_c, uint64_t* tr_nf,
@@ -171,6 +180,10 @@ extern Engine g_engine;
 
 int set_error(int code, const char* fmt, ...);
 int hip_fail(hipError_t e, const char* what);
+// Takes the device address of the no-allocation entry point's scratch array on
+// `device` (once per device; from ensure_device, at init).  A failure leaves
+// that entry point refusing calls and is not an error of the init.
+int noalloc_setup(int device);
 uint64_t uint_pow_u64(uint64_t x, uint64_t n);
 LevelCounts level_counts(const EngineParams& ep, uint32_t ell);
 

@@ -87,7 +87,7 @@ EXPORTS = [
     "portfolio_estimate_delta_for_unbiased", "portfolio_gen_data", "portfolio_free_data",
     "nmlmc_mlmc_last_table", "nmlmc_autotune_level", "nmlmc_sample_levels",
     "nmlmc_batch_begin", "nmlmc_batch_stage", "nmlmc_batch_sample", "nmlmc_batch_mlmc_run",
-    "nmlmc_source_digest", "nmlmc_sampler_model_init", "nmlmc_sampler_model_sample",
+    "nmlmc_source_digest", "nmlmc_sample_noalloc", "nmlmc_sampler_model_init", "nmlmc_sampler_model_sample",
     "nmlmc_set_level_coupled_paths",
 ]
 
@@ -164,6 +164,7 @@ def load(path=LIB_PATH):
         "nmlmc_last_kernel_ms": (dbl, []),
         "nmlmc_last_kernel_name": (C.c_char_p, []),
         "nmlmc_source_digest": (C.c_char_p, []),
+        "nmlmc_sample_noalloc": (i32, [u32, u64, C.POINTER(SampleOutput)]),
         "nmlmc_set_level_coupled_paths": (i32, [i32]),
         "nmlmc_sampler_model_init": (i32, [C.POINTER(ModelData), u64]),
         "nmlmc_sampler_model_sample": (i32, [u32, u64, C.POINTER(SampleOutput)]),
@@ -271,6 +272,14 @@ def sample(ell, M, outer_offset=None, trace=False, moments=4, phi_size=1):
     if trace:
         out.update(phi_f=tr[0], phi_c=tr[1], nf=tr[2], nc=tr[3])
     return out
+
+
+def sample_noalloc(ell, M, moments=4, phi_size=1):
+    """nmlmc_sample_noalloc: a plain-form level call that allocates no device
+    memory (its scratch is the library's static array)."""
+    o = Output(moments, phi_size)
+    check(load().nmlmc_sample_noalloc(ell, M, C.byref(o.c)))
+    return o.as_dict()
 
 
 def sampler_model_init(phi_type, tau, max_loss, seed):
