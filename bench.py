@@ -295,8 +295,14 @@ def time_to_eps(nm, form, cores, eps, M0, setup_fn=None):
         Eg += lv["sums_diff"][0] / lv["M"]
     rms = math.sqrt(r["bias_est"] ** 2 + r["stat_error"] ** 2)
     nm.cleanup_nestedmc_sampler()
+    # Levels the driver froze: after the starting level, a level whose sampled
+    # difference variance is zero gets no further samples (get_optimal_Ml), so
+    # its mean enters the estimate as sampled from its first M0 draws only.
+    frozen = [l for l, lv in enumerate(r["levels"]) if l > r["starting_level"] and lv["M"] > 0
+              and lv["sums_diff"][1] / lv["M"] - (lv["sums_diff"][0] / lv["M"]) ** 2 <= 0.0]
     return {"seconds": round(dt, 3), "estimate": Eg, "rmse_est": rms, "levels": len(r["levels"]),
-            "stat_error": r["stat_error"]}
+            "stat_error": r["stat_error"], "M": [lv["M"] for lv in r["levels"]],
+            "zero_variance_levels": frozen}
 
 
 def portfolio_configs(nm):
@@ -556,7 +562,11 @@ def main():
                                 "exact": exact,
                                 "estimates_within_3_sigma": bool(abs(tm["estimate"] - tp["estimate"]) <= 3 * sig),
                                 "mixed_error_over_rmse": abs(tm["estimate"] - exact) / tm["rmse_est"],
-                                "plain_error_over_rmse": abs(tp["estimate"] - exact) / tp["rmse_est"]}
+                                "plain_error_over_rmse": abs(tp["estimate"] - exact) / tp["rmse_est"],
+                                "note": "a level listed in zero_variance_levels drew no nonzero difference "
+                                        "in its first M0 outer samples and is frozen there by the driver's "
+                                        "allocation (get_optimal_Ml, mlmc.hpp:265-289); its mean (level 3: "
+                                        "-2.48e-5) is then missing from the estimate"}
     if args.portfolio != "none":
         extra["portfolio"] = portfolio_bench(nm, args.portfolio, args.pf_levels, args.pf_M0,
                                              args.pf_rel_eps, not args.no_tte, args.plain_cores,

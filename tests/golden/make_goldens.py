@@ -181,7 +181,15 @@ def mlmc_fixture(R):
              dict(name="nested-like", E=[0.02 * 2.0 ** -(1.0 * l) for l in range(L)],
                   V=[0.02 * 2.0 ** -(1.5 * l) for l in range(L)],
                   W=[32.0 * 2.0 ** l for l in range(L)], Ef=[0.0227] * L, Vf=[0.0222] * L,
-                  tol=2e-3, M0=1024)]
+                  tol=2e-3, M0=1024),
+             # A level whose draws show no fine/coarse difference (V_3 = 0, as
+             # a first batch of a rare-disagreement indicator level can): the
+             # driver never samples it beyond its first M0.
+             dict(name="zero-variance-level",
+                  E=[0.0 if l == 3 else 0.02 * 2.0 ** -(1.0 * l) for l in range(L)],
+                  V=[0.0 if l == 3 else 0.02 * 2.0 ** -(1.5 * l) for l in range(L)],
+                  W=[32.0 * 2.0 ** l for l in range(L)], Ef=[0.0227] * L, Vf=[0.0222] * L,
+                  tol=2e-4, M0=1024)]
     for cs in cases:
         ell, M, summ = R.mlmc_fake_run(cs["tol"], cs["M0"], cs["E"], cs["V"], cs["W"], cs["Ef"],
                                        cs["Vf"])
