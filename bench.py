@@ -310,6 +310,32 @@ def portfolio_configs(nm):
     return importlib.import_module(nm.__name__ + ".configs")
 
 
+def book_sweep(nm, which, levels, M0, form):
+    """Only a book's sweep (levels 0..levels, M_l = ceil(M0 2^-1.5l), default
+    launch shapes): generation, one warm-up sweep and one measured sweep in the
+    given form, for profiler and counter runs.  Prints one JSON line."""
+    cfg = portfolio_configs(nm)
+    a, L, sc = cfg.large_portfolio() if which == "large" else cfg.small_portfolio()
+    nm.init_nestedmc_sampler(1, 64, 2, 0.0, True)
+    nm.set_adaptive_rule(1, 4)
+    nm.set_form(form)
+    pf = nm.Portfolio(a["S0"], a["mu"], a["sigma"], a["counts"], a["strike"], a["maturity"], a["w"],
+                      a["stock_ind"], a["type"], a["prob"], a["W_est"], **sc)
+    pf.comp_params()
+    nm.portfolio_init(pf, 1)
+    nm.portfolio_set_correlation(L)
+    sweep(nm, M0, levels)
+    recs = []
+    t0 = time.perf_counter()
+    sweep(nm, M0, levels, recs)
+    el = time.perf_counter() - t0
+    print(json.dumps({"book": which, "form": form, "levels": levels, "M0": M0, "seconds": round(el, 4),
+                      "kernel_ms": [round(x["kernel_ms"], 3) for x in recs],
+                      "inner_samples": [x["inner"] for x in recs],
+                      "rechecked_decisions": sum(x["rechecks"] for x in recs)}), flush=True)
+    nm.cleanup_nestedmc_sampler()
+
+
 def portfolio_bench(nm, which, levels, M0, rel_eps, tte, plain_cores, tte_M0, tune=True, rel_eps2=None):
     """Portfolio books: per-level outer samples/s of the mixed and plain forms
     (N_base = 64 2^l, 3-sigma doubling capped at 2^4, M_l = ceil(M0 2^-1.5l)),
@@ -383,7 +409,7 @@ def portfolio_bench(nm, which, levels, M0, rel_eps, tte, plain_cores, tte_M0, tu
     return out
 
 
-def batch_bench(nm, n, rel_eps, M0, plain_cores=0, asian_frac=0.0):
+def batch_bench(nm, n, rel_eps, M0, plain_cores=0, asian_frac=0.0, coupled=False):
     """Batched risk: n independent books (configs.batch_books: d in {10, 50,
     100}, P log-uniform in [1e3, 1e5], thresholds at the 95/99/99.5% pilot
     quantiles), each driven by its own MLMC run to eps_i = rel_eps (1 - q_i),
@@ -392,17 +418,19 @@ def batch_bench(nm, n, rel_eps, M0, plain_cores=0, asian_frac=0.0):
     and the batched FP64 forms, and whether each instance's two estimates agree
     within 3 combined standard errors; with plain_cores, also the plain FP64
     form run problem after problem (the reference's way: one problem per
-    process)."""
+    process).  With coupled, the Asians take level-coupled Milstein paths
+    (dt_h0inv 2^floor(l/2) steps at level l, fine and coarse values from the
+    same increments) instead of randomised-level paths."""
     cfg = portfolio_configs(nm)
     t0 = time.perf_counter()
     books = cfg.batch_books(n, asian_frac=asian_frac)
     gen_s = time.perf_counter() - t0
     tols = [rel_eps * (1 - q) for (_, _, _, q) in books]
     out = {"workload": "%d independent books: d in {10,50,100}, P log-uniform in [1e3,1e5] options "
-                       "(%g%% arithmetic Asians by randomised-level Milstein paths, 20%% digitals, the "
+                       "(%g%% arithmetic Asians by %s Milstein paths, 20%% digitals, the "
                        "rest calls/puts; t3 weights), factor-model correlation, thresholds at the pilot "
                        "95/99/99.5%% quantiles; per-instance eps = %g (1 - q)"
-                       % (n, 100 * asian_frac, rel_eps),
+                       % (n, 100 * asian_frac, "level-coupled" if coupled else "randomised-level", rel_eps),
            "instances": n, "options_total": int(sum(len(b[0]["w"]) for b in books)),
            "generate_s": round(gen_s, 2)}
     runs = {}
@@ -410,6 +438,7 @@ def batch_bench(nm, n, rel_eps, M0, plain_cores=0, asian_frac=0.0):
         nm.init_nestedmc_sampler(1, 64, 2, 0.0, True)
         nm.set_adaptive_rule(1, 4)
         nm.set_form(form)
+        nm.set_level_coupled_paths(coupled)
         nm.batch_begin(n)
         keep = []
         for i, (a, L, sc, q) in enumerate(books):
@@ -421,6 +450,7 @@ def batch_bench(nm, n, rel_eps, M0, plain_cores=0, asian_frac=0.0):
         t1 = time.perf_counter()
         r = nm.batch_mlmc_run(tols, M0)
         dt = time.perf_counter() - t1
+        nm.set_level_coupled_paths(False)
         nm.cleanup_nestedmc_sampler()
         runs[label] = r
         out[label] = {"seconds": round(dt, 3), "max_levels": max(len(x["levels"]) for x in r),
@@ -463,7 +493,9 @@ def main():
     ap.add_argument("--batch-M0", type=int, default=1024)
     ap.add_argument("--batch-asian-frac", type=float, default=0.2,
                     help="share of Asian options in the books of the second batched-risk leg (0: skip it)")
-    ap.add_argument("--batch-asian-rel-eps", type=float, default=1.0)
+    ap.add_argument("--batch-asian-rel-eps", type=float, default=0.25)
+    ap.add_argument("--batch-asian-n", type=int, default=16,
+                    help="instances of the Asian batched-risk leg (level-coupled paths)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="use the default launch shapes instead of autotuning each level first")
     ap.add_argument("--M0", type=float, default=1e7)
@@ -489,9 +521,18 @@ def main():
     ap.add_argument("--no-small", action="store_true", help="skip the small book (config 3)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (for scaling rocprofv3 byte counters)")
+    ap.add_argument("--book-sweep", choices=["large", "small"],
+                    help="only run one warm-up and one measured mixed-form sweep of that book "
+                         "(default launch shapes; for profiler and counter runs)")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.book_sweep:
+        import __graft_entry__
+        nm = __graft_entry__.package()
+        nm.load()
+        book_sweep(nm, args.book_sweep, args.pf_levels, args.pf_M0, nm.FORM_MIXED)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the outer samples "
@@ -586,8 +627,8 @@ def main():
         extra["batch"] = batch_bench(nm, args.batch_n, args.batch_rel_eps, args.batch_M0,
                                      0 if args.no_plain else args.plain_cores)
         if args.batch_asian_frac > 0:
-            extra["batch_asian"] = batch_bench(nm, args.batch_n, args.batch_asian_rel_eps, args.batch_M0, 0,
-                                               args.batch_asian_frac)
+            extra["batch_asian"] = batch_bench(nm, args.batch_asian_n, args.batch_asian_rel_eps, args.batch_M0, 0,
+                                               args.batch_asian_frac, coupled=True)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args.cpu_M0, args.levels)

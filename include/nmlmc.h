@@ -276,7 +276,7 @@ uint32_t nmlmc_last_group_lanes(void);
 
 /* Launch-shape autotuner of the batched forms (no counterpart in the reference,
  * whose launch geometry is fixed): times the level call (ell, M) of the staged
- * problem in the current form over segments of 16, 32, 64 and 128 inner units
+ * problem in the current form over segments of 4, 8, 16, 32, 64 and 128 inner units
  * per lane and grids of 4, 8 and 16 blocks per CU, on outer samples set apart
  * for tuning (ids from 2^45), and keeps the fastest for later calls at level
  * ell.  Writes the choice and its kernel time (any pointer may be NULL).  The

@@ -113,14 +113,19 @@ __global__ __launch_bounds__(256) void k_b_init(typename P::Args A) {
 // sums may differ from the one the FP64 sums of the same draws give (the
 // batched FP64 form's).  With |s1 - s1_64| <= e1 and |s2 - s2_64| <= e2
 // (P::decision_bounds, plus 2^-21 s2 for the compensated sums' own rounding),
-// |dist - dist_64| <= e1 / n and |var - var_64| <= ev; both are doubled to
-// leave room for the FP64 side's rounding.  Bounds above a tenth of dist or
-// var leave the decision to the recheck.  Otherwise delta^2_64 is within 40%
-// of delta^2, so the FP64 count new_64 (decreasing in delta^2, as its power -1
-// or -r/2, r < 2) is within a factor 1.7 of new: a decision with new at least
-// a factor 2 from cur is certain.  Nearer ones take the exact interval
-// [qlo, qhi] of delta^2_64 and are uncertain when [nlo, nhi] straddles cur.
-// The uncertain ones go to the FP64 recheck (k_b_test with RECHECK).
+// |dist - dist_64| <= ed = 2 e1 / n and |var - var_64| <= ev (both doubled to
+// leave room for the FP64 side's rounding), so delta^2_64 lies in
+// [qlo, qhi] = [max(|dist| - ed, 0)^2 / (var + ev), (|dist| + ed)^2 / (var - ev)]
+// and the FP64 count new_64 (decreasing in delta^2, as its power -1 or -r/2)
+// in [nlo, nhi]: the decision is uncertain when that interval straddles cur.
+// An outer sample whose mean sits within ed of the threshold has qlo = 0 and
+// nhi infinite: it is certain to refine once nlo > cur, which is the common
+// case (its delta^2 is near 0).  With ev at half of var or more (or a bound
+// that is NaN or infinite) the decision goes to the recheck.  Shortcut: when
+// both bounds are under a tenth of dist and var, delta^2_64 is within 40% of
+// delta^2 and new_64 within a factor 1.7 of new (r < 2), so a decision with
+// new at least a factor 2 from cur is certain without the interval.  The
+// uncertain ones go to the FP64 recheck (k_b_test with RECHECK).
 template <class P, int RULE>
 __device__ __forceinline__ bool decision_uncertain(const BatchedCommon& c, const LevelCounts& lc, double start,
                                                    uint64_t cur, double s1, double s2, double sab, double smm) {
@@ -133,11 +138,14 @@ __device__ __forceinline__ bool decision_uncertain(const BatchedCommon& c, const
     const double ad = fabs(dist);
     const double ed = 2. * e1 / n;
     const double ev = 2. * (e2 + (2. * fabs(s1) + e1) * e1 / n) / (n - 1.);
-    if (!(ed < 0.1 * ad) || !(ev < 0.1 * var)) return true;  // also NaN and infinite bounds
-    const double q = dist * dist / var;
-    const double nc = (RULE == 1) ? lc.tmp / q : pow(q, -c.ep.adaptive_r / 2) * lc.tmp;
-    if (nc >= 2. * n || nc <= 0.5 * n) return false;
-    const double qlo = (ad - ed) * (ad - ed) / (var + ev), qhi = (ad + ed) * (ad + ed) / (var - ev);
+    if (!(ed < INFINITY) || !(ev < 0.5 * var)) return true;  // also NaN bounds and var <= 0
+    if (ed < 0.1 * ad && ev < 0.1 * var) {
+        const double q = dist * dist / var;
+        const double nc = (RULE == 1) ? lc.tmp / q : pow(q, -c.ep.adaptive_r / 2) * lc.tmp;
+        if (nc >= 2. * n || nc <= 0.5 * n) return false;
+    }
+    const double lo = fmax(ad - ed, 0.);
+    const double qlo = lo * lo / (var + ev), qhi = (ad + ed) * (ad + ed) / (var - ev);
     double nlo, nhi;
     if (RULE == 1) {
         nlo = lc.tmp / qhi;
@@ -162,7 +170,18 @@ __global__ __launch_bounds__(256, (mixed_waves<P, MIXED>())) void k_b_test(typen
     const uint32_t nce = aload_u32(RECHECK ? &c.ctl->rc_c[r] : &c.ctl->cnt_c[r]);
     const uint64_t cur_f = c.cf.base << r, cur_c = c.cc.base << r;
     const uint64_t uf = (cur_f + P::spu - 1) / P::spu, uc = (cur_c + P::spu - 1) / P::spu;
-    const uint64_t spe_f = (uf + c.S - 1) / c.S, spe_c = (uc + c.S - 1) / c.S;
+    // Units per segment.  The recheck lists hold few groups: their units are
+    // spread over the whole grid (at least 4 units a lane, at most c.S), so a
+    // group's FP64 pass takes a few samples' latency instead of c.S samples'.
+    uint64_t S = c.S;
+    if (RECHECK) {
+        const uint64_t lanes = (uint64_t)gridDim.x * blockDim.x;
+        const uint64_t units_all = (uint64_t)nfe * uf + (uint64_t)nce * uc;
+        S = (units_all + lanes - 1) / lanes;
+        S = S < 4 ? 4 : S;
+        S = S > c.S ? c.S : S;
+    }
+    const uint64_t spe_f = (uf + S - 1) / S, spe_c = (uc + S - 1) / S;
     const uint64_t segs_f = (uint64_t)nfe * spe_f, total = segs_f + (uint64_t)nce * spe_c;
     if (total == 0) return;  // nothing refined into this round: nothing to flush
     BlockAcc<P::kMulti> acc;
@@ -179,7 +198,7 @@ __global__ __launch_bounds__(256, (mixed_waves<P, MIXED>())) void k_b_test(typen
         const uint32_t m = (r == 0 && !RECHECK) ? (uint32_t)e : te.item;
         const ItemState& it = c.items[m];
         const uint64_t cur = fine ? cur_f : cur_c, units = fine ? uf : uc;
-        const uint64_t u0 = j * c.S, u1 = (u0 + c.S < units) ? u0 + c.S : units;
+        const uint64_t u0 = j * S, u1 = (u0 + S < units) ? u0 + S : units;
         SegAcc s;
         P::template units<MIXED>(A, m, it, (fine ? TAG_FINE_TEST : TAG_COARSE_TEST) + r, u0, u1, 0, cur, s);
         const uint32_t q = it.inst;
@@ -489,6 +508,10 @@ template <class P>
 __global__ __launch_bounds__(256) void k_b_redo_split(typename P::Args A) {
     const BatchedCommon& c = A.c;
     __shared__ uint32_t last;
+    // The block's per-group sums: lanes add into LDS, then one global add per
+    // group the block touched (global adds of every lane into one slot would
+    // serialise on its L2 channel).
+    __shared__ double gsum[kRedoGroups];
     const uint32_t n = aload_u32(&c.ctl->nreeval);
     const uint32_t part = blockIdx.y, parts = gridDim.y;
     for (uint32_t w = blockIdx.x; w < n; w += gridDim.x) {
@@ -500,12 +523,14 @@ __global__ __launch_bounds__(256) void k_b_redo_split(typename P::Args A) {
         const uint64_t U = (uint64_t)sh.groups * gu;
         const uint64_t p0 = U * part / parts, p1 = U * (part + 1) / parts;
         double* slot = c.redo_acc + (uint64_t)w * kRedoGroups;
+        for (uint32_t g = threadIdx.x; g < kRedoGroups; g += blockDim.x) gsum[g] = 0.;
+        __syncthreads();
         double cur = 0.;
         uint32_t cur_g = ~0u;
         for (uint64_t k = p0 + threadIdx.x; k < p1; k += blockDim.x) {
             const uint32_t g = (uint32_t)(k / gu);
             if (g != cur_g) {
-                if (cur_g != ~0u) aadd(&slot[cur_g], cur);
+                if (cur_g != ~0u) atomicAdd(&gsum[cur_g], cur);
                 cur = 0.;
                 cur_g = g;
             }
@@ -514,7 +539,12 @@ __global__ __launch_bounds__(256) void k_b_redo_split(typename P::Args A) {
             P::template units<false>(A, m, it, TAG_FINAL, k, k + 1, i0, i0 + sh.n, s);
             cur += s.s1;
         }
-        if (cur_g != ~0u) aadd(&slot[cur_g], cur);
+        if (cur_g != ~0u) atomicAdd(&gsum[cur_g], cur);
+        __syncthreads();
+        if (p1 > p0) {
+            const uint32_t g_lo = (uint32_t)(p0 / gu), g_hi = (uint32_t)((p1 - 1) / gu);
+            for (uint32_t g = g_lo + threadIdx.x; g <= g_hi; g += blockDim.x) aadd(&slot[g], gsum[g]);
+        }
         wait_acks();
         __syncthreads();
         if (threadIdx.x == 0) last = acount(&c.redo_cnt[w], 1u) == parts - 1;

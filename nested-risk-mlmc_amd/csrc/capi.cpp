@@ -740,8 +740,10 @@ Engine::Shape Engine::batched_shape(uint32_t ell, uint64_t M, uint64_t spu, cons
 
 // Times the batched level call (ell, M) of the staged problem in the current
 // form over candidate launch shapes and keeps the fastest for later calls at
-// this level: segments of 16, 32, 64 and 128 units at 8 blocks per CU, then 4
-// and 16 blocks per CU at the best segment.  Each candidate runs twice (the
+// this level: segments of 4, 8, 16, 32, 64 and 128 units at 8 blocks per CU,
+// then 4 and 16 blocks per CU at the best segment (the small segments spread
+// the few large draw groups of the high levels over more lanes; the large ones
+// pay fewer hand-offs at the low levels).  Each candidate runs twice (the
 // faster kernel time counts) on outer samples 2^45 onwards, far from those of
 // the level's calls, so later results do not depend on whether tuning ran
 // (and per-sample results never depend on the shape).
@@ -770,7 +772,7 @@ int Engine::autotune_level(uint32_t ell, uint64_t M, Shape& best, double& best_m
     };
     best = Shape{};
     best_ms = 1e300;
-    for (uint32_t S : {16u, 32u, 64u, 128u}) {
+    for (uint32_t S : {4u, 8u, 16u, 32u, 64u, 128u}) {
         double ms;
         if (int rc = time_shape(Shape{S, 8}, ms)) return rc;
         if (ms < best_ms) {
