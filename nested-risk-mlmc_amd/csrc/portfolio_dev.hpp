@@ -125,6 +125,10 @@ __device__ __forceinline__ void simulate_dt_d(DStream& rn, double* S, double* A,
     const double sqrt_dt = sqrt(dt);
     double dW_c = 0.;
     const double dt_c = t / (double)(N / coarse_step);
+    // k counts the fine steps since the last coarse step: k reaching
+    // coarse_step is (n + 1) % coarse_step == 0, without a 64-bit remainder
+    // per step.
+    uint32_t k = 0;
     for (uint64_t n = 0; n < N; n++) {
         const double dW = sqrt_dt * rn.randn();
         for (int i = 0; i < 3; i++) {
@@ -134,7 +138,8 @@ __device__ __forceinline__ void simulate_dt_d(DStream& rn, double* S, double* A,
         }
         if (with_coarse) {
             dW_c += dW;
-            if (((n + 1) % coarse_step) == 0) {
+            if (++k == coarse_step) {
+                k = 0;
                 for (int i = 0; i < 3; i++) {
                     const double m = milstein ? (0.5 * (sigma * sigma) * (dW_c * dW_c - dt_c)) : 0.;
                     S[i + 3] += S[i + 3] * (mu * dt_c + sigma * dW_c + m);
