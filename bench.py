@@ -310,9 +310,10 @@ def portfolio_configs(nm):
     return importlib.import_module(nm.__name__ + ".configs")
 
 
-def book_sweep(nm, which, levels, M0, form):
-    """Only a book's sweep (levels 0..levels, M_l = ceil(M0 2^-1.5l), default
-    launch shapes): generation, one warm-up sweep and one measured sweep in the
+def book_sweep(nm, which, levels, M0, form, tune=True):
+    """Only a book's sweep (levels 0..levels, M_l = ceil(M0 2^-1.5l)):
+    generation, launch-shape autotuning (as the portfolio leg does; default
+    shapes with tune false), one warm-up sweep and one measured sweep in the
     given form, for profiler and counter runs.  Prints one JSON line."""
     cfg = portfolio_configs(nm)
     a, L, sc = cfg.large_portfolio() if which == "large" else cfg.small_portfolio()
@@ -324,12 +325,14 @@ def book_sweep(nm, which, levels, M0, form):
     pf.comp_params()
     nm.portfolio_init(pf, 1)
     nm.portfolio_set_correlation(L)
+    shapes = autotune(nm, M0, levels) if tune else None
     sweep(nm, M0, levels)
     recs = []
     t0 = time.perf_counter()
     sweep(nm, M0, levels, recs)
     el = time.perf_counter() - t0
     print(json.dumps({"book": which, "form": form, "levels": levels, "M0": M0, "seconds": round(el, 4),
+                      "launch_shapes": shapes,
                       "kernel_ms": [round(x["kernel_ms"], 3) for x in recs],
                       "inner_samples": [x["inner"] for x in recs],
                       "rechecked_decisions": sum(x["rechecks"] for x in recs)}), flush=True)
@@ -523,7 +526,7 @@ def main():
                     help="only run a 1 GiB device copy (for scaling rocprofv3 byte counters)")
     ap.add_argument("--book-sweep", choices=["large", "small"],
                     help="only run one warm-up and one measured mixed-form sweep of that book "
-                         "(default launch shapes; for profiler and counter runs)")
+                         "(autotuned launch shapes unless --no-autotune; for profiler and counter runs)")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
@@ -532,7 +535,7 @@ def main():
         import __graft_entry__
         nm = __graft_entry__.package()
         nm.load()
-        book_sweep(nm, args.book_sweep, args.pf_levels, args.pf_M0, nm.FORM_MIXED)
+        book_sweep(nm, args.book_sweep, args.pf_levels, args.pf_M0, nm.FORM_MIXED, tune=not args.no_autotune)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the outer samples "
