@@ -68,8 +68,9 @@ __device__ __forceinline__ uint64_t next_count(const BatchedCommon& c, const Lev
 }
 
 // Outer draws.  Round 0 tests every outer sample, so its queue is implicit
-// (entry e is outer sample e); its accumulators are only written when a draw
-// group spans several segments.
+// (entry e is outer sample e); its entries are written zeroed when k_b_test
+// may split a round-0 group over several segments: a group of more units than
+// c.S, or than c.s_test_min when sparse rounds may shrink their segments to it.
 template <class P>
 __global__ __launch_bounds__(256) void k_b_init(typename P::Args A) {
     const BatchedCommon& c = A.c;
@@ -77,8 +78,9 @@ __global__ __launch_bounds__(256) void k_b_init(typename P::Args A) {
     acc.template init<false>(c);
     const bool test_f = c.cf.max > 2 * c.cf.base;
     const bool test_c = c.ell > 0 && c.cc.max > 2 * c.cc.base;
-    const bool acc_f = test_f && (c.cf.base + P::spu - 1) / P::spu > c.S;
-    const bool acc_c = test_c && (c.cc.base + P::spu - 1) / P::spu > c.S;
+    const uint64_t split_above = (c.s_test_min && c.s_test_min < c.S) ? c.s_test_min : c.S;
+    const bool acc_f = test_f && (c.cf.base + P::spu - 1) / P::spu > split_above;
+    const bool acc_c = test_c && (c.cc.base + P::spu - 1) / P::spu > split_above;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         c.ctl->cnt_f[0] = test_f ? (uint32_t)c.M : 0;
         c.ctl->cnt_c[0] = test_c ? (uint32_t)c.M : 0;
@@ -173,12 +175,24 @@ __global__ __launch_bounds__(256, (mixed_waves<P, MIXED>())) void k_b_test(typen
     // Units per segment.  The recheck lists hold few groups: their units are
     // spread over the whole grid (at least 4 units a lane, at most c.S), so a
     // group's FP64 pass takes a few samples' latency instead of c.S samples'.
+    // A round whose units come to less than two segments of c.S per lane (the
+    // later rounds test only the refining outer samples; round 0 of a level
+    // with few outer samples) takes smaller segments, at least c.s_test_min
+    // units (0: no smaller segments), so that every lane has work; per-sample
+    // results do not depend on the segment size.  Every round's entries start
+    // with zeroed sums (appended so, or written by k_b_init for round 0), so
+    // any split is safe.
     uint64_t S = c.S;
-    if (RECHECK) {
+    {
         const uint64_t lanes = (uint64_t)gridDim.x * blockDim.x;
         const uint64_t units_all = (uint64_t)nfe * uf + (uint64_t)nce * uc;
-        S = (units_all + lanes - 1) / lanes;
-        S = S < 4 ? 4 : S;
+        if (RECHECK) {
+            S = (units_all + lanes - 1) / lanes;
+            S = S < 4 ? 4 : S;
+        } else if (c.s_test_min && units_all < 2 * lanes * c.S) {
+            S = (units_all + 2 * lanes - 1) / (2 * lanes);
+            S = S < c.s_test_min ? c.s_test_min : S;
+        }
         S = S > c.S ? c.S : S;
     }
     const uint64_t spe_f = (uf + S - 1) / S, spe_c = (uc + S - 1) / S;
@@ -513,7 +527,7 @@ __global__ __launch_bounds__(256) void k_b_redo_split(typename P::Args A) {
     // serialise on its L2 channel).
     __shared__ double gsum[kRedoGroups];
     const uint32_t n = aload_u32(&c.ctl->nreeval);
-    const uint32_t part = blockIdx.y, parts = gridDim.y;
+    const uint32_t part = blockIdx.y;
     for (uint32_t w = blockIdx.x; w < n; w += gridDim.x) {
         const uint32_t m = c.reeval[w];
         const ItemState& it = c.items[m];
@@ -521,6 +535,13 @@ __global__ __launch_bounds__(256) void k_b_redo_split(typename P::Args A) {
         if (!redo_split<P>(c, w, sh)) continue;
         const uint64_t gu = sh.n / P::spu;  // units per group
         const uint64_t U = (uint64_t)sh.groups * gu;
+        // Blocks of the column this outer sample takes: one per blockDim.x
+        // units, at most gridDim.y; the others skip it (every block computes
+        // the same count from the same state, and the completing block waits
+        // for exactly these).
+        const uint64_t want = (U + blockDim.x - 1) / blockDim.x;
+        const uint32_t parts = want < gridDim.y ? (uint32_t)(want ? want : 1) : gridDim.y;
+        if (part >= parts) continue;
         const uint64_t p0 = U * part / parts, p1 = U * (part + 1) / parts;
         double* slot = c.redo_acc + (uint64_t)w * kRedoGroups;
         for (uint32_t g = threadIdx.x; g < kRedoGroups; g += blockDim.x) gsum[g] = 0.;

@@ -397,6 +397,17 @@ int Engine::prepare_batched(uint64_t M, BatchedCommon& c) {
 
 // Common arguments of a batched level call; checks that every draw-group size
 // is the smallest one times a power of two (finals are bucketed that way).
+// Smallest segment a sparse test round may shrink to (BatchedCommon::
+// s_test_min): portfolios only, since a model inner sample is too cheap for a
+// short segment to pay its hand-off (measured: the model sweep 5% slower at a
+// minimum of 8, the large book's 2.5% faster at 16).  NMLMC_TEST_SEGMENT_MIN
+// overrides it.
+uint32_t test_segment_min(bool portfolio) {
+    uint32_t v = portfolio ? 16u : 0u;
+    if (const char* env = getenv("NMLMC_TEST_SEGMENT_MIN")) v = (uint32_t)strtoul(env, nullptr, 10);
+    return v;
+}
+
 int Engine::batched_common(uint32_t ell, uint64_t M, uint64_t outer_offset, BatchedCommon& c) {
     std::memset(&c, 0, sizeof(c));
     c.ep = ep;
@@ -406,6 +417,7 @@ int Engine::batched_common(uint32_t ell, uint64_t M, uint64_t outer_offset, Batc
     c.M = M;
     c.offset = outer_offset;
     c.ell = ell;
+    c.s_test_min = test_segment_min(problem == PROBLEM_PORTFOLIO);
     auto rounds_of = [](const LevelCounts& lc) {
         uint32_t n = 0;
         for (uint64_t cur = lc.base; lc.max > 2 * cur; cur *= 2) n++;

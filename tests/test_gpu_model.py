@@ -122,7 +122,7 @@ def test_autotuned_launch_shape_keeps_results(nm, oracle_f64):
     orc = _shift_setup(nm, oracle_f64, 1, nm.FORM_BATCHED)
     for ell, M in ((0, 3000), (2, 400)):
         S, B, ms = nm.autotune_level(ell, M)
-        assert S in (16, 32, 64, 128) and B in (4, 8, 16) and ms > 0
+        assert S in (4, 8, 16, 32, 64, 128) and B in (4, 8, 16) and ms > 0
         g = nm.sample(ell, M, trace=True)
         assert nm.last_group_lanes() == S
         o = orc.sample(ell, M, keying=1, outer_offset=0, trace=True)
