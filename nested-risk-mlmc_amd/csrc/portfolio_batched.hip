@@ -136,8 +136,13 @@ struct PortfolioPolicyT {
         }
         const StridedR R = stocks_at_tau(A, m);
         double cv = 0.;
-        if (pf.use_delta_cv)
+        // Unrolled so that a lane issues several stocks' loads before it waits
+        // (a level with few outer samples has no other work to hide them
+        // behind); the additions keep their order.
+        if (pf.use_delta_cv) {
+#pragma unroll 8
             for (uint32_t i = 0; i < pf.d; i++) cv += (R[i] - pf.stocks[i].S0) * pf.delta_coalesce[i];
+        }
         double w0 = 0., det = 0.;
         if (!EXACT_ONLY) det = det_bs_sum(pf, R, w0);  // sub-sampling off: pre-summed BS options
         it.start = det + (-pf.max_loss - cv);
