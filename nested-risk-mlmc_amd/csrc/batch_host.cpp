@@ -202,8 +202,7 @@ int Engine::batch_sample(uint32_t ell, const uint64_t* Ms, sample_output* outs) 
         return hip_fail(e, "hipMemcpy batch tables");
     // Buffers for the largest chunk, then the chains chunk by chunk.
     const uint64_t maxc = std::min(std::max<uint64_t>(total, 1), chunk);
-    PortfolioBatchedHost H;
-    std::memset(&H.c, 0, sizeof(H.c));
+    PortfolioBatchedHost H{};
     int rc = batched_common(ell, maxc, 0, H.c);
     if (rc) return rc;
     H.c.s_test_min = test_segment_min(true);
@@ -214,6 +213,7 @@ int Engine::batch_sample(uint32_t ell, const uint64_t* Ms, sample_output* outs) 
     H.mixed = form == NMLMC_FORM_MIXED;
     H.exact_only = exact_only;
     H.multi = true;
+    H.R32 = (H.mixed && H.exact_only) ? pf_R32 : nullptr;
     H.c.phi_type = phi_type;
     H.c.n_inst = n;
     float ms_total = 0;

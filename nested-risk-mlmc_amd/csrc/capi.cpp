@@ -134,6 +134,7 @@ void Engine::free_batched() {
     bat_ctl = nullptr;
     bat_ctl_h = nullptr;
     pf_R = nullptr;
+    pf_R32 = nullptr;
     bat_cap = bat_fcap = 0;
     pf_R_cap = 0;
 }

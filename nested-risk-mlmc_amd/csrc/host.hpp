@@ -44,6 +44,7 @@ struct Engine {
     double pf_bs_work = 0.;        // work of one init_inner_sampler call
     std::vector<void*> pf_allocs;
     double* pf_R = nullptr;        // batched: outer states [M x d]
+    float* pf_R32 = nullptr;       // their FP32 copies (same buffer, after the FP64 rows)
     size_t pf_R_cap = 0;
 
     StreamState* d_states = nullptr;  // [2 cores]: parity streams (sync, inner)

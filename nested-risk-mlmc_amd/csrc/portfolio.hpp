@@ -135,6 +135,7 @@ struct PortfolioBatchedHost {
     BatchedCommon c;
     PortfolioDevice pf;
     double* R_tau;
+    float* R32;       // FP32 copies of R_tau for the mixed Exact loop (else NULL)
     double bs_work;
     uint32_t blocks;
     bool mixed;

@@ -28,6 +28,7 @@ struct PortfolioPolicyT {
         double* R_tau;  // [M x d] (MULTI: [M x dmax]): stock i of outer sample m at R_tau[m * stride + i]
         double bs_work; // work of one init_inner_sampler call (sub-sampling off)
         PortfolioBatchTables bt;
+        float* R32;     // mixed Exact-only form: FP32 copies of R_tau, same layout (else NULL)
     };
     static constexpr uint64_t spu = 1;
     static constexpr bool kMulti = MULTI;
@@ -121,15 +122,19 @@ struct PortfolioPolicyT {
             DStream os;
             os.init(seed_of(A, it), key1(A, m), 0, 0, 0., 0);
             const double dW_sys = pf.stau * os.randn();
+            float* Rw32 = A.R32 ? A.R32 + m * rstride(A) : nullptr;
             if (pf.d == 1) {
                 const StockRec st = pf.stocks[0];
                 Rw[0] = simulate_stock_d(st.S0, st.mu, st.sigma, pf.tau, dW_sys);
+                if (Rw32) Rw32[0] = (float)Rw[0];
             } else {
                 for (uint32_t i = 0; i < pf.d; i++) {
                     const double dW_ind = pf.stau * os.randn();
                     const double W = pf.rho * dW_sys + pf.s1_rho2 * dW_ind;
                     const StockRec st = pf.stocks[i];
-                    Rw[(uint64_t)i * stride] = simulate_stock_d(st.S0, st.mu, st.sigma, pf.tau, W);
+                    const double v = simulate_stock_d(st.S0, st.mu, st.sigma, pf.tau, W);
+                    Rw[(uint64_t)i * stride] = v;
+                    if (Rw32) Rw32[(uint64_t)i * stride] = (float)v;
                 }
             }
             acc.st(it.inst, 5, os.nn);
@@ -257,6 +262,10 @@ struct PortfolioPolicyT {
             // computed, so one sample's gathers overlap the other's arithmetic.
             const double cp = pf.compute_prob[1];
             const double wc = pf.w_est_const;
+            // The outer sample's stock levels as FP32 (half the bytes of the
+            // FP64 row per gather; the same values the FP64 row rounds to).
+            const float* R32 = A.R32 ? A.R32 + m * rstride(A) : nullptr;
+            auto Rf = [&](uint32_t si) -> float { return R32 ? R32[si] : (float)R[si]; };
             float m2 = 0.f;
             auto term = [&](const MixRec& q, float Rq, const u64x2& blk1, uint32_t idx) {
                 float x, y, mag;
@@ -277,7 +286,7 @@ struct PortfolioPolicyT {
                 const uint32_t ia = pick_cdf(pf, 1, u01_f64(a0.a) * cp);
                 const uint32_t ib = pick_cdf(pf, 1, u01_f64(b0.a) * cp);
                 const MixRec qa = pf.mix[ia], qb = pf.mix[ib];
-                const float Ra = (float)R[qa.stype >> 8], Rb = (float)R[qb.stype >> 8];
+                const float Ra = Rf(qa.stype >> 8), Rb = Rf(qb.stype >> 8);
                 const u64x2 a1 = threefry2x64_20(1, ca, seed, k1);
                 const u64x2 b1k = threefry2x64_20(1, cb, seed, k1);
                 term(qa, Ra, a1, ia);
@@ -288,7 +297,7 @@ struct PortfolioPolicyT {
                 const u64x2 a0 = threefry2x64_20(0, ca, seed, k1);
                 const uint32_t ia = pick_cdf(pf, 1, u01_f64(a0.a) * cp);
                 const MixRec qa = pf.mix[ia];
-                term(qa, (float)R[qa.stype >> 8], threefry2x64_20(1, ca, seed, k1), ia);
+                term(qa, Rf(qa.stype >> 8), threefry2x64_20(1, ca, seed, k1), ia);
             }
             s.nu += 2 * (u1 - u0);
             s.nn += 2 * (u1 - u0);
@@ -530,7 +539,11 @@ __global__ __launch_bounds__(64 * kOuterWaves) void k_pf_outer_mfma(typename P::
 #pragma unroll
         for (int i = 0; i < 4; i++) {
             const uint64_t m = m0 + 16 * mt + (lane >> 4) + 4 * i;
-            if (m < m_end) A.R_tau[m * rs + n] = simulate_stock_d(st.S0, st.mu, st.sigma, pf.tau, pf.stau * acc[i]);
+            if (m < m_end) {
+                const double v = simulate_stock_d(st.S0, st.mu, st.sigma, pf.tau, pf.stau * acc[i]);
+                A.R_tau[m * rs + n] = v;
+                if (A.R32) A.R32[m * rs + n] = (float)v;
+            }
         }
     }
 }
@@ -555,7 +568,7 @@ hipError_t PortfolioPolicyT<EXACT_ONLY, MULTI>::launch_outer(const Args& A, hipS
 template <bool EXACT_ONLY, bool MULTI>
 static hipError_t launch_kind(const PortfolioBatchedHost& H, hipStream_t s) {
     using P = PortfolioPolicyT<EXACT_ONLY, MULTI>;
-    typename P::Args A{H.c, H.pf, H.R_tau, H.bs_work, H.bt};
+    typename P::Args A{H.c, H.pf, H.R_tau, H.bs_work, H.bt, H.R32};
     return H.mixed ? launch_batched_level<P, true>(A, H.blocks, s) : launch_batched_level<P, false>(A, H.blocks, s);
 }
 

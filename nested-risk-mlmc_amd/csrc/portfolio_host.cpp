@@ -278,7 +278,8 @@ int Engine::sample_portfolio_parity(uint32_t ell, uint64_t M, uint32_t, sample_o
     return set_error(NMLMC_OK, "");
 }
 
-// Stock states at tau of M outer samples (batched forms; grown, never shrunk).
+// Stock states at tau of M outer samples (batched forms; grown, never shrunk):
+// cap FP64 values, then cap FP32 copies of them for the mixed Exact loop.
 int Engine::ensure_pf_R(uint64_t M) { return ensure_R((size_t)M * pf.d); }
 
 int Engine::ensure_R(size_t need) {
@@ -286,10 +287,12 @@ int Engine::ensure_R(size_t need) {
     size_t cap = pf_R_cap ? pf_R_cap : 1024;
     if (pf_R) hipFree(pf_R);
     pf_R = nullptr;
+    pf_R32 = nullptr;
     pf_R_cap = 0;
     while (cap < need) cap *= 2;
-    const hipError_t e = hipMalloc(&pf_R, sizeof(double) * cap);
+    const hipError_t e = hipMalloc(&pf_R, (sizeof(double) + sizeof(float)) * cap);
     if (e != hipSuccess) return hip_fail(e, "hipMalloc outer states");
+    pf_R32 = reinterpret_cast<float*>(pf_R + cap);
     pf_R_cap = cap;
     return NMLMC_OK;
 }
@@ -301,7 +304,7 @@ int Engine::sample_portfolio_batched(uint32_t ell, uint64_t M, uint64_t outer_of
         return set_error(NMLMC_ERR_ARG, "batched forms support the Indicator and Max phi types");
     if (pf.chol && pf.d > 256)
         return set_error(NMLMC_ERR_ARG, "batched forms take a correlation factor of at most 256 stocks");
-    PortfolioBatchedHost H;
+    PortfolioBatchedHost H{};
     int rc = batched_common(ell, M, outer_offset, H.c);
     if (rc) return rc;
     H.pf = pf;
@@ -315,6 +318,8 @@ int Engine::sample_portfolio_batched(uint32_t ell, uint64_t M, uint64_t outer_of
     H.blocks = (uint32_t)device_cus() * sh.bpc;
     H.mixed = form == NMLMC_FORM_MIXED;
     H.exact_only = pf.subsample == 1 && pf.nopt[0] == 0 && pf.nopt[2] == 0;
+    H.multi = false;
+    H.R32 = (H.mixed && H.exact_only) ? pf_R32 : nullptr;
     last_kernel = "k_b_final";
     return run_batched(ell, M, H.c, out, [&]() { return launch_portfolio_batched(H, stream); }, tr_phi_f,
                        tr_phi_c, tr_nf, tr_nc);
