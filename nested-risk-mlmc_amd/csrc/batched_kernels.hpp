@@ -990,7 +990,9 @@ hipError_t launch_batched_level(const typename P::Args& A0, uint32_t blocks, hip
     hipLaunchKernelGGL((k_b_final<P, MIXED>), dim3(blocks), dim3(256), 0, s, A);
     if (MIXED) {
         hipLaunchKernelGGL((k_b_redo_split<P>), dim3(256, kRedoParts), dim3(256), 0, s, A);
-        hipLaunchKernelGGL((k_b_redo_seq<P>), dim3(256), dim3(256), 0, s, A);
+        // The sequential redo has work only for two-valued groups or past the
+        // split's slots; a smaller grid dispatches faster (its blocks loop).
+        hipLaunchKernelGGL((k_b_redo_seq<P>), dim3(P::kTwoValued ? 256 : 32), dim3(256), 0, s, A);
     }
     return hipGetLastError();
 }

@@ -529,10 +529,26 @@ __global__ __launch_bounds__(64 * kOuterWaves) void k_pf_outer_mfma(typename P::
         const uint32_t n_hi = 16 * (t + 1) < d ? 16 * (t + 1) : d;
         const uint32_t s_end = (n_hi + 3) / 4;  // columns k < n_hi
         f64x4 acc = {0., 0., 0., 0.};
-        for (uint32_t s = 0; s < s_end; s++) {
+        // The elements of L for steps s .. s + 3 are loaded before the products
+        // of the previous four wait on them (a level with few outer samples has
+        // no other blocks to hide the loads); the products keep their order.
+        // Steps past s_end load nothing and multiply nothing (s_end is
+        // wave-uniform).
+        auto ldb = [&](uint32_t s) -> double {
             const uint32_t k = 4 * s + kq;
-            const double b = (n < d && k <= n) ? pf.chol[(size_t)n * d + k] : 0.;
-            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(zt[k * 64 + ar], b, acc, 0, 0, 0);
+            return (s < s_end && n < d && k <= n) ? pf.chol[(size_t)n * d + k] : 0.;
+        };
+        double b0 = ldb(0), b1 = ldb(1), b2 = ldb(2), b3 = ldb(3);
+        for (uint32_t s = 0; s < s_end; s += 4) {
+            const double c0 = b0, c1 = b1, c2 = b2, c3 = b3;
+            b0 = ldb(s + 4);
+            b1 = ldb(s + 5);
+            b2 = ldb(s + 6);
+            b3 = ldb(s + 7);
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(zt[(4 * s + kq) * 64 + ar], c0, acc, 0, 0, 0);
+            if (s + 1 < s_end) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(zt[(4 * (s + 1) + kq) * 64 + ar], c1, acc, 0, 0, 0);
+            if (s + 2 < s_end) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(zt[(4 * (s + 2) + kq) * 64 + ar], c2, acc, 0, 0, 0);
+            if (s + 3 < s_end) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(zt[(4 * (s + 3) + kq) * 64 + ar], c3, acc, 0, 0, 0);
         }
         if (n >= d) continue;
         const StockRec st = pf.stocks[n];
