@@ -173,8 +173,9 @@ __global__ __launch_bounds__(256, (mixed_waves<P, MIXED>())) void k_b_test(typen
     const uint64_t cur_f = c.cf.base << r, cur_c = c.cc.base << r;
     const uint64_t uf = (cur_f + P::spu - 1) / P::spu, uc = (cur_c + P::spu - 1) / P::spu;
     // Units per segment.  The recheck lists hold few groups: their units are
-    // spread over the whole grid (at least 4 units a lane, at most c.S), so a
-    // group's FP64 pass takes a few samples' latency instead of c.S samples'.
+    // spread over the whole grid (one unit a lane while lanes last, at most
+    // c.S), so a group's FP64 pass takes about one sample's latency instead of
+    // c.S samples'.
     // A round whose units come to less than two segments of c.S per lane (the
     // later rounds test only the refining outer samples; round 0 of a level
     // with few outer samples) takes smaller segments, at least c.s_test_min
@@ -188,7 +189,7 @@ __global__ __launch_bounds__(256, (mixed_waves<P, MIXED>())) void k_b_test(typen
         const uint64_t units_all = (uint64_t)nfe * uf + (uint64_t)nce * uc;
         if (RECHECK) {
             S = (units_all + lanes - 1) / lanes;
-            S = S < 4 ? 4 : S;
+            S = S < 1 ? 1 : S;
         } else if (c.s_test_min && units_all < 2 * lanes * c.S) {
             S = (units_all + 2 * lanes - 1) / (2 * lanes);
             S = S < c.s_test_min ? c.s_test_min : S;

@@ -27,7 +27,7 @@ struct StockRec {
 // One Exact option for the FP32 inner loop of the mixed form (16 bytes).
 struct MixRec {
     float K;         // strike
-    float Tt;        // T - tau
+    float sTt;       // sqrt(T - tau)
     float discp;     // w e^{-rT} / p, p = prob / prob_norm
     uint32_t stype;  // stock index << 8 | (type + 2)
 };
@@ -40,7 +40,8 @@ struct MixStock {
     float sigtau;  // sigma sqrt(tau)
     float S0e;     // S0 e^{(r - sigma^2 / 2) tau}
     float S0;      // S0
-    float pad[3];
+    float invS0;   // 1 / S0 (the control variate's S / S0 as a product)
+    float pad[2];
 };
 
 constexpr uint32_t kGuideLo = 0x7fffffffu, kGuideMulti = 0x80000000u;

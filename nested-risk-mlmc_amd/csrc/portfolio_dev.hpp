@@ -519,8 +519,9 @@ __device__ __forceinline__ float inner_exact_f(const PortfolioDevice& pf, const 
     if (cv) {
         const float diff = R - S0;
         const float c = (float)(o.type == 1);
-        dv1 = diff * disc * (S1 / S0) * (c - (float)(S1 < K));
-        dv2 = anti ? diff * disc * (S2 / S0) * (c - (float)(S2 < K)) : dv1;
+        const float iS0 = __builtin_amdgcn_rcpf(S0);  // within an ulp or so: inside the term's error scale
+        dv1 = diff * disc * (S1 * iS0) * (c - (float)(S1 < K));
+        dv2 = anti ? diff * disc * (S2 * iS0) * (c - (float)(S2 < K)) : dv1;
     }
     // Away from the strike every quantity above is a continuous function of
     // the draws and of R: the stock levels carry a relative error of about
@@ -551,7 +552,8 @@ __device__ __forceinline__ float inner_exact_f(const PortfolioDevice& pf, const 
 __device__ __forceinline__ float inner_exact_mix(const MixRec& q, const MixStock& st, bool anti, bool cv_on,
                                                  float R, float z0, float z1, float& mag) {
     const int type = (int)(q.stype & 0xffu) - 2;
-    const float gT = __expf(st.sig * __fsqrt_rn(q.Tt) * z1 + st.dcoef * q.Tt);
+    // T - tau as the square of the stored root (within an ulp or two of it).
+    const float gT = __expf(st.sig * q.sTt * z1 + st.dcoef * (q.sTt * q.sTt));
     const float S1 = st.S0e * __expf(st.sigtau * z0) * gT;
     const float S2 = anti ? st.S0e * __expf(-st.sigtau * z0) * gT : S1;
     const float St = R * gT;
@@ -564,8 +566,8 @@ __device__ __forceinline__ float inner_exact_mix(const MixRec& q, const MixStock
     const float diff = R - st.S0;
     if (cv) {
         const float c = (float)(type == 1);
-        dv1 = diff * q.discp * __fdividef(S1, st.S0) * (c - (float)(S1 < q.K));
-        dv2 = anti ? diff * q.discp * __fdividef(S2, st.S0) * (c - (float)(S2 < q.K)) : dv1;
+        dv1 = diff * q.discp * (S1 * st.invS0) * (c - (float)(S1 < q.K));
+        dv2 = anti ? diff * q.discp * (S2 * st.invS0) * (c - (float)(S2 < q.K)) : dv1;
     }
     mag = 0.5f * (fabsf(val1) + fabsf(val2) + fabsf(dv1) + fabsf(dv2)) + fabsf(q.discp) * (S1 + S2 + St);
     const float win = 0x1p-17f * q.K;
@@ -574,7 +576,7 @@ __device__ __forceinline__ float inner_exact_mix(const MixRec& q, const MixStock
         const float at_risk = (float)near1 + (float)near2 + 2.f * (float)neart;
         mag += 0x1p18f * 0.5f * fabsf(q.discp) * at_risk;
     } else if (cv) {
-        const float jump = fabsf(diff * q.discp) * __fdividef(q.K, st.S0) * 1.001f;
+        const float jump = fabsf(diff * q.discp) * (q.K * st.invS0) * 1.001f;
         mag += 0x1p18f * 0.5f * jump * ((float)near1 + (float)near2);
     }
     return 0.5f * (val1 + val2 + dv1 + dv2);

@@ -193,7 +193,7 @@ int Engine::stage_portfolio(const portfolio_data* data) {
             const option_params& o = ex[j];
             MixRec& q = mix[j];
             q.K = (float)o.strike;
-            q.Tt = (float)(o.maturity - data->tau);
+            q.sTt = (float)std::sqrt(o.maturity - data->tau);
             q.discp = (float)(o.w * std::exp(-data->r * o.maturity) * P.prob_norm / o.prob);
             q.stype = (o.stock_ind << 8) | (uint32_t)(o.type + 2);
         }
@@ -209,6 +209,7 @@ int Engine::stage_portfolio(const portfolio_data* data) {
             q.sigtau = (float)(st.sigma * std::sqrt(data->tau));
             q.S0e = (float)(st.S0 * std::exp(dcoef * data->tau));
             q.S0 = (float)st.S0;
+            q.invS0 = (float)(1. / st.S0);
         }
         if ((rc = upload(*this, ms.data(), d, &P.mix_stocks))) return rc;
     }
