@@ -508,9 +508,9 @@ __device__ __forceinline__ void redo_finish(const BatchedCommon& c, uint64_t m, 
 // FP64 redo of the outer samples listed by the mixed final phase, from the same
 // draws (their work and draw counts were taken in the FP32 pass).  An outer
 // sample with a redo slot (the first kRedoSlots listed, at most kRedoGroups
-// final groups, antithetic or level 0) is split by k_b_redo_split over the
-// kRedoParts blocks of its grid column: lanes add their groups' FP64 sums into
-// the slot, and the block completing the slot's count forms phi.  The others
+// final groups, antithetic or level 0) is split by k_b_redo_split over up to
+// kRedoParts blocks: lanes add their groups' FP64 sums into the slot, and the
+// block completing the slot's count forms phi.  The others
 // are summed by one block each in k_b_redo_seq, group after group.
 // Two-valued policies' groups have two sums: they take the sequential redo.
 template <class P>
@@ -528,21 +528,26 @@ __global__ __launch_bounds__(256) void k_b_redo_split(typename P::Args A) {
     // serialise on its L2 channel).
     __shared__ double gsum[kRedoGroups];
     const uint32_t n = aload_u32(&c.ctl->nreeval);
-    const uint32_t part = blockIdx.y;
-    for (uint32_t w = blockIdx.x; w < n; w += gridDim.x) {
+    // Blocks per listed outer sample: one per blockDim.x units of the level's
+    // largest final (its fine or coarse count cap), at most gridDim.y.  The
+    // grid's blocks are dealt out that many to a sample, so that the low
+    // levels' small finals (thousands of listed samples) run side by side and
+    // the high levels' large ones (a few samples) split wide.  Every block
+    // computes the same count; the completing block waits for exactly these.
+    const uint64_t cmax = c.cf.max > c.cc.max ? c.cf.max : c.cc.max;
+    const uint64_t want = ((cmax + P::spu - 1) / P::spu + blockDim.x - 1) / blockDim.x;
+    const uint32_t parts = want < gridDim.y ? (uint32_t)(want ? want : 1) : gridDim.y;
+    const uint32_t b = blockIdx.y * gridDim.x + blockIdx.x;
+    const uint32_t cols = (gridDim.x * gridDim.y) / parts;
+    if (b >= cols * parts) return;
+    const uint32_t col = b / parts, part = b % parts;
+    for (uint32_t w = col; w < n; w += cols) {
         const uint32_t m = c.reeval[w];
         const ItemState& it = c.items[m];
         const FinalShape sh = final_shape(c, it);
         if (!redo_split<P>(c, w, sh)) continue;
         const uint64_t gu = sh.n / P::spu;  // units per group
         const uint64_t U = (uint64_t)sh.groups * gu;
-        // Blocks of the column this outer sample takes: one per blockDim.x
-        // units, at most gridDim.y; the others skip it (every block computes
-        // the same count from the same state, and the completing block waits
-        // for exactly these).
-        const uint64_t want = (U + blockDim.x - 1) / blockDim.x;
-        const uint32_t parts = want < gridDim.y ? (uint32_t)(want ? want : 1) : gridDim.y;
-        if (part >= parts) continue;
         const uint64_t p0 = U * part / parts, p1 = U * (part + 1) / parts;
         double* slot = c.redo_acc + (uint64_t)w * kRedoGroups;
         for (uint32_t g = threadIdx.x; g < kRedoGroups; g += blockDim.x) gsum[g] = 0.;
