@@ -75,7 +75,8 @@ struct BatchedCtl {
     uint32_t n_whole;  // outer samples whose final fits one segment (one lane, no atomics)
     uint32_t class_cnt[kMaxClasses], class_off[kMaxClasses + 1], class_cur[kMaxClasses];
     uint32_t final_total;
-    uint32_t nreeval;
+    uint32_t nreeval;      // outer samples listed for the split FP64 redo (reeval[0 ..])
+    uint32_t nreeval_seq;  // outer samples listed for the sequential redo (reeval[M - 1 ..] downwards)
     uint32_t overflow;
     uint32_t n_pend;  // fused first pass: outer samples left for the final phase
     uint32_t rc_f[kMaxRounds + 1], rc_c[kMaxRounds + 1];  // mixed: decisions of round r left to the FP64 recheck

@@ -514,9 +514,8 @@ __device__ __forceinline__ void redo_finish(const BatchedCommon& c, uint64_t m, 
 // are summed by one block each in k_b_redo_seq, group after group.
 // Two-valued policies' groups have two sums: they take the sequential redo.
 template <class P>
-__device__ __forceinline__ bool redo_split(const BatchedCommon& c, uint32_t w, const FinalShape& sh) {
-    return !P::kTwoValued && w < kRedoSlots && !(c.ell > 0 && !c.ep.antithetic) && sh.groups <= kRedoGroups &&
-           sh.n % P::spu == 0;
+__device__ __forceinline__ bool redo_split_shape(const BatchedCommon& c, const FinalShape& sh) {
+    return !P::kTwoValued && !(c.ell > 0 && !c.ep.antithetic) && sh.groups <= kRedoGroups && sh.n % P::spu == 0;
 }
 
 template <class P>
@@ -541,11 +540,11 @@ __global__ __launch_bounds__(256) void k_b_redo_split(typename P::Args A) {
     const uint32_t cols = (gridDim.x * gridDim.y) / parts;
     if (b >= cols * parts) return;
     const uint32_t col = b / parts, part = b % parts;
-    for (uint32_t w = col; w < n; w += cols) {
+    const uint32_t n_split = n < kRedoSlots ? n : kRedoSlots;
+    for (uint32_t w = col; w < n_split; w += cols) {
         const uint32_t m = c.reeval[w];
         const ItemState& it = c.items[m];
         const FinalShape sh = final_shape(c, it);
-        if (!redo_split<P>(c, w, sh)) continue;
         const uint64_t gu = sh.n / P::spu;  // units per group
         const uint64_t U = (uint64_t)sh.groups * gu;
         const uint64_t p0 = U * part / parts, p1 = U * (part + 1) / parts;
@@ -594,12 +593,14 @@ template <class P>
 __global__ __launch_bounds__(256) void k_b_redo_seq(typename P::Args A) {
     const BatchedCommon& c = A.c;
     __shared__ double red[8];
-    const uint32_t n = aload_u32(&c.ctl->nreeval);
-    for (uint32_t w = blockIdx.x; w < n; w += gridDim.x) {
-        const uint32_t m = c.reeval[w];
+    // Its list: the split list's entries past the slots, then the sequential
+    // list (filled from the buffer's end); usually both are empty.
+    const uint32_t n = aload_u32(&c.ctl->nreeval), ns = aload_u32(&c.ctl->nreeval_seq);
+    const uint32_t extra = n > kRedoSlots ? n - kRedoSlots : 0;
+    for (uint32_t q = blockIdx.x; q < extra + ns; q += gridDim.x) {
+        const uint32_t m = q < extra ? c.reeval[kRedoSlots + q] : c.reeval[c.M - 1 - (q - extra)];
         const ItemState& it = c.items[m];
         const FinalShape sh = final_shape(c, it);
-        if (redo_split<P>(c, w, sh)) continue;
         // FP64 sums of a draw group, block-wide: s1 and (two-valued) s1c.
         auto group_sum64 = [&](uint32_t tag, uint64_t i0, uint64_t i1, double* s1c_out) {
             SegAcc s;
@@ -683,16 +684,24 @@ __device__ __forceinline__ bool whole_final(const typename P::Args& A, uint32_t 
     return MIXED && near && c.phi_type == PHI_INDICATOR;
 }
 
-// Lists an outer sample for the FP64 redo (mixed form) and clears its redo
-// slot; every lane of the wave that reaches it must call it.
+// Lists an outer sample for the FP64 redo (mixed form): a final the split redo
+// can take goes to the split list (from the buffer's start) and its slot is
+// cleared; any other goes to the sequential list (from the buffer's end; the
+// two lists hold at most the chunk's M outer samples together).  Every lane of
+// the wave that reaches it must call it.
+template <class P>
 __device__ __forceinline__ void list_reeval(const BatchedCommon& c, bool reeval, uint32_t m) {
-    const uint32_t slot = wave_append(&c.ctl->nreeval, reeval);
-    if (reeval) {
+    const bool split = reeval && redo_split_shape<P>(c, final_shape(c, c.items[m]));
+    const uint32_t slot = wave_append(&c.ctl->nreeval, split);
+    const uint32_t sslot = wave_append(&c.ctl->nreeval_seq, reeval && !split);
+    if (split) {
         c.reeval[slot] = m;
         if (slot < kRedoSlots) {
             for (uint32_t g = 0; g < kRedoGroups; g++) astore(&c.redo_acc[(uint64_t)slot * kRedoGroups + g], 0.);
             __hip_atomic_exchange(&c.redo_cnt[slot], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
+    } else if (reeval) {
+        c.reeval[c.M - 1 - sslot] = m;
     }
 }
 
@@ -827,7 +836,7 @@ __global__ __launch_bounds__(256, (mixed_waves<P, MIXED>())) void k_b_final(type
                 }
             }
         }
-        if (MIXED) list_reeval(c, reeval, m_done);
+        if (MIXED) list_reeval<P>(c, reeval, m_done);
     }
     acc.template flush<true>(c);
 }
@@ -935,7 +944,7 @@ __global__ __launch_bounds__(256, (mixed_waves<P, MIXED>())) void k_b_fused(type
         if (app_c) c.tc[1][sc] = TestEntry{0., 0., 0., 0., (uint32_t)m, 0};
         if (pending) c.pend[sp] = (uint32_t)m;
         if (MIXED) {
-            list_reeval(c, reeval, (uint32_t)m);
+            list_reeval<P>(c, reeval, (uint32_t)m);
             const uint32_t rf = wave_append(&c.ctl->rc_f[0], rck_f);
             const uint32_t rc = wave_append(&c.ctl->rc_c[0], rck_c);
             if (rck_f) c.rq[0][rf] = TestEntry{0., 0., 0., 0., (uint32_t)m, 0};
@@ -996,9 +1005,7 @@ hipError_t launch_batched_level(const typename P::Args& A0, uint32_t blocks, hip
     hipLaunchKernelGGL((k_b_final<P, MIXED>), dim3(blocks), dim3(256), 0, s, A);
     if (MIXED) {
         hipLaunchKernelGGL((k_b_redo_split<P>), dim3(256, kRedoParts), dim3(256), 0, s, A);
-        // The sequential redo has work only for two-valued groups or past the
-        // split's slots; a smaller grid dispatches faster (its blocks loop).
-        hipLaunchKernelGGL((k_b_redo_seq<P>), dim3(P::kTwoValued ? 256 : 32), dim3(256), 0, s, A);
+        hipLaunchKernelGGL((k_b_redo_seq<P>), dim3(256), dim3(256), 0, s, A);
     }
     return hipGetLastError();
 }
