@@ -339,6 +339,42 @@ def book_sweep(nm, which, levels, M0, form, tune=True):
     nm.cleanup_nestedmc_sampler()
 
 
+def batch_sweep(nm, n, asian_frac, levels, M0, form):
+    """Only a level sweep of a batch of n books (configs.batch_books; with
+    asian_frac > 0 the Asians on level-coupled paths): levels 0..levels with
+    M_l = ceil(M0 2^-1.5l) outer samples per instance, one warm-up and one
+    measured sweep in the given form, for profiler and counter runs.  Prints
+    one JSON line."""
+    cfg = portfolio_configs(nm)
+    books = cfg.batch_books(n, asian_frac=asian_frac)
+    nm.init_nestedmc_sampler(1, 64, 2, 0.0, True)
+    nm.set_adaptive_rule(1, 4)
+    nm.set_form(form)
+    nm.set_level_coupled_paths(asian_frac > 0)
+    nm.batch_begin(n)
+    keep = []
+    for i, (a, L, sc, q) in enumerate(books):
+        pf = nm.Portfolio(a["S0"], a["mu"], a["sigma"], a["counts"], a["strike"], a["maturity"], a["w"],
+                          a["stock_ind"], a["type"], a["prob"], a["W_est"], **sc)
+        pf.comp_params()
+        keep.append(pf)
+        nm.batch_stage(i, pf, 1000 + i, L)
+    recs = []
+    for rep in range(2):
+        for ell in range(levels + 1):
+            M = level_M(M0, ell)
+            t0 = time.perf_counter()
+            g = nm.batch_sample(ell, [M] * n)
+            wall = time.perf_counter() - t0
+            if rep:
+                recs.append({"ell": ell, "M_per_instance": M, "kernel_ms": round(nm.last_kernel_ms(), 3),
+                             "wall_ms": round(1e3 * wall, 3), "inner": sum(int(x["stats"][0]) for x in g)})
+    print(json.dumps({"batch": n, "asian_frac": asian_frac, "coupled": asian_frac > 0, "form": form,
+                      "levels": recs}), flush=True)
+    nm.set_level_coupled_paths(False)
+    nm.cleanup_nestedmc_sampler()
+
+
 def portfolio_bench(nm, which, levels, M0, rel_eps, tte, plain_cores, tte_M0, tune=True, rel_eps2=None):
     """Portfolio books: per-level outer samples/s of the mixed and plain forms
     (N_base = 64 2^l, 3-sigma doubling capped at 2^4, M_l = ceil(M0 2^-1.5l)),
@@ -527,7 +563,22 @@ def main():
     ap.add_argument("--book-sweep", choices=["large", "small"],
                     help="only run one warm-up and one measured mixed-form sweep of that book "
                          "(autotuned launch shapes unless --no-autotune; for profiler and counter runs)")
+    ap.add_argument("--batch-sweep", choices=["exact", "asian"],
+                    help="only run one warm-up and one measured level sweep of --batch-sweep-n books "
+                         "(Exact-only, or with 20%% Asians on level-coupled paths) in --batch-sweep-form; "
+                         "for profiler and counter runs")
+    ap.add_argument("--batch-sweep-n", type=int, default=16)
+    ap.add_argument("--batch-sweep-levels", type=int, default=11)
+    ap.add_argument("--batch-sweep-M0", type=float, default=2048)
+    ap.add_argument("--batch-sweep-form", choices=["mixed", "batched"], default="mixed")
     args = ap.parse_args()
+    if args.batch_sweep:
+        import __graft_entry__
+        nm = __graft_entry__.package()
+        nm.load()
+        batch_sweep(nm, args.batch_sweep_n, 0.2 if args.batch_sweep == "asian" else 0.0, args.batch_sweep_levels,
+                    args.batch_sweep_M0, nm.FORM_MIXED if args.batch_sweep_form == "mixed" else nm.FORM_BATCHED)
+        return
     if args.copy_calibration:
         copy_calibration()
         return

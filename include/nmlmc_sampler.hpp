@@ -10,6 +10,7 @@
  *   static uint32_t GetWorkspaceSize(const data&);   reals of workspace per sampler
  *   SAMPLER(void* workspace);
  *   void sample_outer(uint64_t ell, work_t&);         the outer draw
+ *   void copy_outer(const SAMPLER&);                  take another sampler's outer state
  *   real_t init_inner_sampler(uint64_t& count, work_t&);  the inner mean's offset
  *   real_t sample_inner(uint64_t i, uint64_t count, work_t&);  one inner sample
  *   static void add_phi(real_t avg, real_t coeff, real_t* phi);
@@ -30,9 +31,12 @@
  * in the order the sampler makes them: a sample's draws are a pure function of
  * its key and counter, so any lane can compute any sample.
  *
+ * Inner samples run, as in the reference's child grids (nested_sampler.hpp:
+ * 296-298), on a sampler constructed on the outer sample's workspace that took
+ * the outer state with copy_outer from the sampler that drew it.
+ *
  * Restrictions of the batched engine: one scalar phi (gc_phi_size = 1); the
- * sampler's state after sample_outer lives in its members and its workspace
- * (copied per inner sample, so members must be trivially copyable);
+ * sampler's state after sample_outer lives in its members and its workspace;
  * init_inner_sampler must be a deterministic function of that state (its value
  * and its work are taken once per outer sample, as both reference samplers
  * allow); sample_inner is called with the sample's index within its draw group
@@ -129,7 +133,8 @@ struct SamplerPolicy {
     __device__ static void units(const Args& A, uint32_t m, const ItemState&, uint32_t tag, uint64_t u0,
                                  uint64_t u1, uint64_t i0, uint64_t i1, SegAcc& s) {
         static_assert(!F32, "sampler classes run in the FP64 form");
-        S so = *object(A, m);
+        S so(A.store + (uint64_t)m * A.words + A.obj_words);
+        so.copy_outer(*object(A, m));
         const uint64_t k1 = key1(A, m);
         double a1 = 0., a2 = 0.;
         work_t w = 0.;

@@ -39,10 +39,13 @@ constexpr int kMaxClasses = 32;
 // the complement word) and the compensated sums add 2^-23.
 constexpr double kMixedBound = 0x1p-16;
 // FP64 redo of the mixed form: the first kRedoSlots listed outer samples with at
-// most kRedoGroups final groups are split over up to kRedoParts blocks each.
-// 128 groups cover every antithetic final of the capped rule up to a cap of 2^6
-// (N_f / N_c <= 2^7); a level call lists at most a few thousand outer samples.
-constexpr uint32_t kRedoSlots = 32768, kRedoGroups = 128, kRedoParts = 32;
+// most kRedoGroups final groups are split over up to kRedoParts blocks each
+// (one per 256 units of the level's largest final), in a grid of
+// kRedoGridX x kRedoGridY blocks.  128 groups cover every antithetic final of
+// the capped rule up to a cap of 2^6 (N_f / N_c <= 2^7); a level call lists at
+// most a few thousand outer samples.
+constexpr uint32_t kRedoSlots = 32768, kRedoGroups = 128, kRedoParts = 1024;
+constexpr uint32_t kRedoGridX = 256, kRedoGridY = 32;
 
 struct ItemState {
     double Y;        // model outer draw

@@ -528,14 +528,16 @@ __global__ __launch_bounds__(256) void k_b_redo_split(typename P::Args A) {
     __shared__ double gsum[kRedoGroups];
     const uint32_t n = aload_u32(&c.ctl->nreeval);
     // Blocks per listed outer sample: one per blockDim.x units of the level's
-    // largest final (its fine or coarse count cap), at most gridDim.y.  The
+    // largest final (its fine or coarse count cap), at most kRedoParts.  The
     // grid's blocks are dealt out that many to a sample, so that the low
     // levels' small finals (thousands of listed samples) run side by side and
-    // the high levels' large ones (a few samples) split wide.  Every block
-    // computes the same count; the completing block waits for exactly these.
+    // the high levels' large ones (a few samples of up to 10^6 inner samples)
+    // split wide.  Every block computes the same count; the completing block
+    // waits for exactly these.
     const uint64_t cmax = c.cf.max > c.cc.max ? c.cf.max : c.cc.max;
     const uint64_t want = ((cmax + P::spu - 1) / P::spu + blockDim.x - 1) / blockDim.x;
-    const uint32_t parts = want < gridDim.y ? (uint32_t)(want ? want : 1) : gridDim.y;
+    const uint64_t cap = (uint64_t)gridDim.x * gridDim.y < kRedoParts ? (uint64_t)gridDim.x * gridDim.y : kRedoParts;
+    const uint32_t parts = (uint32_t)(want < cap ? (want ? want : 1) : cap);
     const uint32_t b = blockIdx.y * gridDim.x + blockIdx.x;
     const uint32_t cols = (gridDim.x * gridDim.y) / parts;
     if (b >= cols * parts) return;
@@ -976,7 +978,10 @@ hipError_t launch_batched_level(const typename P::Args& A0, uint32_t blocks, hip
     hipError_t e = P::launch_outer(A, s);
     if (e != hipSuccess) return e;
     // Mixed form: the FP64 recheck of round r's uncertain decisions, before
-    // round r + 1 reads its queue.
+    // round r + 1 reads its queue.  Its grid is the level's: a larger one
+    // (measured at four times) cut the high levels' rechecked groups of
+    // 10^5..10^6 samples into so many segments that their hand-off atomics on
+    // the group's sums cost more than the shorter segments saved.
     auto recheck = [&](uint32_t r) {
         if (!MIXED) return;
         if (rule1) hipLaunchKernelGGL((k_b_test<P, false, 1, true>), dim3(blocks), dim3(256), 0, s, A, r);
@@ -1004,7 +1009,7 @@ hipError_t launch_batched_level(const typename P::Args& A0, uint32_t blocks, hip
     hipLaunchKernelGGL((k_b_final_fill<P>), dim3(blocks), dim3(256), 0, s, A);
     hipLaunchKernelGGL((k_b_final<P, MIXED>), dim3(blocks), dim3(256), 0, s, A);
     if (MIXED) {
-        hipLaunchKernelGGL((k_b_redo_split<P>), dim3(256, kRedoParts), dim3(256), 0, s, A);
+        hipLaunchKernelGGL((k_b_redo_split<P>), dim3(kRedoGridX, kRedoGridY), dim3(256), 0, s, A);
         hipLaunchKernelGGL((k_b_redo_seq<P>), dim3(256), dim3(256), 0, s, A);
     }
     return hipGetLastError();

@@ -25,6 +25,7 @@ public:
     static uint32_t GetWorkspaceSize(const data&) { return 0; }
     __device__ sampler(void*) : Y(NAN) {}
     __device__ void sample_outer(uint64_t, work_t&) { Y = randn_sync(); }
+    __device__ void copy_outer(const sampler& rhs) { Y = rhs.Y; }
     __device__ real_t init_inner_sampler(uint64_t&, work_t&) { return -gc_data.max_loss; }
     __device__ real_t sample_inner(uint64_t, uint64_t, work_t& total_work) {
         total_work++;
