@@ -275,6 +275,7 @@ struct BatchedCommon {
     int phi_type;
     uint32_t S;          // inner units per segment
     uint32_t s_test_min; // smallest segment of a sparse later test round (0: rounds keep S)
+    uint32_t redo_all;   // mixed form, tests: every Indicator outer sample takes the FP64 redo
     uint32_t nps;        // normals per inner sample (draw diagnostics)
     uint32_t ups;        // uniforms per inner sample (draw diagnostics)
     uint32_t rounds;     // test rounds launched

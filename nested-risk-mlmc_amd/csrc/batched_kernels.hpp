@@ -683,7 +683,7 @@ __device__ __forceinline__ bool whole_final(const typename P::Args& A, uint32_t 
             near |= fabs(it.start + s1_big / (double)sh.big) <= P::mixed_bound(sab_big / (double)sh.big, it.start);
         outer_phi<P>(c, it, s1_big, phi_small, 0., 0., phi_f, phi_c);
     }
-    return MIXED && near && c.phi_type == PHI_INDICATOR;
+    return MIXED && (near || c.redo_all) && c.phi_type == PHI_INDICATOR;
 }
 
 // Lists an outer sample for the FP64 redo (mixed form): a final the split redo
@@ -828,7 +828,7 @@ __global__ __launch_bounds__(256, (mixed_waves<P, MIXED>())) void k_b_final(type
                     const double mb = it.start + s1b / (double)sh.big;
                     is_near |= fabs(mb) <= P::mixed_bound(aread(&it.sab_big) / (double)sh.big, it.start);
                 }
-                if (MIXED && is_near && c.phi_type == PHI_INDICATOR) {
+                if (MIXED && (is_near || c.redo_all) && c.phi_type == PHI_INDICATOR) {
                     it.phi32_f = phi_f;
                     it.phi32_c = phi_c;
                     reeval = true;

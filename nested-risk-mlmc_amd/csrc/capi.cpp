@@ -419,6 +419,10 @@ int Engine::batched_common(uint32_t ell, uint64_t M, uint64_t outer_offset, Batc
     c.offset = outer_offset;
     c.ell = ell;
     c.s_test_min = test_segment_min(problem == PROBLEM_PORTFOLIO);
+    // Test hook: NMLMC_MIXED_REDO_ALL=1 sends every Indicator outer sample of
+    // the mixed form to the FP64 redo (more work, the same results), so that
+    // tests reach the redo kernels' wide and many-sample paths at will.
+    if (const char* env = getenv("NMLMC_MIXED_REDO_ALL")) c.redo_all = strtoul(env, nullptr, 10) ? 1u : 0u;
     auto rounds_of = [](const LevelCounts& lc) {
         uint32_t n = 0;
         for (uint64_t cur = lc.base; lc.max > 2 * cur; cur *= 2) n++;

@@ -237,9 +237,15 @@ def _cmp_mixed(g, o):
     np.testing.assert_allclose(g["diffsums"], o["diffsums"], rtol=RTOL, atol=1e-300)
 
 
-def test_mixed_form_agrees_with_fp64(nm, oracle_f64):
+@pytest.mark.parametrize("redo_all", [False, True])
+def test_mixed_form_agrees_with_fp64(nm, oracle_f64, redo_all, monkeypatch):
     """Mixed FP32+FP64 form against the FP64 oracle with the same keying:
-    refinement decisions, counts and indicators all equal the FP64 ones."""
+    refinement decisions, counts and indicators all equal the FP64 ones.  With
+    redo_all (NMLMC_MIXED_REDO_ALL) every outer sample takes the FP64 redo: at
+    level 0 twenty thousand small redos side by side, at level 5 finals of up
+    to 32768 units split over 128 blocks each."""
+    if redo_all:
+        monkeypatch.setenv("NMLMC_MIXED_REDO_ALL", "1")
     orc = _shift_setup(nm, oracle_f64, 1, nm.FORM_MIXED)
     for ell, M in ((0, 20000), (2, 5000), (5, 500)):
         g = nm.sample(ell, M, outer_offset=0, trace=True)
@@ -247,6 +253,8 @@ def test_mixed_form_agrees_with_fp64(nm, oracle_f64):
         reev, flips, rck = nm.mixed_stats()
         print("ell", ell, "re-evaluated", reev, "flips", flips, "rechecked decisions", rck, flush=True)
         _cmp_mixed(g, o)
+        if redo_all:
+            assert reev == M
     nm.cleanup_nestedmc_sampler()
     nm.set_form(nm.FORM_PLAIN)
 

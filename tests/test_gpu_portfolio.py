@@ -231,18 +231,27 @@ def test_asian_options_need_the_unbiased_method(nm):
     nm.cleanup_nestedmc_sampler()
 
 
-def test_mixed_portfolio_agrees_with_fp64(nm, oracle_f64):
+@pytest.mark.parametrize("redo_all", [False, True])
+def test_mixed_portfolio_agrees_with_fp64(nm, oracle_f64, redo_all, monkeypatch):
     """Mixed FP32+FP64 form on an Exact portfolio with digitals: refinement
-    decisions and indicators equal the FP64 ones."""
+    decisions and indicators equal the FP64 ones.  With redo_all
+    (NMLMC_MIXED_REDO_ALL) every outer sample takes the FP64 redo, up to finals
+    of 16384 inner samples split over 64 blocks at level 5."""
+    if redo_all:
+        monkeypatch.setenv("NMLMC_MIXED_REDO_ALL", "1")
     d = synthetic(Q=4, counts=(0, 200, 0), seed=21)
     scal = dict(SCAL, max_loss=0.005, subsample=1, use_delta_cv=True, use_antithetic_risk=True,
                 phi_type=0, phi_size=1)
     pf, o = _setup(nm, oracle_f64, d, 1, 32, 2, 0.0, scal, 13, form=2, rule=(1, 4))
-    for ell, M in ((0, 3000), (1, 1500), (3, 300)):
+    calls = ((0, 3000), (1, 1500), (3, 300)) + (((5, 40),) if redo_all else ())
+    for ell, M in calls:
         g = nm.sample(ell, M, outer_offset=0, trace=True)
         r = o.sample(ell, M, keying=1, outer_offset=0, trace=True)
-        print("ell", ell, "mixed stats (redone, flips, rechecked)", nm.mixed_stats(), flush=True)
+        reev, flips, rck = nm.mixed_stats()
+        print("ell", ell, "mixed stats (redone, flips, rechecked)", (reev, flips, rck), flush=True)
         _cmp_mixed(g, r)
+        if redo_all:
+            assert reev == M
     nm.cleanup_nestedmc_sampler()
     nm.set_form(nm.FORM_PLAIN)
 
@@ -370,8 +379,9 @@ def test_option_search_with_ties_and_tiny_probabilities(nm, oracle_f64, form):
     nm.set_form(nm.FORM_PLAIN)
 
 
-@pytest.mark.parametrize("form,segment", [(1, 0), (2, 0), (1, 3), (2, 3)])
-def test_level_coupled_paths_match_oracle(nm, oracle_f64, form, segment, monkeypatch):
+@pytest.mark.parametrize("form,segment,redo_all", [(1, 0, False), (2, 0, False), (1, 3, False), (2, 3, False),
+                                                  (2, 0, True)])
+def test_level_coupled_paths_match_oracle(nm, oracle_f64, form, segment, redo_all, monkeypatch):
     """Level-coupled Milstein paths for the method-2 options (Asians among them)
     beside Exact options: at level l an Asian inner sample runs
     8 2^floor(l/2) steps, and an antithetic final's coarse estimator takes the
@@ -381,6 +391,8 @@ def test_level_coupled_paths_match_oracle(nm, oracle_f64, form, segment, monkeyp
     their fine and coarse sums go through the cross-lane hand-off)."""
     if segment:
         monkeypatch.setenv("NMLMC_SEGMENT", str(segment))
+    if redo_all:  # every outer sample through the sequential FP64 redo (two-valued groups)
+        monkeypatch.setenv("NMLMC_MIXED_REDO_ALL", "1")
     d = _with_asians(51)
     scal = dict(SCAL, max_loss=0.01, subsample=1, use_delta_cv=True, use_antithetic_risk=True,
                 phi_type=1 if form == 1 else 0, phi_size=1)
@@ -398,6 +410,8 @@ def test_level_coupled_paths_match_oracle(nm, oracle_f64, form, segment, monkeyp
         r = o.sample(ell, M, keying=1, outer_offset=0, trace=True)
         print("level", ell, flush=True)
         _cmp_mixed(g, r, rtol=RTOL_UNBIASED) if form == 2 else _cmp(g, r, rtol=RTOL_UNBIASED)
+        if redo_all:
+            assert nm.mixed_stats()[0] == M
     nm.set_level_coupled_paths(False)
     nm.cleanup_nestedmc_sampler()
     nm.set_form(nm.FORM_PLAIN)
