@@ -310,14 +310,14 @@ def portfolio_configs(nm):
     return importlib.import_module(nm.__name__ + ".configs")
 
 
-def book_sweep(nm, which, levels, M0, form, tune=True):
+def book_sweep(nm, which, levels, M0, form, tune=True, cores=1):
     """Only a book's sweep (levels 0..levels, M_l = ceil(M0 2^-1.5l)):
     generation, launch-shape autotuning (as the portfolio leg does; default
     shapes with tune false), one warm-up sweep and one measured sweep in the
     given form, for profiler and counter runs.  Prints one JSON line."""
     cfg = portfolio_configs(nm)
     a, L, sc = cfg.large_portfolio() if which == "large" else cfg.small_portfolio()
-    nm.init_nestedmc_sampler(1, 64, 2, 0.0, True)
+    nm.init_nestedmc_sampler(cores, 64, 2, 0.0, True)
     nm.set_adaptive_rule(1, 4)
     nm.set_form(form)
     pf = nm.Portfolio(a["S0"], a["mu"], a["sigma"], a["counts"], a["strike"], a["maturity"], a["w"],
@@ -325,7 +325,7 @@ def book_sweep(nm, which, levels, M0, form, tune=True):
     pf.comp_params()
     nm.portfolio_init(pf, 1)
     nm.portfolio_set_correlation(L)
-    shapes = autotune(nm, M0, levels) if tune else None
+    shapes = autotune(nm, M0, levels) if (tune and form != nm.FORM_PLAIN) else None
     sweep(nm, M0, levels)
     recs = []
     t0 = time.perf_counter()
@@ -560,8 +560,10 @@ def main():
     ap.add_argument("--no-small", action="store_true", help="skip the small book (config 3)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (for scaling rocprofv3 byte counters)")
+    ap.add_argument("--book-sweep-form", choices=["mixed", "plain"], default="mixed",
+                    help="form of --book-sweep (plain: --plain-cores virtual threads)")
     ap.add_argument("--book-sweep", choices=["large", "small"],
-                    help="only run one warm-up and one measured mixed-form sweep of that book "
+                    help="only run one warm-up and one measured sweep of that book in --book-sweep-form "
                          "(autotuned launch shapes unless --no-autotune; for profiler and counter runs)")
     ap.add_argument("--batch-sweep", choices=["exact", "asian"],
                     help="only run one warm-up and one measured level sweep of --batch-sweep-n books "
@@ -586,7 +588,9 @@ def main():
         import __graft_entry__
         nm = __graft_entry__.package()
         nm.load()
-        book_sweep(nm, args.book_sweep, args.pf_levels, args.pf_M0, nm.FORM_MIXED, tune=not args.no_autotune)
+        plain = args.book_sweep_form == "plain"
+        book_sweep(nm, args.book_sweep, args.pf_levels, args.pf_M0, nm.FORM_PLAIN if plain else nm.FORM_MIXED,
+                   tune=not args.no_autotune, cores=args.plain_cores if plain else 1)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the outer samples "

@@ -285,7 +285,7 @@ int Engine::finish_call(uint32_t ell, sample_output* out, const Partial* result)
 int Engine::sample_model_parity(uint32_t ell, uint64_t M, uint32_t lanes_override,
                                 sample_output* out, double* tr_phi_f, double* tr_phi_c,
                                 uint64_t* tr_nf, uint64_t* tr_nc) {
-    ModelParityArgsHost H;
+    ModelParityArgsHost H{};
     H.mp = mp;
     H.ep = ep;
     H.cf = level_counts(ep, ell);
@@ -816,7 +816,7 @@ int Engine::sample_batched(uint32_t ell, uint64_t M, uint64_t outer_offset, uint
                            uint64_t* tr_nc) {
     if (problem != PROBLEM_MODEL) return sample_portfolio_batched(ell, M, outer_offset, lanes_override, out,
                                                                    tr_phi_f, tr_phi_c, tr_nf, tr_nc);
-    ModelBatchedHost H;
+    ModelBatchedHost H{};
     int rc = batched_common(ell, M, outer_offset, H.c);
     if (rc) return rc;
     H.mp = mp;

@@ -242,7 +242,7 @@ int Engine::sample_portfolio_parity(uint32_t ell, uint64_t M, uint32_t, sample_o
                                     double* tr_phi_f, double* tr_phi_c, uint64_t* tr_nf, uint64_t* tr_nc) {
     if (pf.coupled)
         return set_error(NMLMC_ERR_STATE, "level-coupled paths run in the batched forms (nmlmc_set_form)");
-    PortfolioParityHost H;
+    PortfolioParityHost H{};
     H.pf = pf;
     H.ep = ep;
     H.cf = level_counts(ep, ell);
@@ -339,7 +339,7 @@ int Engine::sample_delta_unbiased(uint32_t option_index, uint64_t M, uint32_t mo
         return set_error(NMLMC_ERR_ARG, "moments must be 1..%d with output buffers", kMaxMoments);
     int rc = ensure_partials(cores);
     if (rc) return rc;
-    PortfolioDeltaHost H;
+    PortfolioDeltaHost H{};
     H.pf = pf;
     H.seed = seed;
     H.M = M;

@@ -12,7 +12,14 @@
 //     chunks of 64 samples and find their inner word offsets by a ballot
 //     prefix count, so every draw matches the sequential reference;
 //   - other configurations (no sub-sampling, Unbiased options, a cached inner
-//     normal) run the reference loop on lane 0.
+//     normal) run the reference loop on thread 0.
+// A virtual thread may take a block of W <= 4 waves (W > 1 only for books whose
+// options are all Exact, chosen when few virtual threads have outer samples):
+// the outer draw's stocks and the inner loop's samples are then spread over
+// the block's lanes.  With every option Exact, sample k draws method 1 from
+// sync word P_s + k whatever its value, and its option uniform and Box-Muller
+// pair from inner words P_i + 3k .. P_i + 3k + 2, so any lane can take any
+// sample; the lanes' sums are added wave by wave in a fixed order.
 #include "portfolio.hpp"
 #include "portfolio_dev.hpp"
 
@@ -33,6 +40,10 @@ struct PortfolioParityArgs {
     uint64_t* tr_nc;
 };
 
+// Doubles of LDS scratch a block needs beyond R_tau and z: the wave sums of a
+// block of up to 4 waves (3 each), or thread 0's broadcast (11 values).
+constexpr uint32_t kParityRed = 32;
+
 struct InnerSums {
     double s1, s2, work;
     uint64_t nu, nn;
@@ -43,13 +54,56 @@ __device__ __forceinline__ double phi_value(int phi_type, double u, double coeff
 }
 
 // inner_sampler::sample_inner (nested_sampler.hpp:131-161) over `count` samples
-// from the thread's streams (ss: sync, si: inner), wave-uniform result.
+// from the thread's streams (ss: sync, si: inner), block-uniform result.  red:
+// LDS scratch of at least 3 W doubles (W > 1).
+template <uint32_t W>
 __device__ InnerSums parity_inner(const PortfolioParityArgs& A, const double* R, uint64_t k1, StreamState& ss,
-                                  StreamState& si, uint64_t count) {
+                                  StreamState& si, uint64_t count, double* red) {
     const PortfolioDevice& pf = A.pf;
-    const uint32_t lane = threadIdx.x;
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     InnerSums r{0., 0., 0., 0, 0};
     const bool fast = pf.subsample != 0 && !(pf.compute_prob[2] > 0) && !si.is_saved;
+    if (W > 1 && fast) {
+        // Every option Exact (the only books given W > 1): method 1 for every
+        // sample, three inner words each.
+        double s1 = 0., s2 = 0., work = 0.;
+        for (uint64_t k = tid; k < count; k += 64 * W) {
+            const uint64_t q = si.P + 3 * k;
+            const double u = u01_f64(stream_word(A.seed, k1, q));
+            uint32_t idx;
+            double prob;
+            if (pf.subsample == -1) {
+                idx = (uint32_t)(ceil(u * (double)pf.nopt[1]) - 1);
+                prob = 1. / pf.options_count;
+            } else {
+                idx = pick_cdf(pf, 1, u * pf.compute_prob[1]);
+                prob = pf.options[1][idx].prob / pf.prob_norm;
+            }
+            const OptionRec o = pf.options[1][idx];
+            double x, y;
+            stream_bm_f64(A.seed, k1, q + 1, x, y);
+            const double v = inner_exact_d(pf, o, R, x, y, work) / prob;
+            s1 += v;
+            s2 += v * v;
+        }
+        const double a = wave_sum(s1), b = wave_sum(s2), c = wave_sum(work);
+        if (lane == 0) {
+            red[3 * wave] = a;
+            red[3 * wave + 1] = b;
+            red[3 * wave + 2] = c;
+        }
+        __syncthreads();
+        double t1 = 0., t2 = 0., tw = 0.;
+        for (uint32_t w = 0; w < W; w++) {
+            t1 += red[3 * w];
+            t2 += red[3 * w + 1];
+            tw += red[3 * w + 2];
+        }
+        __syncthreads();  // red is reused by the next call
+        ss.P += count;
+        si.P += 3 * count;
+        return InnerSums{t1, t2, tw, 2 * count, 2 * count};
+    }
     if (fast) {
         double s1 = 0., s2 = 0., work = 0.;
         uint64_t nn = 0;
@@ -99,8 +153,8 @@ __device__ InnerSums parity_inner(const PortfolioParityArgs& A, const double* R,
         si.P += base;
         return r;
     }
-    // Reference loop on lane 0.
-    if (lane == 0) {
+    // Reference loop on thread 0.
+    if (tid == 0) {
         DStream sy, in;
         sy.init(A.seed, k1, 0, ss.P, ss.saved, ss.is_saved);
         in.init(A.seed, k1, 0, si.P, si.saved, si.is_saved);
@@ -118,29 +172,58 @@ __device__ InnerSums parity_inner(const PortfolioParityArgs& A, const double* R,
         si.saved = in.saved;
         si.is_saved = in.is_saved;
     }
-    r.s1 = __shfl(r.s1, 0, 64);
-    r.s2 = __shfl(r.s2, 0, 64);
-    r.work = __shfl(r.work, 0, 64);
-    r.nu = __shfl(r.nu, 0, 64);
-    r.nn = __shfl(r.nn, 0, 64);
-    ss.P = __shfl(ss.P, 0, 64);
-    ss.saved = __shfl(ss.saved, 0, 64);
-    ss.is_saved = __shfl(ss.is_saved, 0, 64);
-    si.P = __shfl(si.P, 0, 64);
-    si.saved = __shfl(si.saved, 0, 64);
-    si.is_saved = __shfl(si.is_saved, 0, 64);
+    if (W == 1) {
+        r.s1 = __shfl(r.s1, 0, 64);
+        r.s2 = __shfl(r.s2, 0, 64);
+        r.work = __shfl(r.work, 0, 64);
+        r.nu = __shfl(r.nu, 0, 64);
+        r.nn = __shfl(r.nn, 0, 64);
+        ss.P = __shfl(ss.P, 0, 64);
+        ss.saved = __shfl(ss.saved, 0, 64);
+        ss.is_saved = __shfl(ss.is_saved, 0, 64);
+        si.P = __shfl(si.P, 0, 64);
+        si.saved = __shfl(si.saved, 0, 64);
+        si.is_saved = __shfl(si.is_saved, 0, 64);
+        return r;
+    }
+    // W > 1: thread 0's results to the whole block through LDS.
+    uint64_t* ru = reinterpret_cast<uint64_t*>(red);
+    if (tid == 0) {
+        red[0] = r.s1;
+        red[1] = r.s2;
+        red[2] = r.work;
+        ru[3] = r.nu;
+        ru[4] = r.nn;
+        ru[5] = ss.P;
+        red[6] = ss.saved;
+        ru[7] = ss.is_saved;
+        ru[8] = si.P;
+        red[9] = si.saved;
+        ru[10] = si.is_saved;
+    }
+    __syncthreads();
+    r = InnerSums{red[0], red[1], red[2], ru[3], ru[4]};
+    ss.P = ru[5];
+    ss.saved = red[6];
+    ss.is_saved = (uint32_t)ru[7];
+    si.P = ru[8];
+    si.saved = red[9];
+    si.is_saved = (uint32_t)ru[10];
+    __syncthreads();
     return r;
 }
 
-__global__ __launch_bounds__(64) void k_portfolio_parity(PortfolioParityArgs A) {
-    extern __shared__ double smem[];  // R_tau[d], z[d]
+template <uint32_t W>
+__global__ __launch_bounds__(64 * W) void k_portfolio_parity(PortfolioParityArgs A) {
+    extern __shared__ double smem[];  // R_tau[d], z[d], reduction scratch[kParityRed]
     const PortfolioDevice& pf = A.pf;
     const EngineParams& ep = A.ep;
-    const uint32_t t = blockIdx.x, lane = threadIdx.x;
+    const uint32_t t = blockIdx.x, tid = threadIdx.x;
     const uint64_t k1 = t;
     const uint32_t d = pf.d;
     double* R = smem;
     double* z = smem + d;
+    double* red = smem + 2 * d;
 
     uint64_t q = A.M / A.T, rr = A.M % A.T;
     if (t < rr) {
@@ -159,9 +242,9 @@ __global__ __launch_bounds__(64) void k_portfolio_parity(PortfolioParityArgs A) 
         // ---- portfolio::sampler::sample_outer ----
         uint64_t used;
         if (pf.chol) {
-            for (uint32_t j = lane; j < d; j += 64) z[j] = normal_at_f64(A.seed, k1, ss, j);
+            for (uint32_t j = tid; j < d; j += 64 * W) z[j] = normal_at_f64(A.seed, k1, ss, j);
             __syncthreads();
-            for (uint32_t i = lane; i < d; i += 64) {
+            for (uint32_t i = tid; i < d; i += 64 * W) {
                 double a = 0.;
                 for (uint32_t j = 0; j <= i; j++) a += pf.chol[(size_t)i * d + j] * z[j];
                 const StockRec st = pf.stocks[i];
@@ -171,14 +254,14 @@ __global__ __launch_bounds__(64) void k_portfolio_parity(PortfolioParityArgs A) 
         } else {
             const double dW_sys = pf.stau * normal_at_f64(A.seed, k1, ss, 0);
             if (d == 1) {
-                if (lane == 0) R[0] = simulate_stock_d(pf.stocks[0].S0, pf.stocks[0].mu, pf.stocks[0].sigma, pf.tau, dW_sys);
+                if (tid == 0) R[0] = simulate_stock_d(pf.stocks[0].S0, pf.stocks[0].mu, pf.stocks[0].sigma, pf.tau, dW_sys);
                 used = 1;
             } else {
-                for (uint32_t i = lane; i < d; i += 64) {
+                for (uint32_t i = tid; i < d; i += 64 * W) {
                     const double dW_ind = pf.stau * normal_at_f64(A.seed, k1, ss, 1 + i);
-                    const double W = pf.rho * dW_sys + pf.s1_rho2 * dW_ind;
+                    const double Wi = pf.rho * dW_sys + pf.s1_rho2 * dW_ind;
                     const StockRec st = pf.stocks[i];
-                    R[i] = simulate_stock_d(st.S0, st.mu, st.sigma, pf.tau, W);
+                    R[i] = simulate_stock_d(st.S0, st.mu, st.sigma, pf.tau, Wi);
                 }
                 used = 1 + d;
             }
@@ -196,7 +279,7 @@ __global__ __launch_bounds__(64) void k_portfolio_parity(PortfolioParityArgs A) 
         const double start = (pf.subsample == 0) ? det_bs_sum(pf, R, w_det) + constant : constant;
 
         auto test_round = [&](uint64_t cur) {
-            const InnerSums s = parity_inner(A, R, k1, ss, si, cur);
+            const InnerSums s = parity_inner<W>(A, R, k1, ss, si, cur, red);
             work += s.work + A.bs_work;
             stats[4] += s.nu;
             stats[5] += s.nn;
@@ -225,7 +308,7 @@ __global__ __launch_bounds__(64) void k_portfolio_parity(PortfolioParityArgs A) 
             return cur;
         };
         auto mean_of = [&](uint64_t n) {
-            const InnerSums s = parity_inner(A, R, k1, ss, si, n);
+            const InnerSums s = parity_inner<W>(A, R, k1, ss, si, n, red);
             work += s.work + A.bs_work;
             stats[4] += s.nu;
             stats[5] += s.nn;
@@ -267,7 +350,7 @@ __global__ __launch_bounds__(64) void k_portfolio_parity(PortfolioParityArgs A) 
             for (uint32_t i = 0; i < ps; i++) dphi[i] = f_phi[i] - c_phi[i];
             add_moments(dphi, ps, d_sums, ep.diff_moments);
         }
-        if (lane == 0) {
+        if (tid == 0) {
             if (A.tr_phi_f) A.tr_phi_f[m] = f_phi[0];
             if (A.tr_phi_c) A.tr_phi_c[m] = fine_only ? 0. : c_phi[0];
             if (A.tr_nf) A.tr_nf[m] = nf;
@@ -275,7 +358,7 @@ __global__ __launch_bounds__(64) void k_portfolio_parity(PortfolioParityArgs A) 
         }
         __syncthreads();
     }
-    if (lane == 0) {
+    if (tid == 0) {
         A.states[2 * t] = ss;
         A.states[2 * t + 1] = si;
         Partial& p = A.partials[t];
@@ -305,8 +388,29 @@ hipError_t launch_portfolio_parity(const PortfolioParityHost& H, hipStream_t s) 
     A.tr_phi_c = H.tr_phi_c;
     A.tr_nf = H.tr_nf;
     A.tr_nc = H.tr_nc;
-    const size_t lds = sizeof(double) * 2 * (size_t)H.pf.d;
-    hipLaunchKernelGGL(k_portfolio_parity, dim3(H.T), dim3(64), lds, s, A);
+    // Waves per virtual thread: more than one only for books whose options are
+    // all Exact, while the virtual threads with outer samples (min(M, T)) are
+    // too few to fill the device: up to 4, while they stay within 1536 waves
+    // (the kernel runs at one wave a SIMD; measured on the large book at 128
+    // and 362 outer samples: 2.6x and 1.3x faster at 4 waves, and slower once
+    // the waves outnumber the SIMDs; 8 waves spill).  NMLMC_PLAIN_WAVES overrides
+    // it (1, 2 or 4); per-sample results do not depend on it.
+    uint32_t W = 1;
+    const bool exact_only = H.pf.subsample != 0 && H.pf.nopt[0] == 0 && H.pf.nopt[2] == 0;
+    if (exact_only) {
+        const uint64_t act = H.M < (uint64_t)H.T ? H.M : (uint64_t)H.T;
+        while (W < 4 && act * (2 * W) <= 1536) W *= 2;
+        if (const char* env = getenv("NMLMC_PLAIN_WAVES")) {
+            const uint32_t v = (uint32_t)strtoul(env, nullptr, 10);
+            W = (v >= 4) ? 4 : (v >= 2) ? 2 : 1;
+        }
+    }
+    const size_t lds = sizeof(double) * (2 * (size_t)H.pf.d + kParityRed);
+    switch (W) {
+    case 4: hipLaunchKernelGGL(k_portfolio_parity<4>, dim3(H.T), dim3(256), lds, s, A); break;
+    case 2: hipLaunchKernelGGL(k_portfolio_parity<2>, dim3(H.T), dim3(128), lds, s, A); break;
+    default: hipLaunchKernelGGL(k_portfolio_parity<1>, dim3(H.T), dim3(64), lds, s, A); break;
+    }
     return hipGetLastError();
 }
 

@@ -354,6 +354,21 @@ def test_config_shaped_portfolios(nm, oracle_f64, form, which):
     nm.set_form(nm.FORM_PLAIN)
 
 
+@pytest.mark.parametrize("waves", [1, 2, 4])
+def test_plain_waves_per_virtual_thread(nm, oracle_f64, waves, monkeypatch):
+    """Plain form on a book whose options are all Exact, each virtual thread on
+    a block of 1, 2 or 4 waves (NMLMC_PLAIN_WAVES): per-sample results equal
+    the reference keying's (the samples' draws sit at known stream positions,
+    so the block's lanes split them)."""
+    monkeypatch.setenv("NMLMC_PLAIN_WAVES", str(waves))
+    cfg = _configs(nm)
+    a, L, sc = cfg.small_portfolio(P=2000, n_pilot=300)
+    pf, o = _setup(nm, oracle_f64, a, 16, 64, 2, 0.0, sc, 37, form=0, chol=L, rule=(1, 4))
+    for ell, M in ((0, 100), (2, 24), (4, 7)):
+        _cmp(nm.sample(ell, M, trace=True), o.sample(ell, M, trace=True))
+    nm.cleanup_nestedmc_sampler()
+
+
 @pytest.mark.parametrize("form", [0, 1])
 def test_option_search_with_ties_and_tiny_probabilities(nm, oracle_f64, form):
     """Sub-sampling probabilities spanning 16 decades, so that many exclusive-
