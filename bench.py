@@ -235,33 +235,42 @@ def cpu_config1(M0, levels=8):
     except OSError:
         pass
 
-    def sweep_cpu(fn, reps=2):
-        # The best of reps sweeps (per level and overall).
-        best, best_per = 0., None
-        for _ in range(reps):
-            per, total = [], 0
-            t0 = time.perf_counter()
-            for ell in range(levels + 1):
-                M = level_M(M0, ell)
-                t1 = time.perf_counter()
-                fn(ell, M)
-                per.append(round(M / (time.perf_counter() - t1), 1))
-                total += M
-            rate = round(total / (time.perf_counter() - t0), 1)
-            best_per = per if best_per is None else [max(a, b) for a, b in zip(best_per, per)]
-            best = max(best, rate)
-        return best, best_per
+    def sweep(fn):
+        per, total = [], 0
+        t0 = time.perf_counter()
+        for ell in range(levels + 1):
+            M = level_M(M0, ell)
+            t1 = time.perf_counter()
+            fn(ell, M)
+            per.append(round(M / (time.perf_counter() - t1), 1))
+            total += M
+        return round(total / (time.perf_counter() - t0), 1), per
 
-    # Each sweep continues the streams of the one before (the same work).
+    def best(runs):
+        # The best rate over the runs, overall and per level.
+        return (max(r for r, _ in runs),
+                [max(v) for v in zip(*(p for _, p in runs))])
+
+    # Each sweep continues the streams of the one before (the same work).  The
+    # two engines' sweeps alternate, so that a change of the host's load during
+    # the measurement falls on both.
     orc = Oracle("f64").new(threads, 32, 2, 0.0, True)
     orc.model_init(1, 0, 0.5, 2.0, 1)
-    port, port_per = sweep_cpu(lambda l, M: orc.sample(l, M))
-    out = {"cores": threads, "M0": M0, "levels": levels, "port": port, "port_per_level": port_per}
-    if ref_available("f64"):
+    have_ref = ref_available("f64")
+    if have_ref:
         ref = Ref("f64").init(threads, 32, 2, 0.0, True)
         ref.shift_init(2.0, 1)
-        rv, rper = sweep_cpu(lambda l, M: ref.shift_sample(l, M))
-        out.update(reference=rv, reference_per_level=rper, port_over_reference=round(port / rv, 3))
+    port_runs, ref_runs = [], []
+    for _ in range(5):
+        port_runs.append(sweep(lambda l, M: orc.sample(l, M)))
+        if have_ref:
+            ref_runs.append(sweep(lambda l, M: ref.shift_sample(l, M)))
+    port, port_per = best(port_runs)
+    out = {"cores": threads, "M0": M0, "levels": levels, "port": port, "port_per_level": port_per}
+    if have_ref:
+        rv, rper = best(ref_runs)
+        out.update(reference=rv, reference_per_level=rper, port_over_reference=round(port / rv, 3),
+                   ratio_per_run=[round(a / b, 3) for (a, _), (b, _) in zip(port_runs, ref_runs)])
     return out
 
 
@@ -695,7 +704,8 @@ def main():
             cpu["calibration"] = {"workload": "config1 sweep (non-adaptive shift model), same threads",
                                   "cores": c1["cores"], "port_outer_samples_per_s": c1["port"],
                                   "reference_outer_samples_per_s": c1["reference"],
-                                  "port_over_reference": c1["port_over_reference"]}
+                                  "port_over_reference": c1["port_over_reference"],
+                                  "ratio_per_run": c1["ratio_per_run"]}
 
     line = {
         "metric": "outer samples/s per level; time to RMSE eps: batched FP32+FP64 vs plain FP64",

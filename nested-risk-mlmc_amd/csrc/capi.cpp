@@ -92,7 +92,7 @@ int Engine::ensure_device() {
     if (e != hipSuccess) return hip_fail(e, "hipStreamCreate");
     if ((e = hipEventCreate(&ev0)) != hipSuccess) return hip_fail(e, "hipEventCreate");
     if ((e = hipEventCreate(&ev1)) != hipSuccess) return hip_fail(e, "hipEventCreate");
-    if ((e = hipMalloc(&d_out, sizeof(Partial))) != hipSuccess) return hip_fail(e, "hipMalloc");
+    if ((e = hipMalloc(&d_out, sizeof(Partial) * (1 + kReduceBlocks))) != hipSuccess) return hip_fail(e, "hipMalloc");
     if ((e = hipHostMalloc(&h_out, sizeof(Partial))) != hipSuccess) return hip_fail(e, "hipHostMalloc");
     // The no-allocation entry point's scratch address; if it cannot be taken,
     // that entry point reports it and everything else works on.

@@ -21,6 +21,9 @@ struct Partial {
     uint64_t stats[kStatsExt];
 };
 constexpr int kPartialFields = 2 * kMaxPhi * kMaxMoments + 1 + kStatsExt;
+// The reduced record of a level call is followed by this many staging records
+// of the two-pass reduction (launch_reduce_partials).
+constexpr uint32_t kReduceBlocks = 256;
 
 // get_samples_count's constructor state (nested_sampler.hpp:225-239), computed on
 // the host with the host libm so that tmp matches the reference bit for bit.

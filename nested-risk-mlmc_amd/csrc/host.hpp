@@ -50,14 +50,15 @@ struct Engine {
     StreamState* d_states = nullptr;  // [2 cores]: parity streams (sync, inner)
     Partial* d_partials = nullptr;    // [cores] or [batched groups]
     size_t partial_cap = 0;
-    Partial* d_out = nullptr;
+    Partial* d_out = nullptr;         // the reduced record, then kReduceBlocks staging records
     Partial* h_out = nullptr;
     double* d_ws = nullptr;           // per-group workspace (portfolio R_tau)
     size_t ws_cap = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // Scratch of a plain-form level call: the per-block partial records (room
-    // for scr_cap) and the reduced record.  NULL: the engine's own buffers,
+    // for scr_cap) and the reduced record, followed by kReduceBlocks staging
+    // records of the reduction.  NULL: the engine's own buffers,
     // grown on demand; the no-allocation entry point points them at its
     // static device array for the length of its call.
     Partial* scr_partials = nullptr;

@@ -57,6 +57,8 @@ hipError_t launch_model_batched(const ModelBatchedHost& H, hipStream_t s);
 inline uint64_t parity_blocks(uint64_t T, uint32_t G) { return (T * G + 255) / 256; }
 
 hipError_t launch_model_parity(const ModelParityArgsHost& H, hipStream_t s);
+// Reduces n Partial records into out[0] in a fixed order; out has room for
+// 1 + kReduceBlocks records (the rest are the reduction's staging records).
 hipError_t launch_reduce_partials(const Partial* P, uint32_t n, Partial* out, hipStream_t s);
 
 }  // namespace nmlmc

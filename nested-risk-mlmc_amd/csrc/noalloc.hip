@@ -4,14 +4,15 @@
 //
 // The plain form's level call needs scratch beyond its inputs and outputs: one
 // partial record per block of virtual threads (k_model_parity,
-// k_portfolio_parity) and the record the fixed-order reduction writes.  Here
-// they live in a file-scope device array of kNoAllocRecords + 1 records
-// (5.5 MB; each GPU the library runs on holds its own copy until the process
-// ends).  Each block writes its whole record and the reduction writes the
-// whole result, so nothing needs clearing between calls.  The array's device
-// address is taken once per device at set-up (noalloc_setup, from the
-// library's init) and kept; host code uses that address, never the array's
-// name.  Calls that overlap on one GPU share the array and race.
+// k_portfolio_parity), the record the fixed-order reduction writes and its
+// kReduceBlocks staging records.  Here they live in a file-scope device array
+// of kNoAllocRecords + 1 + kReduceBlocks records (5.6 MB; each GPU the library
+// runs on holds its own copy until the process ends).  Each block writes its
+// whole record and each pass of the reduction writes whole records, so nothing
+// needs clearing between calls.  The array's device address is taken once per
+// device at set-up (noalloc_setup, from the library's init) and kept; host code
+// uses that address, never the array's name.  Calls that overlap on one GPU
+// share the array and race.
 #include <hip/hip_runtime.h>
 
 #include "host.hpp"
@@ -21,7 +22,7 @@ namespace nmlmc {
 constexpr uint32_t kNoAllocRecords = 16384;  // 65536 virtual threads x 64 lanes / 256
 constexpr int kNoAllocDevices = 64;
 
-static __device__ __align__(256) Partial scratch[kNoAllocRecords + 1];
+static __device__ __align__(256) Partial scratch[kNoAllocRecords + 1 + kReduceBlocks];
 
 // The array's device address per device, from hipGetSymbolAddress.
 static Partial* g_scratch_addr[kNoAllocDevices] = {};

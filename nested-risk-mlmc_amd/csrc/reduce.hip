@@ -1,52 +1,85 @@
-// Fixed-order reduction of per-block Partial records into one record: one
-// 256-thread block per field, strided partial sums then an LDS tree, so the
-// result does not depend on the order in which blocks finished.
+// Fixed-order reduction of per-block Partial records into one record, in two
+// passes: kReduceBlocks blocks each reduce a contiguous range of the records
+// into a staging record, then one block reduces the staging records.  The
+// ranges depend only on the record count, so the result does not depend on the
+// order in which the producing blocks finished.
 #include "batched.hpp"
 #include "engine.hpp"
 
 namespace nmlmc {
 
+namespace {
+
+constexpr int kRedF = kPartialFields;           // 8-byte words of a record
+constexpr int kRedR = 256 / kRedF;              // records read side by side per block step
+constexpr int kRedSum = 2 * kMaxPhi * kMaxMoments + 1;  // leading FP64 words; then the u64 stats
+constexpr uint32_t kRedMinChunk = 16;
+
+static_assert(sizeof(Partial) == 8 * kPartialFields, "a Partial is kPartialFields 8-byte words");
+
+// Block b reduces records [b chunk, min(n, (b+1) chunk)) into out[b].  Thread
+// (word f, row j) sums word f of the range's records j, j + kRedR, ... in
+// that order (loads issued four steps ahead of the adds; the order of the adds
+// is the plain loop's); the rows are then added in row order.  A record's
+// words are contiguous, so each step reads kRedR whole records.
 __global__ __launch_bounds__(256) void k_reduce_partials(const Partial* __restrict__ P, uint32_t n,
-                                                         Partial* out) {
-    __shared__ double sd[256];
-    __shared__ uint64_t su[256];
-    constexpr int nsum = 2 * kMaxPhi * kMaxMoments;
-    const int field = blockIdx.x;  // sums, then work, then the stats
-    const int tid = threadIdx.x;
-    if (field <= nsum) {
-        double acc = 0.;
-        for (uint32_t t = tid; t < n; t += 256) {
-            const Partial& p = P[t];
-            acc += field < kMaxPhi * kMaxMoments ? p.f[field]
-                   : (field < nsum ? p.d[field - kMaxPhi * kMaxMoments] : p.work);
+                                                         uint32_t chunk, Partial* __restrict__ out) {
+    __shared__ uint64_t acc_s[kRedR][kRedF];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lo = blockIdx.x * chunk;
+    const uint32_t hi = min(n, lo + chunk);
+    const uint64_t* W = reinterpret_cast<const uint64_t*>(P);
+    if (tid < (uint32_t)(kRedR * kRedF)) {
+        const uint32_t f = tid % kRedF, j = tid / kRedF;
+        const bool fp = f < (uint32_t)kRedSum;
+        double ad = 0.;
+        uint64_t au = 0;
+        for (uint32_t r = lo + j; r < hi; r += 4 * kRedR) {
+            uint64_t v[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const uint32_t rk = r + k * kRedR;
+                v[k] = rk < hi ? W[(size_t)rk * kRedF + f] : 0ull;
+            }
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                if (r + k * kRedR < hi) {
+                    if (fp) ad += __longlong_as_double((long long)v[k]);
+                    else au += v[k];
+                }
+            }
         }
-        sd[tid] = acc;
-        __syncthreads();
-        for (int o = 128; o > 0; o >>= 1) {
-            if (tid < o) sd[tid] += sd[tid + o];
-            __syncthreads();
+        acc_s[j][f] = fp ? (uint64_t)__double_as_longlong(ad) : au;
+    }
+    __syncthreads();
+    if (tid < (uint32_t)kRedF) {
+        uint64_t* o = reinterpret_cast<uint64_t*>(out + blockIdx.x);
+        if (tid < (uint32_t)kRedSum) {
+            double a = 0.;
+            for (int j = 0; j < kRedR; j++) a += __longlong_as_double((long long)acc_s[j][tid]);
+            o[tid] = (uint64_t)__double_as_longlong(a);
+        } else {
+            uint64_t a = 0;
+            for (int j = 0; j < kRedR; j++) a += acc_s[j][tid];
+            o[tid] = a;
         }
-        if (tid == 0) {
-            if (field < kMaxPhi * kMaxMoments) out->f[field] = sd[0];
-            else if (field < nsum) out->d[field - kMaxPhi * kMaxMoments] = sd[0];
-            else out->work = sd[0];
-        }
-    } else {
-        const int k = field - nsum - 1;
-        uint64_t acc = 0;
-        for (uint32_t t = tid; t < n; t += 256) acc += P[t].stats[k];
-        su[tid] = acc;
-        __syncthreads();
-        for (int o = 128; o > 0; o >>= 1) {
-            if (tid < o) su[tid] += su[tid + o];
-            __syncthreads();
-        }
-        if (tid == 0) out->stats[k] = su[0];
     }
 }
 
+}  // namespace
+
 hipError_t launch_reduce_partials(const Partial* P, uint32_t n, Partial* out, hipStream_t s) {
-    hipLaunchKernelGGL(k_reduce_partials, dim3(kPartialFields), dim3(256), 0, s, P, n, out);
+    const uint32_t chunk = max(kRedMinChunk, (n + kReduceBlocks - 1) / kReduceBlocks);
+    const uint32_t nb = n ? (n + chunk - 1) / chunk : 1;
+    if (nb == 1) {
+        hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, s, P, n, chunk, out);
+        return hipGetLastError();
+    }
+    Partial* stage = out + 1;  // nb <= kReduceBlocks staging records after the result
+    hipLaunchKernelGGL(k_reduce_partials, dim3(nb), dim3(256), 0, s, P, n, chunk, stage);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, s, (const Partial*)stage, nb, nb, out);
     return hipGetLastError();
 }
 
