@@ -216,6 +216,9 @@ int Engine::batch_sample(uint32_t ell, const uint64_t* Ms, sample_output* outs) 
     H.R32 = (H.mixed && H.exact_only) ? pf_R32 : nullptr;
     H.c.phi_type = phi_type;
     H.c.n_inst = n;
+    // Every chunk's chain clears the control block and its accumulators first
+    // (zero_ctl, set by batched_common) and leaves them filled.
+    ctl_dirty = true;
     float ms_total = 0;
     for (size_t k = 0; k < nc; k++) {
         const uint64_t Mk = ((const uint64_t*)h_btab)[tab_off[k] + tabs[k].n_seg];

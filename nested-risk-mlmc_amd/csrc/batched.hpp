@@ -82,6 +82,7 @@ struct BatchedCtl {
     uint32_t nreeval_seq;  // outer samples listed for the sequential redo (reeval[M - 1 ..] downwards)
     uint32_t overflow;
     uint32_t n_pend;  // fused first pass: outer samples left for the final phase
+    uint32_t count_done;  // blocks of k_b_final_count that have counted
     uint32_t rc_f[kMaxRounds + 1], rc_c[kMaxRounds + 1];  // mixed: decisions of round r left to the FP64 recheck
     Partial acc;  // level sums, work and stats (atomics)
 };
@@ -294,6 +295,7 @@ struct BatchedCommon {
     BatchedCtl* ctl;
     Partial* inst_out;   // batch of problems: per-instance level accumulators (else NULL: ctl->acc)
     uint32_t n_inst;     // instances in the batch (0 for a single problem)
+    uint32_t zero_ctl;   // 1: the chain starts by clearing ctl (and inst_out); 0: ctl is clear already
     double* tr_phi_f;
     double* tr_phi_c;
     uint64_t* tr_nf;
@@ -388,9 +390,32 @@ struct BlockAcc {
 };
 
 // Defined once (reduce.hip): clears the control block (and a batch's
-// per-instance accumulators); turns the final-group
-// class counts into offsets and flags an over-full pool.
+// per-instance accumulators).
 __global__ void k_b_zero(BatchedCtl* ctl, Partial* inst_out, uint32_t n_inst);
-__global__ void k_b_final_alloc(BatchedCtl* ctl, uint32_t fcap);
+// Launches k_b_zero on the control block alone.
+hipError_t launch_clear_ctl(BatchedCtl* ctl, hipStream_t s);
+
+// Turns the final-group class counts into offsets and flags an over-full pool;
+// run by every thread of the block of k_b_final_count that counts last, after
+// every block's counts were acknowledged.  The counts are read with atomics,
+// one per thread side by side (read one after another they cost ~0.5 us each).
+static_assert(kMaxClasses + 1 <= 256, "one thread per class count");
+__device__ __forceinline__ void final_alloc(BatchedCtl* ctl, uint32_t fcap) {
+    __shared__ uint32_t cnt[kMaxClasses + 1];
+    const uint32_t t = threadIdx.x;
+    if (t <= (uint32_t)kMaxClasses) cnt[t] = aread_u32(t < (uint32_t)kMaxClasses ? &ctl->class_cnt[t] : &ctl->n_whole);
+    __syncthreads();
+    if (t != 0) return;
+    uint32_t off = 0;
+    for (int i = 0; i < kMaxClasses; i++) {
+        ctl->class_off[i] = off;
+        ctl->class_cur[i] = 0;
+        off += cnt[i];
+    }
+    ctl->class_off[kMaxClasses] = off;
+    ctl->final_total = off + cnt[kMaxClasses];
+    ctl->overflow = ctl->final_total > fcap;
+    ctl->n_whole = 0;  // refilled as a cursor by k_b_final_fill
+}
 
 }  // namespace nmlmc

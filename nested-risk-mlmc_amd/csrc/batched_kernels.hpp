@@ -2,9 +2,9 @@
 // policy P and on the precision of the inner arithmetic.
 //
 // Launch order of one level call (launch_batched_level):
-//   k_b_zero, [P's outer-draw kernel], k_b_init and k_b_test(r) for r = 0..R-1
-//   (or k_b_fused and k_b_test(r) for r = 1..R-1), k_b_final_count,
-//   k_b_final_alloc, k_b_final_fill, k_b_final, [k_b_redo_split and
+//   [k_b_zero unless the control block is clear already], [P's outer-draw kernel], k_b_init and k_b_test(r) for r = 0..R-1
+//   (or k_b_fused and k_b_test(r) for r = 1..R-1), k_b_final_count (its
+//   last block lays the final pool out), k_b_final_fill, k_b_final, [k_b_redo_split and
 //   k_b_redo_seq when MIXED]
 #pragma once
 #include "batched.hpp"
@@ -370,9 +370,17 @@ __global__ __launch_bounds__(256) void k_b_final_count(typename P::Args A) {
     }
     __syncthreads();
     for (int i = threadIdx.x; i < kMaxClasses; i += blockDim.x)
-        if (hist[i]) atomicAdd(&c.ctl->class_cnt[i], hist[i]);
-    if (threadIdx.x == 0 && hist[kMaxClasses]) atomicAdd(&c.ctl->n_whole, hist[kMaxClasses]);
+        if (hist[i]) acount(&c.ctl->class_cnt[i], hist[i]);
+    if (threadIdx.x == 0 && hist[kMaxClasses]) acount(&c.ctl->n_whole, hist[kMaxClasses]);
     acc.template flush<false>(c);
+    // The block that counts last lays the pool out (one launch fewer than a
+    // kernel of its own); its counts are acknowledged before it counts itself.
+    __shared__ uint32_t last;
+    wait_acks();
+    __syncthreads();
+    if (threadIdx.x == 0) last = acount(&c.ctl->count_done, 1u) == gridDim.x - 1;
+    __syncthreads();
+    if (last) final_alloc(c.ctl, c.fcap);
 }
 
 // Writes each outer sample's final entries: whole samples after the class
@@ -596,47 +604,54 @@ __global__ __launch_bounds__(256) void k_b_redo_seq(typename P::Args A) {
     const BatchedCommon& c = A.c;
     __shared__ double red[8];
     // Its list: the split list's entries past the slots, then the sequential
-    // list (filled from the buffer's end); usually both are empty.
+    // list (filled from the buffer's end); usually both are empty.  The draw
+    // groups are summed at one call site of P::units: a helper taking A by
+    // reference that the compiler does not inline makes it copy the kernel's
+    // argument block into scratch memory on every lane at entry, which costs
+    // more than the usual empty launch.
     const uint32_t n = aload_u32(&c.ctl->nreeval), ns = aload_u32(&c.ctl->nreeval_seq);
     const uint32_t extra = n > kRedoSlots ? n - kRedoSlots : 0;
     for (uint32_t q = blockIdx.x; q < extra + ns; q += gridDim.x) {
         const uint32_t m = q < extra ? c.reeval[kRedoSlots + q] : c.reeval[c.M - 1 - (q - extra)];
         const ItemState& it = c.items[m];
         const FinalShape sh = final_shape(c, it);
-        // FP64 sums of a draw group, block-wide: s1 and (two-valued) s1c.
-        auto group_sum64 = [&](uint32_t tag, uint64_t i0, uint64_t i1, double* s1c_out) {
-            SegAcc s;
+        // Unpaired fine and coarse means (two groups: the fine count, then the
+        // coarse count) or the antithetic draw groups of sh.n.
+        const bool unpaired = c.ell > 0 && !c.ep.antithetic;
+        const uint32_t groups = unpaired ? 2u : sh.groups;
+        const double coeff = (double)(1.f / (float)sh.groups);
+        double phi_f = 0., phi_c = 0., s1_big = 0., phi_small = 0.;
+        for (uint32_t g = 0; g < groups; g++) {
+            const uint32_t tag = (unpaired && g == 1) ? TAG_FINAL_COARSE : TAG_FINAL;
+            const uint64_t i0 = unpaired ? 0 : (uint64_t)g * sh.n;
+            const uint64_t i1 = unpaired ? (g == 0 ? it.nf : it.nc) : i0 + sh.n;
+            // FP64 sums of the group, block-wide: s1 and (two-valued) s1c.
+            SegAcc sa;
             const uint64_t ub = i0 / P::spu, ue = (i1 + P::spu - 1) / P::spu;
             for (uint64_t u = ub + threadIdx.x; u < ue; u += blockDim.x)
-                P::template units<false>(A, m, it, tag, u, u + 1, i0, i1, s);
-            const double v = wave_sum(s.s1);
-            const double vc = P::kTwoValued ? wave_sum(s.s1c) : 0.;
+                P::template units<false>(A, m, it, tag, u, u + 1, i0, i1, sa);
+            const double v = wave_sum(sa.s1);
+            const double vc = P::kTwoValued ? wave_sum(sa.s1c) : 0.;
             if ((threadIdx.x & 63) == 0) {
                 red[threadIdx.x >> 6] = v;
                 red[4 + (threadIdx.x >> 6)] = vc;
             }
             __syncthreads();
-            const double t = ((red[0] + red[1]) + red[2]) + red[3];
-            if (s1c_out) *s1c_out = P::kTwoValued ? ((red[4] + red[5]) + red[6]) + red[7] : t;
+            const double s1 = ((red[0] + red[1]) + red[2]) + red[3];
+            const double s1c = P::kTwoValued ? ((red[4] + red[5]) + red[6]) + red[7] : s1;
             __syncthreads();
-            return t;
-        };
-        double phi_f = 0., phi_c = 0.;
-        if (c.ell > 0 && !c.ep.antithetic) {
-            phi_f = P::phi(c, it.start + group_sum64(TAG_FINAL, 0, it.nf, nullptr) / (double)it.nf, 1., 0.);
-            phi_c = P::phi(c, it.start + group_sum64(TAG_FINAL_COARSE, 0, it.nc, nullptr) / (double)it.nc, 1., 0.);
-        } else {
-            const double coeff = (double)(1.f / (float)sh.groups);
-            double s1_big = 0., phi_small = 0.;
-            for (uint32_t g = 0; g < sh.groups; g++) {
-                double s1c, sb, ss;
-                const double s1 = group_sum64(TAG_FINAL, (uint64_t)g * sh.n, (uint64_t)(g + 1) * sh.n, &s1c);
+            if (unpaired) {
+                const double ph = P::phi(c, it.start + s1 / (double)(i1 - i0), 1., 0.);
+                if (g == 0) phi_f = ph;
+                else phi_c = ph;
+            } else {
+                double sb, ss;
                 side_sums<P>(c, sh, s1, s1c, sb, ss);
                 phi_small = P::phi(c, it.start + ss / (double)sh.n, coeff, phi_small);
                 s1_big += sb;
             }
-            outer_phi<P>(c, it, s1_big, phi_small, 0., 0., phi_f, phi_c);
         }
+        if (!unpaired) outer_phi<P>(c, it, s1_big, phi_small, 0., 0., phi_f, phi_c);
         if (threadIdx.x == 0) redo_finish(c, m, it, phi_f, phi_c);
     }
 }
@@ -974,7 +989,7 @@ hipError_t launch_batched_level(const typename P::Args& A0, uint32_t blocks, hip
     typename P::Args A = A0;
     A.c.fused = fused_applies<P>(A.c) ? 1u : 0u;
     const bool rule1 = A.c.ep.adapt_rule == 1;
-    hipLaunchKernelGGL(k_b_zero, dim3(1), dim3(256), 0, s, A.c.ctl, A.c.inst_out, A.c.n_inst);
+    if (A.c.zero_ctl) hipLaunchKernelGGL(k_b_zero, dim3(1), dim3(256), 0, s, A.c.ctl, A.c.inst_out, A.c.n_inst);
     hipError_t e = P::launch_outer(A, s);
     if (e != hipSuccess) return e;
     // Mixed form: the FP64 recheck of round r's uncertain decisions, before
@@ -1004,8 +1019,9 @@ hipError_t launch_batched_level(const typename P::Args& A0, uint32_t blocks, hip
         else hipLaunchKernelGGL((k_b_test<P, MIXED, 0>), dim3(blocks), dim3(256), 0, s, A, r);
         recheck(r);
     }
-    hipLaunchKernelGGL((k_b_final_count<P>), dim3(blocks), dim3(256), 0, s, A);
-    hipLaunchKernelGGL(k_b_final_alloc, dim3(1), dim3(64), 0, s, A.c.ctl, A.c.fcap);
+    // At most 256 blocks: each takes a ticket on one counter when done, and
+    // the tickets of a 1024-block grid queue on that address for ~10 us.
+    hipLaunchKernelGGL((k_b_final_count<P>), dim3(blocks < 256 ? blocks : 256), dim3(256), 0, s, A);
     hipLaunchKernelGGL((k_b_final_fill<P>), dim3(blocks), dim3(256), 0, s, A);
     hipLaunchKernelGGL((k_b_final<P, MIXED>), dim3(blocks), dim3(256), 0, s, A);
     if (MIXED) {

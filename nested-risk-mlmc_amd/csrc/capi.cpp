@@ -132,6 +132,7 @@ void Engine::free_batched() {
     bat_redo_acc = nullptr;
     bat_redo_cnt = nullptr;
     bat_ctl = nullptr;
+    ctl_dirty = true;
     bat_ctl_h = nullptr;
     pf_R = nullptr;
     pf_R32 = nullptr;
@@ -360,7 +361,10 @@ int Engine::prepare_batched(uint64_t M, BatchedCommon& c) {
         if ((e = hipMalloc(&bat_pend, sizeof(uint32_t) * cap)) != hipSuccess) return hip_fail(e, "hipMalloc pending");
         bat_cap = cap;
     }
-    const uint64_t want_f = 3 * M + 4096;
+    uint64_t want_f = 3 * M + 4096;
+    // Test hook: a first pool of this many entries, so that a level call
+    // overflows it and takes the grow-and-redo path.
+    if (const char* env = getenv("NMLMC_FINAL_POOL")) want_f = std::max<uint64_t>(1, strtoull(env, nullptr, 10));
     if (want_f > bat_fcap) {
         if (bat_fpool) hipFree(bat_fpool);
         bat_fpool = nullptr;
@@ -376,6 +380,7 @@ int Engine::prepare_batched(uint64_t M, BatchedCommon& c) {
     }
     if (!bat_ctl) {
         if ((e = hipMalloc(&bat_ctl, sizeof(BatchedCtl))) != hipSuccess) return hip_fail(e, "hipMalloc ctl");
+        ctl_dirty = true;
         if ((e = hipHostMalloc(&bat_ctl_h, sizeof(BatchedCtl))) != hipSuccess) return hip_fail(e, "hipHostMalloc ctl");
     }
     c.items = bat_items;
@@ -393,6 +398,7 @@ int Engine::prepare_batched(uint64_t M, BatchedCommon& c) {
     c.redo_acc = bat_redo_acc;
     c.redo_cnt = bat_redo_cnt;
     c.ctl = bat_ctl;
+    c.zero_ctl = 1;
     return NMLMC_OK;
 }
 
@@ -469,10 +475,14 @@ int Engine::run_batched(uint32_t ell, uint64_t M, BatchedCommon& c, sample_outpu
             if (k >= sweep_slot_cap) return set_error(NMLMC_ERR_STATE, "level sweep: no control slot left");
             c.M = std::min(chunk, M - start);
             c.offset = offset0 + start;
+            c.zero_ctl = ctl_dirty ? 1u : 0u;
+            ctl_dirty = true;
             hipError_t e = launch();
             if (e != hipSuccess) return hip_fail(e, "launch batched level");
             e = hipMemcpyAsync(sweep_slots + k, bat_ctl, sizeof(BatchedCtl), hipMemcpyDeviceToHost, stream);
             if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync ctl");
+            if ((e = launch_clear_ctl(bat_ctl, stream)) != hipSuccess) return hip_fail(e, "launch k_b_zero");
+            ctl_dirty = false;
             sweep_chain_level.push_back(sweep_level);
         }
         c.offset = offset0;
@@ -501,11 +511,17 @@ int Engine::run_batched(uint32_t ell, uint64_t M, BatchedCommon& c, sample_outpu
         for (int attempt = 0; attempt < 4; attempt++) {
             hipError_t e = hipEventRecord(ev0, stream);
             if (e != hipSuccess) return hip_fail(e, "hipEventRecord");
+            c.zero_ctl = ctl_dirty ? 1u : 0u;
+            ctl_dirty = true;
             e = launch();
             if (e != hipSuccess) return hip_fail(e, "launch batched level");
             if ((e = hipEventRecord(ev1, stream)) != hipSuccess) return hip_fail(e, "hipEventRecord");
             e = hipMemcpyAsync(bat_ctl_h, bat_ctl, sizeof(BatchedCtl), hipMemcpyDeviceToHost, stream);
             if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync ctl");
+            // Cleared for the next chain once copied out (outside the timed
+            // chain, like the copy: one block, about 2 us).
+            if ((e = launch_clear_ctl(bat_ctl, stream)) != hipSuccess) return hip_fail(e, "launch k_b_zero");
+            ctl_dirty = false;
             if ((e = hipStreamSynchronize(stream)) != hipSuccess) return hip_fail(e, "batched kernels");
             if (!bat_ctl_h->overflow) break;
             // More final groups than the pool holds: grow it and redo the chunk
@@ -618,6 +634,9 @@ int Engine::sample_levels(uint32_t n, const uint32_t* ells, const uint64_t* Ms, 
         hipGraph_t g = nullptr;
         e = hipStreamEndCapture(stream, &g);
         if (rc || e != hipSuccess) {
+            // The captured chains never run: the control block's state is
+            // the one before the capture, which may not be clear.
+            ctl_dirty = true;
             if (g) hipGraphDestroy(g);
             return rc ? rc : hip_fail(e, "hipStreamEndCapture");
         }
@@ -639,14 +658,21 @@ int Engine::sample_levels(uint32_t n, const uint32_t* ells, const uint64_t* Ms, 
             sweep_exec = nullptr;
             if ((e = hipGraphInstantiate(&sweep_exec, g, nullptr, nullptr, 0)) != hipSuccess) {
                 sweep_exec = nullptr;
+                ctl_dirty = true;
                 hipGraphDestroy(g);
                 return hip_fail(e, "hipGraphInstantiate");
             }
             sweep_exec_nodes = nodes;
         }
         hipGraphDestroy(g);
-        if ((e = hipEventRecord(ev0, stream)) != hipSuccess) return hip_fail(e, "hipEventRecord");
-        if ((e = hipGraphLaunch(sweep_exec, stream)) != hipSuccess) return hip_fail(e, "hipGraphLaunch");
+        if ((e = hipEventRecord(ev0, stream)) != hipSuccess) {
+            ctl_dirty = true;
+            return hip_fail(e, "hipEventRecord");
+        }
+        if ((e = hipGraphLaunch(sweep_exec, stream)) != hipSuccess) {
+            ctl_dirty = true;
+            return hip_fail(e, "hipGraphLaunch");
+        }
     } else if (rc) {
         hipStreamSynchronize(stream);
         return rc;

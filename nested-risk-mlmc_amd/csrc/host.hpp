@@ -80,6 +80,13 @@ struct Engine {
     double* bat_redo_acc = nullptr;
     uint32_t* bat_redo_cnt = nullptr;
     BatchedCtl* bat_ctl = nullptr;
+    // The control block is clear between single-problem chains: run_batched
+    // copies it out after each chain and then clears it, so the next chain
+    // need not start with a clearing launch (which leaves the GPU idle while
+    // the host enqueues the chain's next kernel).  True when that may not hold
+    // (new buffer, an enqueue that failed, a batch chain, a capture not run):
+    // the next chain then clears it first.
+    bool ctl_dirty = true;
     BatchedCtl* bat_ctl_h = nullptr;
     uint64_t bat_cap = 0, bat_fcap = 0;
     uint64_t mixed_reeval = 0, mixed_flips = 0, mixed_rechecks = 0;

@@ -83,6 +83,11 @@ hipError_t launch_reduce_partials(const Partial* P, uint32_t n, Partial* out, hi
     return hipGetLastError();
 }
 
+hipError_t launch_clear_ctl(BatchedCtl* ctl, hipStream_t s) {
+    hipLaunchKernelGGL(k_b_zero, dim3(1), dim3(256), 0, s, ctl, (Partial*)nullptr, 0u);
+    return hipGetLastError();
+}
+
 __global__ void k_b_zero(BatchedCtl* ctl, Partial* inst_out, uint32_t n_inst) {
     uint32_t* p = (uint32_t*)ctl;
     const uint32_t n = sizeof(BatchedCtl) / 4;
@@ -90,20 +95,6 @@ __global__ void k_b_zero(BatchedCtl* ctl, Partial* inst_out, uint32_t n_inst) {
     uint32_t* q = (uint32_t*)inst_out;
     const uint32_t nq = inst_out ? n_inst * (uint32_t)(sizeof(Partial) / 4) : 0;
     for (uint32_t i = threadIdx.x; i < nq; i += blockDim.x) q[i] = 0;
-}
-
-__global__ void k_b_final_alloc(BatchedCtl* ctl, uint32_t fcap) {
-    if (threadIdx.x != 0) return;
-    uint32_t off = 0;
-    for (int i = 0; i < kMaxClasses; i++) {
-        ctl->class_off[i] = off;
-        ctl->class_cur[i] = 0;
-        off += ctl->class_cnt[i];
-    }
-    ctl->class_off[kMaxClasses] = off;
-    ctl->final_total = off + ctl->n_whole;
-    ctl->overflow = ctl->final_total > fcap;
-    ctl->n_whole = 0;  // refilled as a cursor by k_b_final_fill
 }
 
 }  // namespace nmlmc

@@ -163,6 +163,30 @@ def test_level_sweep_matches_lone_calls(nm, oracle_f64, form, graph, monkeypatch
     nm.set_form(nm.FORM_PLAIN)
 
 
+@pytest.mark.parametrize("form", [1, 2])
+def test_final_pool_overflow_redo(nm, oracle_f64, form, monkeypatch):
+    """A level call whose final-group pool overflows grows the pool and redoes
+    the call from the same keyed draws: a lone call in place, a call of a
+    level sweep on its own after the sweep.  The results equal those of calls
+    whose pool was large enough.  Lone calls and sweeps alternate, so the
+    clearing of the control block between chains is crossed by the redos."""
+    ells, Ms = [0, 1, 2, 3], [2500, 900, 300, 100]
+    _shift_setup(nm, oracle_f64, 1, form)
+    ref = [_sums(nm.sample(l, M)) for l, M in zip(ells, Ms)]
+    ref2 = [_sums(nm.sample(l, M)) for l, M in zip(ells, Ms)]
+    nm.cleanup_nestedmc_sampler()
+    monkeypatch.setenv("NMLMC_FINAL_POOL", "64")
+    _shift_setup(nm, oracle_f64, 1, form)
+    assert [_sums(nm.sample(l, M)) for l, M in zip(ells, Ms)] == ref
+    assert [_sums(nm.sample(l, M)) for l, M in zip(ells, Ms)] == ref2
+    nm.cleanup_nestedmc_sampler()
+    _shift_setup(nm, oracle_f64, 1, form)
+    assert [_sums(r) for r in nm.sample_levels(ells, Ms)] == ref
+    assert [_sums(nm.sample(l, M)) for l, M in zip(ells, Ms)] == ref2
+    nm.cleanup_nestedmc_sampler()
+    nm.set_form(nm.FORM_PLAIN)
+
+
 def test_mlmc_driver_sweeps_match_lone_calls(nm, oracle_f64):
     """The native MLMC driver in the batched form samples each step's levels as
     one sweep; run over lone level calls instead (the callback driver, same
