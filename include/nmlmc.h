@@ -234,12 +234,13 @@ int nmlmc_batch_sample(uint32_t ell, uint32_t n, const uint64_t* M, sample_outpu
  * form of model_sample / portfolio_sample) that allocates no memory: the same
  * kernels and results as model_sample / portfolio_sample(ell, 0, M, out) of
  * the staged problem, with their scratch (one partial record per block of
- * virtual threads, and the reduced record) in a static device array of 16384
- * + 1 records set up at init_nestedmc_sampler.  No counterpart in the
- * reference: for callers that cannot allocate (HIP graph capture, serving
- * loops, harnesses that forbid hipMalloc).  Limit: cores x lanes / 256 <= 16384
- * blocks (65536 virtual threads of 64 lanes; the portfolio: cores <= 16384),
- * else NMLMC_ERR_ARG.  Calls that overlap on one GPU share the array and race.
+ * virtual threads, the reduced record and the reduction's staging records) in
+ * a static device array of 16384 + 1 + 256 records set up at
+ * init_nestedmc_sampler.  No counterpart in the reference: for callers that
+ * cannot allocate (HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc).  Limit: min(M, cores) x lanes / 256 <= 16384 blocks (the virtual
+ * threads holding outer samples: 65536 of 64 lanes; the portfolio: min(M,
+ * cores) <= 16384), else NMLMC_ERR_ARG.  Calls that overlap on one GPU share the array and race.
  * NMLMC_ERR_STATE if the library is not initialised or the form is not plain. */
 int nmlmc_sample_noalloc(uint32_t ell, uint64_t M, sample_output* out);
 

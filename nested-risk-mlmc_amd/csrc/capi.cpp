@@ -78,6 +78,14 @@ LevelCounts level_counts(const EngineParams& ep, uint32_t ell) {
     return lc;
 }
 
+// Virtual threads with outer samples up to which a plain model level call whose
+// inner loops fill 256 lanes runs one virtual thread per 256-lane block
+// (NMLMC_PLAIN_WIDE_MAX overrides; 0 keeps at most a wave).
+static uint64_t plain_wide_max() {
+    if (const char* env = getenv("NMLMC_PLAIN_WIDE_MAX")) return strtoull(env, nullptr, 10);
+    return 32768;
+}
+
 static uint32_t pow2_ceil_u32(uint64_t x) {
     uint32_t g = 1;
     while (g < x && g < 64) g <<= 1;
@@ -297,10 +305,24 @@ int Engine::sample_model_parity(uint32_t ell, uint64_t M, uint32_t lanes_overrid
     H.ell = ell;
     const int nps = mp.variant == MODEL_REF ? 2 : 1;
     const uint64_t pairs = (H.cf.base * nps + 1) / 2;
-    H.G = lanes_override ? pow2_ceil_u32(lanes_override) : pow2_ceil_u32(pairs);
-    if (const char* s = getenv("NMLMC_GROUP_LANES")) H.G = pow2_ceil_u32(strtoul(s, nullptr, 10));
+    // Lanes per virtual thread: the next power of two of the base count's
+    // Box-Muller pairs, at most a wave; a whole block of 256 lanes when at most
+    // plain_wide_max() virtual threads hold outer samples and the pairs fill
+    // it (the deep levels, whose few outer samples each take up to 2^cap times
+    // the base count on one virtual thread) and the partial records fit.
+    uint64_t want = lanes_override;
+    if (const char* s = getenv("NMLMC_GROUP_LANES")) want = strtoul(s, nullptr, 10);
+    if (want) {
+        H.G = want >= 256 ? 256 : pow2_ceil_u32(want);
+    } else {
+        H.G = pow2_ceil_u32(pairs);
+        const uint64_t act = plain_active(M, cores);
+        if (pairs >= 256 && act <= plain_wide_max() && (!scr_partials || parity_blocks(act, 256) <= scr_cap))
+            H.G = 256;
+    }
     last_G = H.G;
-    const uint64_t nblocks = parity_blocks(cores, H.G);
+    const uint64_t nblocks = parity_blocks(plain_active(M, cores), H.G);
+    H.blocks = (uint32_t)nblocks;
     Partial *parts, *res;
     int rc = plain_buffers(nblocks, parts, res);
     if (rc) return rc;

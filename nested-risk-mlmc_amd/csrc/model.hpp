@@ -37,6 +37,7 @@ struct ModelParityArgsHost {
     uint32_t T;
     uint32_t ell;
     uint32_t G;
+    uint32_t blocks;  // launched: the blocks of the virtual threads that hold outer samples
     StreamState* states;
     Partial* partials;
     double* tr_phi_f;
@@ -55,6 +56,10 @@ hipError_t launch_model_batched(const ModelBatchedHost& H, hipStream_t s);
 
 // Blocks of 256 threads covering T groups of G lanes; one Partial per block.
 inline uint64_t parity_blocks(uint64_t T, uint32_t G) { return (T * G + 255) / 256; }
+// Virtual threads that hold outer samples in a level call of M: libgomp's static
+// schedule gives thread t a share only when t < M.  The others draw nothing and
+// keep their streams, so their blocks are not launched (at least one block is).
+inline uint64_t plain_active(uint64_t M, uint64_t T) { return M == 0 ? 1 : (M < T ? M : T); }
 
 hipError_t launch_model_parity(const ModelParityArgsHost& H, hipStream_t s);
 // Reduces n Partial records into out[0] in a fixed order; out has room for

@@ -4,11 +4,13 @@
 // the OpenMP build's semantics: T virtual threads, thread t owning two streams
 // keyed {seed, t} whose positions persist across calls, thread t taking the
 // libgomp static-schedule block of [0, M) and walking its outer samples in
-// order.  On the GPU a virtual thread is an aligned group of G lanes (G | 64):
-// the group walks the outer samples in order while its lanes split every inner
-// loop, each lane computing the draws at the stream positions it owns.  Every
-// decision (adaptive counts, phi, moments) is group-uniform, so the draws and
-// the per-sample results equal those of the sequential reference.
+// order.  On the GPU a virtual thread is an aligned group of G lanes (G | 64),
+// or a whole block of G = 256 lanes (four waves) when few virtual threads hold
+// outer samples and their inner loops are long: the group walks the outer
+// samples in order while its lanes split every inner loop, each lane computing
+// the draws at the stream positions it owns.  Every decision (adaptive counts,
+// phi, moments) is group-uniform, so the draws and the per-sample results
+// equal those of the sequential reference.
 #include "engine.hpp"
 #include "model.hpp"
 
@@ -29,6 +31,24 @@ struct ModelParityArgs {
     uint64_t* tr_nf;
     uint64_t* tr_nc;
 };
+
+// Sum over the G lanes of a virtual thread.  G = 256 is the whole block (one
+// virtual thread per block, so every thread reaches the barriers): wave sums,
+// then the four added in wave order.
+template <int G>
+__device__ __forceinline__ double vt_sum(double v) {
+    if constexpr (G <= 64) {
+        return group_sum<G>(v);
+    } else {
+        static_assert(G == 256, "a virtual thread is at most one 256-thread block");
+        __shared__ double red[4];
+        v = wave_sum(v);
+        __syncthreads();  // the previous sum's reads are done
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+        __syncthreads();
+        return ((red[0] + red[1]) + red[2]) + red[3];
+    }
+}
 
 // Inner loop over `count` model samples from stream state st (the group's
 // inner stream), lanes of the group splitting the Box-Muller pairs
@@ -75,8 +95,8 @@ __device__ __forceinline__ void parity_inner(const ModelParams& mp, uint64_t k0,
             s2 += d * d;
         }
     }
-    s1_out = group_sum<G>(s1);
-    s2_out = want_s2 ? group_sum<G>(s2) : 0.;
+    s1_out = vt_sum<G>(s1);
+    s2_out = want_s2 ? vt_sum<G>(s2) : 0.;
     st = advance_normals_f64(k0, k1, st, K);
 }
 
@@ -133,8 +153,16 @@ __device__ __forceinline__ uint64_t parity_count(const ModelParams& mp, const En
     return cur;
 }
 
+// Waves per SIMD the kernel is compiled for: 2 keeps it at 255 VGPRs without
+// spills (left alone the compiler reserves the whole register file and runs one
+// wave a SIMD; the model sweep is 1.33x faster at 2; at 4 it spills 350-530 B
+// a lane and is 1.18x).
+#ifndef NMLMC_PLAIN_MODEL_WAVES
+#define NMLMC_PLAIN_MODEL_WAVES 2
+#endif
+
 template <int VAR, int G>
-__global__ __launch_bounds__(256) void k_model_parity(ModelParityArgs A) {
+__global__ __launch_bounds__(256, NMLMC_PLAIN_MODEL_WAVES) void k_model_parity(ModelParityArgs A) {
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t t = gtid / G;
     const uint32_t gl = gtid % G;
@@ -228,10 +256,8 @@ __global__ __launch_bounds__(256) void k_model_parity(ModelParityArgs A) {
 }
 
 template <int VAR>
-static hipError_t launch_var(const ModelParityArgs& A, uint32_t G, hipStream_t s) {
-    const uint64_t threads = (uint64_t)A.T * G;
-    const dim3 grid((unsigned)parity_blocks(A.T, G)), block(256);
-    (void)threads;
+static hipError_t launch_var(const ModelParityArgs& A, uint32_t G, uint32_t blocks, hipStream_t s) {
+    const dim3 grid(blocks), block(256);
     switch (G) {
     case 1: hipLaunchKernelGGL((k_model_parity<VAR, 1>), grid, block, 0, s, A); break;
     case 2: hipLaunchKernelGGL((k_model_parity<VAR, 2>), grid, block, 0, s, A); break;
@@ -239,6 +265,7 @@ static hipError_t launch_var(const ModelParityArgs& A, uint32_t G, hipStream_t s
     case 8: hipLaunchKernelGGL((k_model_parity<VAR, 8>), grid, block, 0, s, A); break;
     case 16: hipLaunchKernelGGL((k_model_parity<VAR, 16>), grid, block, 0, s, A); break;
     case 32: hipLaunchKernelGGL((k_model_parity<VAR, 32>), grid, block, 0, s, A); break;
+    case 256: hipLaunchKernelGGL((k_model_parity<VAR, 256>), grid, block, 0, s, A); break;
     default: hipLaunchKernelGGL((k_model_parity<VAR, 64>), grid, block, 0, s, A); break;
     }
     return hipGetLastError();
@@ -260,8 +287,8 @@ hipError_t launch_model_parity(const ModelParityArgsHost& H, hipStream_t s) {
     A.tr_phi_c = H.tr_phi_c;
     A.tr_nf = H.tr_nf;
     A.tr_nc = H.tr_nc;
-    if (H.mp.variant == MODEL_REF) return launch_var<MODEL_REF>(A, H.G, s);
-    return launch_var<MODEL_SHIFT>(A, H.G, s);
+    if (H.mp.variant == MODEL_REF) return launch_var<MODEL_REF>(A, H.G, H.blocks, s);
+    return launch_var<MODEL_SHIFT>(A, H.G, H.blocks, s);
 }
 
 }  // namespace nmlmc

@@ -96,6 +96,7 @@ struct PortfolioParityHost {
     LevelCounts cf, cc;
     uint64_t seed, M;
     uint32_t T, ell;
+    uint32_t blocks;  // launched: one per virtual thread that holds outer samples
     double bs_work;
     StreamState* states;
     Partial* partials;

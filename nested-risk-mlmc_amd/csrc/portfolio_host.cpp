@@ -252,8 +252,9 @@ int Engine::sample_portfolio_parity(uint32_t ell, uint64_t M, uint32_t, sample_o
     H.T = cores;
     H.ell = ell;
     H.bs_work = pf_bs_work;
+    H.blocks = (uint32_t)plain_active(M, cores);
     Partial *parts, *res;
-    int rc = plain_buffers(cores, parts, res);
+    int rc = plain_buffers(H.blocks, parts, res);
     if (rc) return rc;
     H.states = d_states;
     H.partials = parts;
@@ -269,7 +270,7 @@ int Engine::sample_portfolio_parity(uint32_t ell, uint64_t M, uint32_t, sample_o
     if (e != hipSuccess) return hip_fail(e, "hipEventRecord");
     if ((e = launch_portfolio_parity(H, stream)) != hipSuccess) return hip_fail(e, "launch k_portfolio_parity");
     if ((e = hipEventRecord(ev1, stream)) != hipSuccess) return hip_fail(e, "hipEventRecord");
-    if ((e = launch_reduce_partials(parts, cores, res, stream)) != hipSuccess)
+    if ((e = launch_reduce_partials(parts, H.blocks, res, stream)) != hipSuccess)
         return hip_fail(e, "launch k_reduce_partials");
     last_kernel = "k_portfolio_parity";
     rc = finish_call(ell, out, res);

@@ -13,7 +13,7 @@
 //     prefix count, so every draw matches the sequential reference;
 //   - other configurations (no sub-sampling, Unbiased options, a cached inner
 //     normal) run the reference loop on thread 0.
-// A virtual thread may take a block of W <= 4 waves (W > 1 only for books whose
+// A virtual thread may take a block of W <= 8 waves (W > 1 only for books whose
 // options are all Exact, chosen when few virtual threads have outer samples):
 // the outer draw's stocks and the inner loop's samples are then spread over
 // the block's lanes.  With every option Exact, sample k draws method 1 from
@@ -40,9 +40,12 @@ struct PortfolioParityArgs {
     uint64_t* tr_nc;
 };
 
-// Doubles of LDS scratch a block needs beyond R_tau and z: the wave sums of a
-// block of up to 4 waves (3 each), or thread 0's broadcast (11 values).
+// Waves a virtual thread may take, and the doubles of LDS scratch a block
+// needs beyond R_tau and z: the wave sums of a block of up to kMaxW waves (3
+// each), or thread 0's broadcast (11 values).
+constexpr uint32_t kMaxW = 8;
 constexpr uint32_t kParityRed = 32;
+static_assert(3 * kMaxW <= kParityRed && 11 <= kParityRed, "LDS reduction scratch");
 
 struct InnerSums {
     double s1, s2, work;
@@ -55,17 +58,18 @@ __device__ __forceinline__ double phi_value(int phi_type, double u, double coeff
 
 // inner_sampler::sample_inner (nested_sampler.hpp:131-161) over `count` samples
 // from the thread's streams (ss: sync, si: inner), block-uniform result.  red:
-// LDS scratch of at least 3 W doubles (W > 1).
-template <uint32_t W>
-__device__ InnerSums parity_inner(const PortfolioParityArgs& A, const double* R, uint64_t k1, StreamState& ss,
+// LDS scratch of at least 3 W doubles.  EXACT_ONLY: a sub-sampled book whose
+// options are all Exact (the only books given W > 1); its kernel holds no BS or
+// Unbiased code, which keeps it to a fraction of the general kernel's registers.
+template <uint32_t W, bool EXACT_ONLY>
+__device__ __forceinline__ InnerSums parity_inner(const PortfolioParityArgs& A, const double* R, uint64_t k1, StreamState& ss,
                                   StreamState& si, uint64_t count, double* red) {
     const PortfolioDevice& pf = A.pf;
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     InnerSums r{0., 0., 0., 0, 0};
     const bool fast = pf.subsample != 0 && !(pf.compute_prob[2] > 0) && !si.is_saved;
-    if (W > 1 && fast) {
-        // Every option Exact (the only books given W > 1): method 1 for every
-        // sample, three inner words each.
+    if (EXACT_ONLY && fast) {
+        // Every option Exact: method 1 for every sample, three inner words each.
         double s1 = 0., s2 = 0., work = 0.;
         for (uint64_t k = tid; k < count; k += 64 * W) {
             const uint64_t q = si.P + 3 * k;
@@ -104,7 +108,7 @@ __device__ InnerSums parity_inner(const PortfolioParityArgs& A, const double* R,
         si.P += 3 * count;
         return InnerSums{t1, t2, tw, 2 * count, 2 * count};
     }
-    if (fast) {
+    if (!EXACT_ONLY && fast) {
         double s1 = 0., s2 = 0., work = 0.;
         uint64_t nn = 0;
         uint64_t base = 0;
@@ -160,7 +164,18 @@ __device__ InnerSums parity_inner(const PortfolioParityArgs& A, const double* R,
         in.init(A.seed, k1, 0, si.P, si.saved, si.is_saved);
         double s1 = 0., s2 = 0., work = 0.;
         for (uint64_t i = 0; i < count; i++) {
-            const double d = inner_sample_d(pf, R, sy, in, work);
+            double d;
+            if constexpr (EXACT_ONLY) {
+                // pick_option and the Exact branch of inner_sample_d: the
+                // method uniform is drawn, and method 1 is the only one.
+                uint32_t method, idx;
+                const double prob = pick_option(pf, sy, in, method, idx);
+                const double z0 = in.randn();
+                const double z1 = in.randn();
+                d = inner_exact_d(pf, pf.options[1][idx], R, z0, z1, work) / prob;
+            } else {
+                d = inner_sample_d(pf, R, sy, in, work);
+            }
             s1 += d;
             s2 += d * d;
         }
@@ -213,8 +228,18 @@ __device__ InnerSums parity_inner(const PortfolioParityArgs& A, const double* R,
     return r;
 }
 
-template <uint32_t W>
-__global__ __launch_bounds__(64 * W) void k_portfolio_parity(PortfolioParityArgs A) {
+// Waves per SIMD the Exact-only kernels are compiled for (the general kernel
+// is left to the compiler, which gives it one).  Measured on the large and
+// small books' plain sweeps: 2 (what the compiler picks unasked, 235-255
+// VGPRs, 336 B of scratch a lane) 18.5 and 13.2 ms, 3 (168 VGPRs, 624 B)
+// 17.0 and 12.9 ms, 4 (128 VGPRs, 800 B) 17.4 and 13.0 ms.
+#ifndef NMLMC_PLAIN_EXACT_WAVES
+#define NMLMC_PLAIN_EXACT_WAVES 3
+#endif
+
+template <uint32_t W, bool EXACT_ONLY>
+__global__ __launch_bounds__(64 * W, (EXACT_ONLY ? NMLMC_PLAIN_EXACT_WAVES : 1)) void k_portfolio_parity(
+    PortfolioParityArgs A) {
     extern __shared__ double smem[];  // R_tau[d], z[d], reduction scratch[kParityRed]
     const PortfolioDevice& pf = A.pf;
     const EngineParams& ep = A.ep;
@@ -276,10 +301,12 @@ __global__ __launch_bounds__(64 * W) void k_portfolio_parity(PortfolioParityArgs
         const double constant = -pf.max_loss - cv;
         // init_inner_sampler: the pre-summed BS options without sub-sampling.
         double w_det = 0.;
-        const double start = (pf.subsample == 0) ? det_bs_sum(pf, R, w_det) + constant : constant;
+        double start = constant;
+        if constexpr (!EXACT_ONLY)
+            if (pf.subsample == 0) start += det_bs_sum(pf, R, w_det);
 
         auto test_round = [&](uint64_t cur) {
-            const InnerSums s = parity_inner<W>(A, R, k1, ss, si, cur, red);
+            const InnerSums s = parity_inner<W, EXACT_ONLY>(A, R, k1, ss, si, cur, red);
             work += s.work + A.bs_work;
             stats[4] += s.nu;
             stats[5] += s.nn;
@@ -308,7 +335,7 @@ __global__ __launch_bounds__(64 * W) void k_portfolio_parity(PortfolioParityArgs
             return cur;
         };
         auto mean_of = [&](uint64_t n) {
-            const InnerSums s = parity_inner<W>(A, R, k1, ss, si, n, red);
+            const InnerSums s = parity_inner<W, EXACT_ONLY>(A, R, k1, ss, si, n, red);
             work += s.work + A.bs_work;
             stats[4] += s.nu;
             stats[5] += s.nn;
@@ -390,26 +417,35 @@ hipError_t launch_portfolio_parity(const PortfolioParityHost& H, hipStream_t s) 
     A.tr_nc = H.tr_nc;
     // Waves per virtual thread: more than one only for books whose options are
     // all Exact, while the virtual threads with outer samples (min(M, T)) are
-    // too few to fill the device: up to 4, while they stay within 1536 waves
-    // (the kernel runs at one wave a SIMD; measured on the large book at 128
-    // and 362 outer samples: 2.6x and 1.3x faster at 4 waves, and slower once
-    // the waves outnumber the SIMDs; 8 waves spill).  NMLMC_PLAIN_WAVES overrides
-    // it (1, 2 or 4); per-sample results do not depend on it.
+    // too few to fill the device: up to kMaxW, doubling while they stay within
+    // a budget of 6144 waves (the Exact-only kernel runs three waves a SIMD;
+    // the large and small books' plain sweeps took 16.9 and 12.5 ms at a
+    // budget of 1536, 16.3 and 11.7 ms at 6144, 16.5 and 11.5 ms at 12288).
+    // NMLMC_PLAIN_WAVE_BUDGET and NMLMC_PLAIN_WAVES (1, 2, 4 or 8) override it;
+    // per-sample results do not depend on it.
     uint32_t W = 1;
     const bool exact_only = H.pf.subsample != 0 && H.pf.nopt[0] == 0 && H.pf.nopt[2] == 0;
     if (exact_only) {
         const uint64_t act = H.M < (uint64_t)H.T ? H.M : (uint64_t)H.T;
-        while (W < 4 && act * (2 * W) <= 1536) W *= 2;
+        uint64_t budget = 6144;
+        if (const char* env = getenv("NMLMC_PLAIN_WAVE_BUDGET")) budget = strtoull(env, nullptr, 10);
+        uint32_t wmax = kMaxW;
+        if (const char* env = getenv("NMLMC_PLAIN_WAVES_MAX")) wmax = (uint32_t)strtoul(env, nullptr, 10);
+        while (W < wmax && W < kMaxW && act * (2 * W) <= budget) W *= 2;
         if (const char* env = getenv("NMLMC_PLAIN_WAVES")) {
             const uint32_t v = (uint32_t)strtoul(env, nullptr, 10);
-            W = (v >= 4) ? 4 : (v >= 2) ? 2 : 1;
+            W = (v >= 8) ? 8 : (v >= 4) ? 4 : (v >= 2) ? 2 : 1;
         }
     }
     const size_t lds = sizeof(double) * (2 * (size_t)H.pf.d + kParityRed);
     switch (W) {
-    case 4: hipLaunchKernelGGL(k_portfolio_parity<4>, dim3(H.T), dim3(256), lds, s, A); break;
-    case 2: hipLaunchKernelGGL(k_portfolio_parity<2>, dim3(H.T), dim3(128), lds, s, A); break;
-    default: hipLaunchKernelGGL(k_portfolio_parity<1>, dim3(H.T), dim3(64), lds, s, A); break;
+    case 8: hipLaunchKernelGGL((k_portfolio_parity<8, true>), dim3(H.blocks), dim3(512), lds, s, A); break;
+    case 4: hipLaunchKernelGGL((k_portfolio_parity<4, true>), dim3(H.blocks), dim3(256), lds, s, A); break;
+    case 2: hipLaunchKernelGGL((k_portfolio_parity<2, true>), dim3(H.blocks), dim3(128), lds, s, A); break;
+    default:
+        if (exact_only) hipLaunchKernelGGL((k_portfolio_parity<1, true>), dim3(H.blocks), dim3(64), lds, s, A);
+        else hipLaunchKernelGGL((k_portfolio_parity<1, false>), dim3(H.blocks), dim3(64), lds, s, A);
+        break;
     }
     return hipGetLastError();
 }

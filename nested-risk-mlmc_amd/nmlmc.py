@@ -201,15 +201,19 @@ class Output:
     """Owns the arrays of a sample_output."""
 
     def __init__(self, moments=4, phi_size=1):
-        self.fsums = np.zeros(moments * phi_size)
-        self.diffsums = np.zeros(moments * phi_size)
+        n = moments * phi_size
+        self._f = (C.c_double * n)()
+        self._d = (C.c_double * n)()
         self.c = SampleOutput()
-        self.c.fsums = self.fsums.ctypes.data_as(C.POINTER(C.c_double))
-        self.c.diffsums = self.diffsums.ctypes.data_as(C.POINTER(C.c_double))
+        self.c.fsums = C.cast(self._f, C.POINTER(C.c_double))
+        self.c.diffsums = C.cast(self._d, C.POINTER(C.c_double))
+        self.fsums = np.frombuffer(self._f, np.float64)
+        self.diffsums = np.frombuffer(self._d, np.float64)
 
     def as_dict(self):
         return dict(M=self.c.M, fsums=self.fsums.copy(), diffsums=self.diffsums.copy(),
-                    work=self.c.total_work, stats=np.array(list(self.c.inner_stats), np.uint64))
+                    work=self.c.total_work,
+                    stats=np.frombuffer(self.c.inner_stats, np.uint64).copy())
 
 
 # ---- reference entry points ---------------------------------------------------

@@ -50,9 +50,10 @@ def test_plain_reference_model_matches_oracle(nm, oracle_f64, cores, adaptive_r,
     nm.cleanup_nestedmc_sampler()
 
 
-@pytest.mark.parametrize("lanes", [1, 4, 16, 64])
+@pytest.mark.parametrize("lanes", [1, 4, 16, 64, 256])
 def test_plain_lane_groups_do_not_change_results(nm, oracle_f64, lanes, monkeypatch):
-    """The lanes-per-virtual-thread split is invisible in the results."""
+    """The lanes-per-virtual-thread split is invisible in the results (256: one
+    virtual thread per four-wave block, its sums reduced across the waves)."""
     monkeypatch.setenv("NMLMC_GROUP_LANES", str(lanes))
     orc = oracle_f64.new(16, 32, 4, 1.5, True)
     ml = _ref_max_loss(orc)

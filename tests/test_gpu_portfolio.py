@@ -354,10 +354,10 @@ def test_config_shaped_portfolios(nm, oracle_f64, form, which):
     nm.set_form(nm.FORM_PLAIN)
 
 
-@pytest.mark.parametrize("waves", [1, 2, 4])
+@pytest.mark.parametrize("waves", [1, 2, 4, 8])
 def test_plain_waves_per_virtual_thread(nm, oracle_f64, waves, monkeypatch):
     """Plain form on a book whose options are all Exact, each virtual thread on
-    a block of 1, 2 or 4 waves (NMLMC_PLAIN_WAVES): per-sample results equal
+    a block of 1, 2, 4 or 8 waves (NMLMC_PLAIN_WAVES): per-sample results equal
     the reference keying's (the samples' draws sit at known stream positions,
     so the block's lanes split them)."""
     monkeypatch.setenv("NMLMC_PLAIN_WAVES", str(waves))

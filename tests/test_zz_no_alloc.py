@@ -13,7 +13,7 @@ from portfolio_util import synthetic, to_library
 
 pytestmark = pytest.mark.gpu
 
-CALLS = [(3, 5000), (1, 700)]  # the second call smaller
+CALLS = [(3, 70000), (1, 700)]  # the second call smaller; the first holds every virtual thread
 
 
 def _run(nm, staging, cores, entry):
@@ -49,7 +49,8 @@ def _same(a, b):
 @pytest.mark.parametrize("cores", [4096, 65536])
 def test_noalloc_model_matches_model_sample(nm, cores):
     """Reference model, adaptive rule, 4096 and 65536 virtual threads (at level
-    3 a thread takes 64 lanes: 65536 threads fill the scratch's 16384 blocks)."""
+    3 a thread takes 64 lanes: 65536 threads, all holding outer samples at
+    M = 70000, fill the scratch's 16384 blocks)."""
     def stage():
         nm.model_init(0, 0.02, 0.02 * (2.2414 ** 2 - 1), 11)
         return None
@@ -81,18 +82,20 @@ def test_noalloc_portfolio_matches_portfolio_sample(nm):
 def test_noalloc_refusals(nm):
     """More blocks than the scratch holds, a batched form: errors with a
     status, nothing launched.  Reference model, N0 = 32, beta = 4: a level-0
-    thread takes 32 lanes (32 Box-Muller pairs), a level-1 thread 64, so 131072
-    virtual threads need 16384 blocks at level 0 (the limit) and 32768 at
-    level 1."""
+    thread takes 32 lanes (32 Box-Muller pairs), a level-1 thread 64.  Only the
+    virtual threads holding outer samples (min(M, T)) are launched, so 131072
+    of them need 16384 blocks at level 0 (the limit) and 32768 at level 1,
+    while 100 need one or a few blocks at either level."""
     lib = nm.load()
     out = nm.Output(4, 1)
     nm.init_nestedmc_sampler(131072, 32, 4, 1.5, True)
     nm.model_init(0, 0.02, 0.1, 3)
-    assert lib.nmlmc_sample_noalloc(0, 100, C.byref(out.c)) == 0
-    assert lib.nmlmc_sample_noalloc(1, 100, C.byref(out.c)) == 2  # NMLMC_ERR_ARG
+    assert lib.nmlmc_sample_noalloc(0, 131072, C.byref(out.c)) == 0
+    assert lib.nmlmc_sample_noalloc(1, 100, C.byref(out.c)) == 0
+    assert lib.nmlmc_sample_noalloc(1, 131072, C.byref(out.c)) == 2  # NMLMC_ERR_ARG
     assert b"scratch holds" in lib.nmlmc_last_error_string()
     with pytest.raises(nm.NmlmcError):
-        nm.sample_noalloc(1, 100)
+        nm.sample_noalloc(1, 131072)
     nm.set_form(nm.FORM_BATCHED)
     assert lib.nmlmc_sample_noalloc(0, 100, C.byref(out.c)) == 1  # NMLMC_ERR_STATE
     nm.set_form(nm.FORM_PLAIN)
