@@ -442,20 +442,21 @@ __device__ __forceinline__ uint32_t pick_index(const PortfolioDevice& pf, DStrea
 
 // One inner sample of portfolio::sampler::sample_inner (330-334) with the
 // reference's draw order.
-template <class RT>
+// UNBIASED = false: for books without Unbiased options, whose code is then left out.
+template <bool UNBIASED = true, class RT>
 __device__ __forceinline__ double inner_sample_d(const PortfolioDevice& pf, RT R_tau, DStream& sync,
                                                  DStream& inner, double& work) {
     if (pf.subsample == 0) {
         // sample_inner_all_options (336-344).
         double ret = 0.;
-        for (uint32_t m = 1; m < 3; m++)
+        for (uint32_t m = 1; m < (UNBIASED ? 3u : 2u); m++)
             for (uint32_t j = 0; j < pf.nopt[m]; j++) {
                 const OptionRec o = pf.options[m][j];
                 if (m == 1) {
                     const double z0 = inner.randn();
                     const double z1 = inner.randn();
                     ret += inner_exact_d(pf, o, R_tau, z0, z1, work);
-                } else {
+                } else if constexpr (UNBIASED) {
                     ret += inner_unbiased_d(pf, o, R_tau, sync, inner, work);
                 }
             }
@@ -465,12 +466,15 @@ __device__ __forceinline__ double inner_sample_d(const PortfolioDevice& pf, RT R
     const double prob = pick_option(pf, sync, inner, method, idx);
     const OptionRec o = pf.options[method][idx];
     double v;
-    if (method == 0) v = inner_bs_d(pf, o, R_tau, work);
-    else if (method == 1) {
+    if (method == 0) {
+        v = inner_bs_d(pf, o, R_tau, work);
+    } else if (!UNBIASED || method == 1) {
         const double z0 = inner.randn();
         const double z1 = inner.randn();
         v = inner_exact_d(pf, o, R_tau, z0, z1, work);
-    } else v = inner_unbiased_d(pf, o, R_tau, sync, inner, work);
+    } else {
+        v = inner_unbiased_d(pf, o, R_tau, sync, inner, work);
+    }
     return v / prob;
 }
 

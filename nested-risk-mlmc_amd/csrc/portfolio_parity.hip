@@ -61,9 +61,14 @@ __device__ __forceinline__ double phi_value(int phi_type, double u, double coeff
 // LDS scratch of at least 3 W doubles.  EXACT_ONLY: a sub-sampled book whose
 // options are all Exact (the only books given W > 1); its kernel holds no BS or
 // Unbiased code, which keeps it to a fraction of the general kernel's registers.
-template <uint32_t W, bool EXACT_ONLY>
+// Kernel variants by the book's option methods: the general one, books of
+// Exact options only (sub-sampled), and books without Unbiased options.
+enum { PAR_GENERAL = 0, PAR_EXACT = 1, PAR_NO_UNBIASED = 2 };
+
+template <uint32_t W, int KIND>
 __device__ __forceinline__ InnerSums parity_inner(const PortfolioParityArgs& A, const double* R, uint64_t k1, StreamState& ss,
                                   StreamState& si, uint64_t count, double* red) {
+    constexpr bool EXACT_ONLY = KIND == PAR_EXACT;
     const PortfolioDevice& pf = A.pf;
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     InnerSums r{0., 0., 0., 0, 0};
@@ -174,7 +179,7 @@ __device__ __forceinline__ InnerSums parity_inner(const PortfolioParityArgs& A, 
                 const double z1 = in.randn();
                 d = inner_exact_d(pf, pf.options[1][idx], R, z0, z1, work) / prob;
             } else {
-                d = inner_sample_d(pf, R, sy, in, work);
+                d = inner_sample_d<KIND != PAR_NO_UNBIASED>(pf, R, sy, in, work);
             }
             s1 += d;
             s2 += d * d;
@@ -228,18 +233,28 @@ __device__ __forceinline__ InnerSums parity_inner(const PortfolioParityArgs& A, 
     return r;
 }
 
-// Waves per SIMD the Exact-only kernels are compiled for (the general kernel
-// is left to the compiler, which gives it one).  Measured on the large and
-// small books' plain sweeps: 2 (what the compiler picks unasked, 235-255
-// VGPRs, 336 B of scratch a lane) 18.5 and 13.2 ms, 3 (168 VGPRs, 624 B)
-// 17.0 and 12.9 ms, 4 (128 VGPRs, 800 B) 17.4 and 13.0 ms.
+// Waves per SIMD the kernels are compiled for.  Exact-only, measured on the
+// large and small books' plain sweeps: 2 (what the compiler picks unasked,
+// 235-255 VGPRs, 336 B of scratch a lane) 18.5 and 13.2 ms, 3 (168 VGPRs,
+// 624 B) 17.0 and 12.9 ms, 4 (128 VGPRs, 800 B) 17.4 and 13.0 ms.  The
+// other two, left alone, get the whole register file and one wave a SIMD; at
+// 2 a book of 4000 BS and 6000 Exact options sweeps (levels 0..4) in 24.1
+// against 31.0 ms, one of 3000 BS, 5000 Exact and 2000 Unbiased options in
+// 1.53 against 2.28 s (at 3 the spills make both slower).
 #ifndef NMLMC_PLAIN_EXACT_WAVES
 #define NMLMC_PLAIN_EXACT_WAVES 3
 #endif
+#ifndef NMLMC_PLAIN_NO_UNBIASED_WAVES
+#define NMLMC_PLAIN_NO_UNBIASED_WAVES 2
+#endif
+#ifndef NMLMC_PLAIN_GENERAL_WAVES
+#define NMLMC_PLAIN_GENERAL_WAVES 2
+#endif
 
-template <uint32_t W, bool EXACT_ONLY>
-__global__ __launch_bounds__(64 * W, (EXACT_ONLY ? NMLMC_PLAIN_EXACT_WAVES : 1)) void k_portfolio_parity(
+template <uint32_t W, int KIND>
+__global__ __launch_bounds__(64 * W, (KIND == PAR_EXACT ? NMLMC_PLAIN_EXACT_WAVES : KIND == PAR_NO_UNBIASED ? NMLMC_PLAIN_NO_UNBIASED_WAVES : NMLMC_PLAIN_GENERAL_WAVES)) void k_portfolio_parity(
     PortfolioParityArgs A) {
+    constexpr bool EXACT_ONLY = KIND == PAR_EXACT;
     extern __shared__ double smem[];  // R_tau[d], z[d], reduction scratch[kParityRed]
     const PortfolioDevice& pf = A.pf;
     const EngineParams& ep = A.ep;
@@ -306,7 +321,7 @@ __global__ __launch_bounds__(64 * W, (EXACT_ONLY ? NMLMC_PLAIN_EXACT_WAVES : 1))
             if (pf.subsample == 0) start += det_bs_sum(pf, R, w_det);
 
         auto test_round = [&](uint64_t cur) {
-            const InnerSums s = parity_inner<W, EXACT_ONLY>(A, R, k1, ss, si, cur, red);
+            const InnerSums s = parity_inner<W, KIND>(A, R, k1, ss, si, cur, red);
             work += s.work + A.bs_work;
             stats[4] += s.nu;
             stats[5] += s.nn;
@@ -335,7 +350,7 @@ __global__ __launch_bounds__(64 * W, (EXACT_ONLY ? NMLMC_PLAIN_EXACT_WAVES : 1))
             return cur;
         };
         auto mean_of = [&](uint64_t n) {
-            const InnerSums s = parity_inner<W, EXACT_ONLY>(A, R, k1, ss, si, n, red);
+            const InnerSums s = parity_inner<W, KIND>(A, R, k1, ss, si, n, red);
             work += s.work + A.bs_work;
             stats[4] += s.nu;
             stats[5] += s.nn;
@@ -439,12 +454,16 @@ hipError_t launch_portfolio_parity(const PortfolioParityHost& H, hipStream_t s) 
     }
     const size_t lds = sizeof(double) * (2 * (size_t)H.pf.d + kParityRed);
     switch (W) {
-    case 8: hipLaunchKernelGGL((k_portfolio_parity<8, true>), dim3(H.blocks), dim3(512), lds, s, A); break;
-    case 4: hipLaunchKernelGGL((k_portfolio_parity<4, true>), dim3(H.blocks), dim3(256), lds, s, A); break;
-    case 2: hipLaunchKernelGGL((k_portfolio_parity<2, true>), dim3(H.blocks), dim3(128), lds, s, A); break;
+    case 8: hipLaunchKernelGGL((k_portfolio_parity<8, PAR_EXACT>), dim3(H.blocks), dim3(512), lds, s, A); break;
+    case 4: hipLaunchKernelGGL((k_portfolio_parity<4, PAR_EXACT>), dim3(H.blocks), dim3(256), lds, s, A); break;
+    case 2: hipLaunchKernelGGL((k_portfolio_parity<2, PAR_EXACT>), dim3(H.blocks), dim3(128), lds, s, A); break;
     default:
-        if (exact_only) hipLaunchKernelGGL((k_portfolio_parity<1, true>), dim3(H.blocks), dim3(64), lds, s, A);
-        else hipLaunchKernelGGL((k_portfolio_parity<1, false>), dim3(H.blocks), dim3(64), lds, s, A);
+        if (exact_only)
+            hipLaunchKernelGGL((k_portfolio_parity<1, PAR_EXACT>), dim3(H.blocks), dim3(64), lds, s, A);
+        else if (H.pf.nopt[2] == 0)
+            hipLaunchKernelGGL((k_portfolio_parity<1, PAR_NO_UNBIASED>), dim3(H.blocks), dim3(64), lds, s, A);
+        else
+            hipLaunchKernelGGL((k_portfolio_parity<1, PAR_GENERAL>), dim3(H.blocks), dim3(64), lds, s, A);
         break;
     }
     return hipGetLastError();
