@@ -22,10 +22,17 @@ namespace nmlmc {
 #ifndef NMLMC_GENERAL_WAVES
 #define NMLMC_GENERAL_WAVES 2
 #endif
+// FP64 forms (the batched FP64 form, the mixed form's rechecks): no cap; their
+// kernels hold 140-220 VGPRs, two or three waves a SIMD.  Capped at 3 the
+// model's batched FP64 sweep, the batch's FP64 sweep and the large book's
+// mixed sweep ran within 1% of uncapped; at 4 the batch ran 1.18x slower.
+#ifndef NMLMC_FP64_WAVES
+#define NMLMC_FP64_WAVES 1
+#endif
 template <class P, bool MIXED>
 constexpr int mixed_waves() {
     if (!P::kMixedWaves) return NMLMC_GENERAL_WAVES;
-    return MIXED ? NMLMC_MIXED_WAVES : 1;
+    return MIXED ? NMLMC_MIXED_WAVES : NMLMC_FP64_WAVES;
 }
 
 __device__ __forceinline__ uint32_t class_of(uint64_t n, uint64_t base_min) {
