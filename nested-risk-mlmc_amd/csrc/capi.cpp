@@ -94,6 +94,13 @@ static uint32_t pow2_ceil_u32(uint64_t x) {
 
 int Engine::ensure_device() {
     if (stream) return NMLMC_OK;
+    // Host waits on the level calls spin rather than yield (NMLMC_SPIN_WAIT=0
+    // keeps the runtime's default).  Possible only before the process's first
+    // HIP context exists; otherwise refused, and the refusal is cleared.
+    const char* spin = getenv("NMLMC_SPIN_WAIT");
+    if (spin && spin[0] == '1') {
+        if (hipSetDeviceFlags(hipDeviceScheduleSpin) != hipSuccess) (void)hipGetLastError();
+    }
     hipError_t e = hipGetDevice(&device);
     if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
     e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking);
