@@ -194,6 +194,25 @@ __device__ __forceinline__ void block_flush(const LaneAcc& a, Partial* out) {
     }
 }
 
+// A grid-stride pass over work items [0, total) split into kXcds contiguous
+// slices, slice x taken by the blocks of XCD x.  The dispatcher deals a grid's
+// blocks out to the 8 XCDs round-robin (block b on XCD b % 8), so a plain
+// grid stride sends neighbouring items to every XCD and each XCD's L2 ends up
+// holding every item's data (outer states, item records); sliced, an XCD
+// touches a disjoint eighth of the items.  The lanes of a wave still take
+// consecutive items, active lanes forming a prefix of the wave.
+constexpr uint32_t kXcds = 8;
+struct XcdRange {
+    uint64_t k0, end, stride;
+};
+__device__ __forceinline__ XcdRange xcd_range(uint64_t total, bool sliced = true) {
+    const uint32_t g = gridDim.x, b = blockIdx.x, bd = blockDim.x;
+    if (!sliced || g % kXcds) return XcdRange{(uint64_t)b * bd + threadIdx.x, total, (uint64_t)g * bd};
+    const uint32_t x = b % kXcds, lb = b / kXcds;
+    const uint64_t lo = total * x / kXcds, hi = total * (x + 1) / kXcds;
+    return XcdRange{lo + (uint64_t)lb * bd + threadIdx.x, hi, (uint64_t)(g / kXcds) * bd};
+}
+
 // Cross-lane hand-off of partial sums.  Contributors add their sums with
 // device-scope atomics, wait until those are acknowledged (s_waitcnt
 // vmcnt(0): performed at device scope), then count their segment; the lane
@@ -296,6 +315,7 @@ struct BatchedCommon {
     Partial* inst_out;   // batch of problems: per-instance level accumulators (else NULL: ctl->acc)
     uint32_t n_inst;     // instances in the batch (0 for a single problem)
     uint32_t zero_ctl;   // 1: the chain starts by clearing ctl (and inst_out); 0: ctl is clear already
+    uint32_t xcd_sliced; // 1: grid-stride passes split per XCD (xcd_range)
     double* tr_phi_f;
     double* tr_phi_c;
     uint64_t* tr_nf;

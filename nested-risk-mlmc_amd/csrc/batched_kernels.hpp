@@ -92,8 +92,8 @@ __global__ __launch_bounds__(256) void k_b_init(typename P::Args A) {
         c.ctl->cnt_f[0] = test_f ? (uint32_t)c.M : 0;
         c.ctl->cnt_c[0] = test_c ? (uint32_t)c.M : 0;
     }
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t m = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; m < c.M; m += stride) {
+    const XcdRange xr = xcd_range(c.M, c.xcd_sliced);
+    for (uint64_t m = xr.k0; m < xr.end; m += xr.stride) {
         ItemState& it = c.items[m];
         P::init_item(A, m, it, acc);
         const uint32_t q = it.inst;
@@ -210,8 +210,8 @@ __global__ __launch_bounds__(256, (mixed_waves<P, MIXED>())) void k_b_test(typen
     acc.template init<false>(c);
     TestEntry* const lf = RECHECK ? c.rq[0] : c.tf[r & 1];
     TestEntry* const lc_ = RECHECK ? c.rq[1] : c.tc[r & 1];
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < total; k += stride) {
+    const XcdRange xr = xcd_range(total, c.xcd_sliced);
+    for (uint64_t k = xr.k0; k < xr.end; k += xr.stride) {
         const bool fine = k < segs_f;
         const uint64_t kk = fine ? k : k - segs_f;
         const uint64_t spe = fine ? spe_f : spe_c;
@@ -752,8 +752,8 @@ __global__ __launch_bounds__(256, (mixed_waves<P, MIXED>())) void k_b_final(type
     const uint32_t n_whole = c.ctl->n_whole;
     const uint32_t whole_off = c.ctl->class_off[kMaxClasses];
     const uint64_t total = total_split + n_whole;
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < total; k += stride) {
+    const XcdRange xr = xcd_range(total, c.xcd_sliced);
+    for (uint64_t k = xr.k0; k < xr.end; k += xr.stride) {
         bool reeval = false;
         uint32_t m_done = 0;
         const bool split = k < total_split;
@@ -881,8 +881,8 @@ __global__ __launch_bounds__(256, (mixed_waves<P, MIXED>())) void k_b_fused(type
     acc.template init<true>(c);
     const bool test_f = c.cf.max > 2 * c.cf.base;
     const bool test_c = c.ell > 0 && c.cc.max > 2 * c.cc.base;
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t m = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; m < c.M; m += stride) {
+    const XcdRange xr = xcd_range(c.M, c.xcd_sliced);
+    for (uint64_t m = xr.k0; m < xr.end; m += xr.stride) {
         ItemState it;
         P::init_item(A, m, it, acc);
         const uint32_t q = it.inst;

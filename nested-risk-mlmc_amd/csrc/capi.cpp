@@ -428,6 +428,7 @@ int Engine::prepare_batched(uint64_t M, BatchedCommon& c) {
     c.redo_cnt = bat_redo_cnt;
     c.ctl = bat_ctl;
     c.zero_ctl = 1;
+    c.xcd_sliced = xcd_slices(problem == PROBLEM_PORTFOLIO);
     return NMLMC_OK;
 }
 
@@ -438,6 +439,16 @@ int Engine::prepare_batched(uint64_t M, BatchedCommon& c) {
 // short segment to pay its hand-off (measured: the model sweep 5% slower at a
 // minimum of 8, the large book's 2.5% faster at 16).  NMLMC_TEST_SEGMENT_MIN
 // overrides it.
+// Portfolio items read their outer states (d values each) in every inner
+// sample: sliced per XCD (xcd_range), each L2 holds an eighth of them (the
+// large book 1% and the 64-book batch 5% faster).  The model's items carry no
+// such data, and its uneven per-item work balances better unsliced (1%
+// faster).  NMLMC_XCD_SLICES=0/1 overrides.
+uint32_t xcd_slices(bool portfolio) {
+    if (const char* xs = getenv("NMLMC_XCD_SLICES")) return xs[0] == '0' ? 0u : 1u;
+    return portfolio ? 1u : 0u;
+}
+
 uint32_t test_segment_min(bool portfolio) {
     uint32_t v = portfolio ? 16u : 0u;
     if (const char* env = getenv("NMLMC_TEST_SEGMENT_MIN")) v = (uint32_t)strtoul(env, nullptr, 10);

@@ -284,6 +284,26 @@ def test_mixed_form_agrees_with_fp64(nm, oracle_f64, redo_all, monkeypatch):
     nm.set_form(nm.FORM_PLAIN)
 
 
+@pytest.mark.parametrize("sliced", ["1", "0"])
+@pytest.mark.parametrize("form", [1, 2])
+def test_xcd_slicing_keeps_results(nm, oracle_f64, sliced, form, monkeypatch):
+    """The batched forms' passes over outer samples, test entries and final
+    entries split into one slice per XCD or not (NMLMC_XCD_SLICES; the model's
+    default is unsliced): per-sample results equal the oracle's batched keying
+    either way."""
+    monkeypatch.setenv("NMLMC_XCD_SLICES", sliced)
+    orc = _shift_setup(nm, oracle_f64, 1, form)
+    for ell, M in ((0, 20000), (3, 900)):
+        g = nm.sample(ell, M, outer_offset=0, trace=True)
+        o = orc.sample(ell, M, keying=1, outer_offset=0, trace=True)
+        if form == nm.FORM_MIXED:
+            _cmp_mixed(g, o)
+        else:
+            _cmp(g, o)
+    nm.cleanup_nestedmc_sampler()
+    nm.set_form(nm.FORM_PLAIN)
+
+
 @pytest.mark.parametrize("anti", [True, False])
 def test_mixed_reference_rule_and_model(nm, oracle_f64, anti):
     """Mixed form with the reference's adaptive rule (N = delta^-r tmp, r = 1.5)
