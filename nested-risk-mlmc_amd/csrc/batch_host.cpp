@@ -206,7 +206,7 @@ int Engine::batch_sample(uint32_t ell, const uint64_t* Ms, sample_output* outs) 
     int rc = batched_common(ell, maxc, 0, H.c);
     if (rc) return rc;
     H.c.s_test_min = test_segment_min(true);
-    H.c.xcd_sliced = xcd_slices(true);
+    H.c.xcd_sliced = xcd_slices(exact_only);
     if ((rc = ensure_R((size_t)maxc * dmax))) return rc;
     std::memset(&H.pf, 0, sizeof(H.pf));
     H.R_tau = pf_R;

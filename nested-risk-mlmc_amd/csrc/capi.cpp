@@ -428,7 +428,8 @@ int Engine::prepare_batched(uint64_t M, BatchedCommon& c) {
     c.redo_cnt = bat_redo_cnt;
     c.ctl = bat_ctl;
     c.zero_ctl = 1;
-    c.xcd_sliced = xcd_slices(problem == PROBLEM_PORTFOLIO);
+    c.xcd_sliced = xcd_slices(problem == PROBLEM_PORTFOLIO && pf.subsample == 1 && pf.nopt[0] == 0 &&
+                              pf.nopt[2] == 0);
     return NMLMC_OK;
 }
 
@@ -439,14 +440,16 @@ int Engine::prepare_batched(uint64_t M, BatchedCommon& c) {
 // short segment to pay its hand-off (measured: the model sweep 5% slower at a
 // minimum of 8, the large book's 2.5% faster at 16).  NMLMC_TEST_SEGMENT_MIN
 // overrides it.
-// Portfolio items read their outer states (d values each) in every inner
-// sample: sliced per XCD (xcd_range), each L2 holds an eighth of them (the
-// large book 1% and the 64-book batch 5% faster).  The model's items carry no
-// such data, and its uneven per-item work balances better unsliced (1%
-// faster).  NMLMC_XCD_SLICES=0/1 overrides.
-uint32_t xcd_slices(bool portfolio) {
+// Items of sub-sampled books of Exact options read their outer states (d
+// values each) in every inner sample: sliced per XCD (xcd_range), each L2
+// holds an eighth of them (the large book 1% and the 64-book batch 5%
+// faster).  The model's items carry no such data, and its uneven per-item
+// work balances better unsliced (1%); so do books with Asians on coupled
+// paths, whose deep levels hold few long items (the 16-book batch 4%).
+// NMLMC_XCD_SLICES=0/1 overrides.
+uint32_t xcd_slices(bool exact_book) {
     if (const char* xs = getenv("NMLMC_XCD_SLICES")) return xs[0] == '0' ? 0u : 1u;
-    return portfolio ? 1u : 0u;
+    return exact_book ? 1u : 0u;
 }
 
 uint32_t test_segment_min(bool portfolio) {

@@ -196,6 +196,6 @@ int noalloc_setup(int device);
 uint64_t uint_pow_u64(uint64_t x, uint64_t n);
 LevelCounts level_counts(const EngineParams& ep, uint32_t ell);
 uint32_t test_segment_min(bool portfolio);
-uint32_t xcd_slices(bool portfolio);
+uint32_t xcd_slices(bool exact_book);
 
 }  // namespace nmlmc
