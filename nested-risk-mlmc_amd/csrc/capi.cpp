@@ -94,8 +94,8 @@ static uint32_t pow2_ceil_u32(uint64_t x) {
 
 int Engine::ensure_device() {
     if (stream) return NMLMC_OK;
-    // Host waits on the level calls spin rather than yield (NMLMC_SPIN_WAIT=0
-    // keeps the runtime's default).  Possible only before the process's first
+    // NMLMC_SPIN_WAIT=1: host waits on the level calls spin rather than yield
+    // (unset: the runtime's default).  Possible only before the process's first
     // HIP context exists; otherwise refused, and the refusal is cleared.
     const char* spin = getenv("NMLMC_SPIN_WAIT");
     if (spin && spin[0] == '1') {
@@ -433,13 +433,6 @@ int Engine::prepare_batched(uint64_t M, BatchedCommon& c) {
     return NMLMC_OK;
 }
 
-// Common arguments of a batched level call; checks that every draw-group size
-// is the smallest one times a power of two (finals are bucketed that way).
-// Smallest segment a sparse test round may shrink to (BatchedCommon::
-// s_test_min): portfolios only, since a model inner sample is too cheap for a
-// short segment to pay its hand-off (measured: the model sweep 5% slower at a
-// minimum of 8, the large book's 2.5% faster at 16).  NMLMC_TEST_SEGMENT_MIN
-// overrides it.
 // Items of sub-sampled books of Exact options read their outer states (d
 // values each) in every inner sample: sliced per XCD (xcd_range), each L2
 // holds an eighth of them (the large book 1% and the 64-book batch 5%
@@ -452,12 +445,19 @@ uint32_t xcd_slices(bool exact_book) {
     return exact_book ? 1u : 0u;
 }
 
+// Smallest segment a sparse test round may shrink to (BatchedCommon::
+// s_test_min): portfolios only, since a model inner sample is too cheap for a
+// short segment to pay its hand-off (measured: the model sweep 5% slower at a
+// minimum of 8, the large book's 2.5% faster at 16).  NMLMC_TEST_SEGMENT_MIN
+// overrides it.
 uint32_t test_segment_min(bool portfolio) {
     uint32_t v = portfolio ? 16u : 0u;
     if (const char* env = getenv("NMLMC_TEST_SEGMENT_MIN")) v = (uint32_t)strtoul(env, nullptr, 10);
     return v;
 }
 
+// Common arguments of a batched level call; checks that every draw-group size
+// is the smallest one times a power of two (finals are bucketed that way).
 int Engine::batched_common(uint32_t ell, uint64_t M, uint64_t outer_offset, BatchedCommon& c) {
     std::memset(&c, 0, sizeof(c));
     c.ep = ep;
