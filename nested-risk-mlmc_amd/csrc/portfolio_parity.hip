@@ -56,18 +56,18 @@ __device__ __forceinline__ double phi_value(int phi_type, double u, double coeff
     return phi_scalar(phi_type, u, coeff, phi);
 }
 
-// inner_sampler::sample_inner (nested_sampler.hpp:131-161) over `count` samples
-// from the thread's streams (ss: sync, si: inner), block-uniform result.  red:
-// LDS scratch of at least 3 W doubles.  EXACT_ONLY: a sub-sampled book whose
-// options are all Exact (the only books given W > 1); its kernel holds no BS or
-// Unbiased code, which keeps it to a fraction of the general kernel's registers.
-// Kernel variants by the book's option methods: the general one, books of
-// Exact options only (sub-sampled), and books without Unbiased options.
+// Kernel variants by the book's option methods: the general one; sub-sampled
+// books of Exact options only (the only books given W > 1), whose kernel holds
+// no BS or Unbiased code; books without Unbiased options.  Leaving code out
+// keeps a kernel to a fraction of the general one's registers.
 enum { PAR_GENERAL = 0, PAR_EXACT = 1, PAR_NO_UNBIASED = 2 };
 
+// inner_sampler::sample_inner (nested_sampler.hpp:131-161) over `count` samples
+// from the thread's streams (ss: sync, si: inner), block-uniform result.  red:
+// LDS scratch of at least 3 W doubles.
 template <uint32_t W, int KIND>
-__device__ __forceinline__ InnerSums parity_inner(const PortfolioParityArgs& A, const double* R, uint64_t k1, StreamState& ss,
-                                  StreamState& si, uint64_t count, double* red) {
+__device__ __forceinline__ InnerSums parity_inner(const PortfolioParityArgs& A, const double* R, uint64_t k1,
+                                                  StreamState& ss, StreamState& si, uint64_t count, double* red) {
     constexpr bool EXACT_ONLY = KIND == PAR_EXACT;
     const PortfolioDevice& pf = A.pf;
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
