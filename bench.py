@@ -348,12 +348,20 @@ def book_sweep(nm, which, levels, M0, form, tune=True, cores=1):
     nm.cleanup_nestedmc_sampler()
 
 
-def batch_sweep(nm, n, asian_frac, levels, M0, form):
+def batch_tune(nm, n, levels, M0):
+    """Launch-shape autotuning of a batch's level calls before the timed
+    region, as the single-problem legs tune theirs: level l at M_l = ceil(M0
+    2^-1.5l) outer samples per instance (the tuning calls draw outer samples
+    set apart for tuning).  Returns [segment, blocks per CU] per level."""
+    return [list(nm.batch_autotune_level(ell, [level_M(M0, ell)] * n)[:2]) for ell in range(levels + 1)]
+
+
+def batch_sweep(nm, n, asian_frac, levels, M0, form, tune=True):
     """Only a level sweep of a batch of n books (configs.batch_books; with
     asian_frac > 0 the Asians on level-coupled paths): levels 0..levels with
-    M_l = ceil(M0 2^-1.5l) outer samples per instance, one warm-up and one
-    measured sweep in the given form, for profiler and counter runs.  Prints
-    one JSON line."""
+    M_l = ceil(M0 2^-1.5l) outer samples per instance, launch shapes autotuned
+    per level (tune), one warm-up and one measured sweep in the given form, for
+    profiler and counter runs.  Prints one JSON line."""
     cfg = portfolio_configs(nm)
     books = cfg.batch_books(n, asian_frac=asian_frac)
     nm.init_nestedmc_sampler(1, 64, 2, 0.0, True)
@@ -368,6 +376,7 @@ def batch_sweep(nm, n, asian_frac, levels, M0, form):
         pf.comp_params()
         keep.append(pf)
         nm.batch_stage(i, pf, 1000 + i, L)
+    shapes = batch_tune(nm, n, levels, M0) if tune else None
     recs = []
     for rep in range(2):
         for ell in range(levels + 1):
@@ -379,7 +388,7 @@ def batch_sweep(nm, n, asian_frac, levels, M0, form):
                 recs.append({"ell": ell, "M_per_instance": M, "kernel_ms": round(nm.last_kernel_ms(), 3),
                              "wall_ms": round(1e3 * wall, 3), "inner": sum(int(x["stats"][0]) for x in g)})
     print(json.dumps({"batch": n, "asian_frac": asian_frac, "coupled": asian_frac > 0, "form": form,
-                      "levels": recs}), flush=True)
+                      "launch_shapes": shapes, "levels": recs}), flush=True)
     nm.set_level_coupled_paths(False)
     nm.cleanup_nestedmc_sampler()
 
@@ -457,7 +466,7 @@ def portfolio_bench(nm, which, levels, M0, rel_eps, tte, plain_cores, tte_M0, tu
     return out
 
 
-def batch_bench(nm, n, rel_eps, M0, plain_cores=0, asian_frac=0.0, coupled=False):
+def batch_bench(nm, n, rel_eps, M0, plain_cores=0, asian_frac=0.0, coupled=False, tune=True):
     """Batched risk: n independent books (configs.batch_books: d in {10, 50,
     100}, P log-uniform in [1e3, 1e5], thresholds at the 95/99/99.5% pilot
     quantiles), each driven by its own MLMC run to eps_i = rel_eps (1 - q_i),
@@ -468,7 +477,9 @@ def batch_bench(nm, n, rel_eps, M0, plain_cores=0, asian_frac=0.0, coupled=False
     form run problem after problem (the reference's way: one problem per
     process).  With coupled, the Asians take level-coupled Milstein paths
     (dt_h0inv 2^floor(l/2) steps at level l, fine and coarse values from the
-    same increments) instead of randomised-level paths."""
+    same increments) instead of randomised-level paths.  With tune, each form's
+    level calls 0..12 are autotuned first (batch_tune, M_l = ceil(2048
+    2^-1.5l) per instance), outside the timed region."""
     cfg = portfolio_configs(nm)
     t0 = time.perf_counter()
     books = cfg.batch_books(n, asian_frac=asian_frac)
@@ -495,6 +506,7 @@ def batch_bench(nm, n, rel_eps, M0, plain_cores=0, asian_frac=0.0, coupled=False
             pf.comp_params()
             keep.append(pf)
             nm.batch_stage(i, pf, 1000 + i, L)
+        shapes = batch_tune(nm, n, 12, 2048) if tune else None
         t1 = time.perf_counter()
         r = nm.batch_mlmc_run(tols, M0)
         dt = time.perf_counter() - t1
@@ -502,7 +514,8 @@ def batch_bench(nm, n, rel_eps, M0, plain_cores=0, asian_frac=0.0, coupled=False
         nm.cleanup_nestedmc_sampler()
         runs[label] = r
         out[label] = {"seconds": round(dt, 3), "max_levels": max(len(x["levels"]) for x in r),
-                      "outer_samples": int(sum(lv["M"] for x in r for lv in x["levels"]))}
+                      "outer_samples": int(sum(lv["M"] for x in r for lv in x["levels"])),
+                      "launch_shapes": shapes}
         print("# batch %s: whole-batch time to tolerance %.2f s" % (label, dt), file=sys.stderr, flush=True)
     if plain_cores:
         # The reference's way: one problem at a time, plain FP64 form.
@@ -588,7 +601,8 @@ def main():
         nm = __graft_entry__.package()
         nm.load()
         batch_sweep(nm, args.batch_sweep_n, 0.2 if args.batch_sweep == "asian" else 0.0, args.batch_sweep_levels,
-                    args.batch_sweep_M0, nm.FORM_MIXED if args.batch_sweep_form == "mixed" else nm.FORM_BATCHED)
+                    args.batch_sweep_M0, nm.FORM_MIXED if args.batch_sweep_form == "mixed" else nm.FORM_BATCHED,
+                    tune=not args.no_autotune)
         return
     if args.copy_calibration:
         copy_calibration()
@@ -692,10 +706,10 @@ def main():
             extra["config1"]["cpu"] = cpu_config1(args.config1_M0)
     if args.batch_n > 0 and not args.no_tte:
         extra["batch"] = batch_bench(nm, args.batch_n, args.batch_rel_eps, args.batch_M0,
-                                     0 if args.no_plain else args.plain_cores)
+                                     0 if args.no_plain else args.plain_cores, tune=not args.no_autotune)
         if args.batch_asian_frac > 0:
             extra["batch_asian"] = batch_bench(nm, args.batch_asian_n, args.batch_asian_rel_eps, args.batch_M0, 0,
-                                               args.batch_asian_frac, coupled=True)
+                                               args.batch_asian_frac, coupled=True, tune=not args.no_autotune)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args.cpu_M0, args.levels)

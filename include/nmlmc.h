@@ -229,6 +229,13 @@ int nmlmc_sample_levels(uint32_t n, const uint32_t* ells, const uint64_t* Ms, sa
 int nmlmc_batch_begin(uint32_t n);
 int nmlmc_batch_stage(uint32_t i, const portfolio_data* data, uint64_t seed, const double* chol, uint32_t d);
 int nmlmc_batch_sample(uint32_t ell, uint32_t n, const uint64_t* M, sample_output* outs);
+/* nmlmc_autotune_level for the batch level call (ell, M[0..n-1]): the same
+ * candidate shapes, timed on outer ids from 2^45 of every instance (the
+ * instances' own outer ids do not move), the fastest kept for later batch
+ * calls at level ell; dropped by nmlmc_batch_begin.  n as for
+ * nmlmc_batch_sample. */
+int nmlmc_batch_autotune_level(uint32_t ell, uint32_t n, const uint64_t* M, uint32_t* segment,
+                               uint32_t* blocks_per_cu, double* ms);
 
 /* A level call of the plain FP64 form (nmlmc_set_form(NMLMC_FORM_PLAIN), the
  * form of model_sample / portfolio_sample) that allocates no memory: the same

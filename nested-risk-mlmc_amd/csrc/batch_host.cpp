@@ -37,6 +37,7 @@ int Engine::batch_begin(uint32_t n) {
     if (n == 0 || n > kMaxInst) return set_error(NMLMC_ERR_ARG, "a batch holds 1..%u problems", kMaxInst);
     if (ensure_device()) return nmlmc_last_error();
     free_batch();
+    tuned.clear();  // shapes tuned for another workload
     batch.resize(n);
     if (hipMalloc(&d_bpf, sizeof(PortfolioDevice) * n) != hipSuccess ||
         hipMalloc(&d_bseed, sizeof(uint64_t) * n) != hipSuccess || hipMalloc(&d_bwork, sizeof(double) * n) != hipSuccess)
@@ -134,7 +135,8 @@ int Engine::batch_sample(uint32_t ell, const uint64_t* Ms, sample_output* outs) 
                 fill = 0;
             }
             const uint64_t take = std::min(left, chunk - fill);
-            chunks.back().push_back(Seg{i, fill, take, batch[i].next_outer[ell] + done});
+            const uint64_t first = batch_tuning ? (1ull << 45) : batch[i].next_outer[ell];
+            chunks.back().push_back(Seg{i, fill, take, first + done});
             fill += take;
             done += take;
             left -= take;
@@ -274,13 +276,41 @@ int Engine::batch_sample(uint32_t ell, const uint64_t* Ms, sample_output* outs) 
         reev += t.stats[6];
         flips += t.stats[7];
         rck += t.stats[8];
-        batch[i].next_outer[ell] += Ms[i];
+        if (!batch_tuning) batch[i].next_outer[ell] += Ms[i];
     }
     last_ms = ms_total;
     last_kernel = "k_b_final";
     mixed_reeval = reev;
     mixed_flips = flips;
     mixed_rechecks = rck;
+    return set_error(NMLMC_OK, "");
+}
+
+// Times the batch level call (ell, Ms) over the candidate launch shapes, as
+// autotune_level does for a single problem, on outer ids from 2^45 of every
+// instance (the instances' level calls then draw what they would have drawn
+// without it), and keeps the fastest for later batch calls at level ell.
+int Engine::batch_autotune_level(uint32_t ell, const uint64_t* Ms, Shape& best, double& best_ms) {
+    if (batch.empty()) return set_error(NMLMC_ERR_STATE, "no batch (nmlmc_batch_begin)");
+    if (!Ms) return set_error(NMLMC_ERR_ARG, "batch_autotune_level: NULL argument");
+    for (size_t i = 0; i < batch.size(); i++)
+        if (Ms[i] >= (1ull << 45)) return set_error(NMLMC_ERR_ARG, "batch_autotune_level: bad M");
+    const size_t n = batch.size();
+    std::vector<double> f(n * kMaxPhi * kMaxMoments), d(n * kMaxPhi * kMaxMoments);
+    std::vector<sample_output> outs(n);
+    for (size_t i = 0; i < n; i++) {
+        outs[i] = sample_output{};
+        outs[i].fsums = f.data() + i * kMaxPhi * kMaxMoments;
+        outs[i].diffsums = d.data() + i * kMaxPhi * kMaxMoments;
+    }
+    batch_tuning = true;
+    const int rc = tune_shapes(ell, [&](double& ms) {
+        const int r = batch_sample(ell, Ms, outs.data());
+        ms = last_ms;
+        return r;
+    }, best, best_ms);
+    batch_tuning = false;
+    if (rc) return rc;
     return set_error(NMLMC_OK, "");
 }
 

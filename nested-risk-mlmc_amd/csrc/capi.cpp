@@ -833,25 +833,18 @@ Engine::Shape Engine::batched_shape(uint32_t ell, uint64_t M, uint64_t spu, cons
 // faster kernel time counts) on outer samples 2^45 onwards, far from those of
 // the level's calls, so later results do not depend on whether tuning ran
 // (and per-sample results never depend on the shape).
-int Engine::autotune_level(uint32_t ell, uint64_t M, Shape& best, double& best_ms) {
-    if (!inited || problem < 0) return set_error(NMLMC_ERR_STATE, "no problem staged");
-    if (form == NMLMC_FORM_PLAIN) return set_error(NMLMC_ERR_ARG, "autotuning applies to the batched forms");
-    if (M == 0 || M >= (1ull << 45) || ell > 40) return set_error(NMLMC_ERR_ARG, "autotune: bad level or M");
-    std::vector<double> f(kMaxPhi * kMaxMoments), d(kMaxPhi * kMaxMoments);
-    sample_output out{};
-    out.fsums = f.data();
-    out.diffsums = d.data();
+int Engine::tune_shapes(uint32_t ell, const std::function<int(double&)>& run, Shape& best, double& best_ms) {
     auto time_shape = [&](Shape s, double& ms) -> int {
         force = s;
         ms = 1e300;
         for (int rep = 0; rep < 2; rep++) {
-            out.M = M;
-            const int rc = sample_batched(ell, M, 1ull << 45, 0, &out, nullptr, nullptr, nullptr, nullptr);
+            double t = 0.;
+            const int rc = run(t);
             if (rc) {
                 force = Shape{};
                 return rc;
             }
-            ms = std::min(ms, (double)last_ms);
+            ms = std::min(ms, t);
         }
         force = Shape{};
         return NMLMC_OK;
@@ -876,6 +869,24 @@ int Engine::autotune_level(uint32_t ell, uint64_t M, Shape& best, double& best_m
         }
     }
     tuned[((uint64_t)form << 32) | ell] = best;
+    return NMLMC_OK;
+}
+
+int Engine::autotune_level(uint32_t ell, uint64_t M, Shape& best, double& best_ms) {
+    if (!inited || problem < 0) return set_error(NMLMC_ERR_STATE, "no problem staged");
+    if (form == NMLMC_FORM_PLAIN) return set_error(NMLMC_ERR_ARG, "autotuning applies to the batched forms");
+    if (M == 0 || M >= (1ull << 45) || ell > 40) return set_error(NMLMC_ERR_ARG, "autotune: bad level or M");
+    std::vector<double> f(kMaxPhi * kMaxMoments), d(kMaxPhi * kMaxMoments);
+    sample_output out{};
+    out.fsums = f.data();
+    out.diffsums = d.data();
+    const int rc = tune_shapes(ell, [&](double& ms) {
+        out.M = M;
+        const int r = sample_batched(ell, M, 1ull << 45, 0, &out, nullptr, nullptr, nullptr, nullptr);
+        ms = last_ms;
+        return r;
+    }, best, best_ms);
+    if (rc) return rc;
     return set_error(NMLMC_OK, "");
 }
 
@@ -1044,6 +1055,21 @@ uint32_t nmlmc_last_group_lanes(void) { return g_engine.last_G; }
 
 int nmlmc_sample_levels(uint32_t n, const uint32_t* ells, const uint64_t* Ms, sample_output* outs) {
     return g_engine.sample_levels(n, ells, Ms, outs);
+}
+
+int nmlmc_batch_autotune_level(uint32_t ell, uint32_t n, const uint64_t* M, uint32_t* segment,
+                               uint32_t* blocks_per_cu, double* ms) {
+    if (n != g_engine.batch.size())
+        return set_error(NMLMC_ERR_ARG, "batch_autotune_level: %u array entries for a batch of %zu instances", n,
+                         g_engine.batch.size());
+    nmlmc::Engine::Shape best;
+    double best_ms = 0.;
+    const int rc = g_engine.batch_autotune_level(ell, M, best, best_ms);
+    if (rc) return rc;
+    if (segment) *segment = best.S;
+    if (blocks_per_cu) *blocks_per_cu = best.bpc;
+    if (ms) *ms = best_ms;
+    return set_error(NMLMC_OK, "");
 }
 
 int nmlmc_autotune_level(uint32_t ell, uint64_t M, uint32_t* segment, uint32_t* blocks_per_cu, double* ms) {

@@ -134,6 +134,9 @@ struct Engine {
     Shape force{};                    // the candidate autotune_level is timing
     Shape batched_shape(uint32_t ell, uint64_t M, uint64_t spu, const BatchedCommon& c, uint32_t lanes_override);
     int autotune_level(uint32_t ell, uint64_t M, Shape& best, double& best_ms);
+    // The candidate loop of the autotuners: run(ms) makes one level call at
+    // the forced shape and reports its kernel time.
+    int tune_shapes(uint32_t ell, const std::function<int(double&)>& run, Shape& best, double& best_ms);
     // Level sweeps (sample_levels): several batched level calls enqueued back
     // to back, each chain's control block copied to its own pinned host slot.
     bool sweep_on = false;                    // run_batched enqueues and returns
@@ -171,6 +174,10 @@ struct Engine {
     int batch_begin(uint32_t n);
     int batch_stage(uint32_t i, const portfolio_data* data, uint64_t seed, const double* chol, uint32_t d);
     int batch_sample(uint32_t ell, const uint64_t* Ms, sample_output* outs);
+    // Launch-shape autotuning of a batch level call (outer ids from 2^45 while
+    // batch_tuning is set; the instances' outer ids do not move).
+    bool batch_tuning = false;
+    int batch_autotune_level(uint32_t ell, const uint64_t* Ms, Shape& best, double& best_ms);
     void free_batch();
     int prepare_batched(uint64_t M, BatchedCommon& c);
     int batched_common(uint32_t ell, uint64_t M, uint64_t outer_offset, BatchedCommon& c);

@@ -87,6 +87,7 @@ EXPORTS = [
     "portfolio_estimate_delta_for_unbiased", "portfolio_gen_data", "portfolio_free_data",
     "nmlmc_mlmc_last_table", "nmlmc_autotune_level", "nmlmc_sample_levels",
     "nmlmc_batch_begin", "nmlmc_batch_stage", "nmlmc_batch_sample", "nmlmc_batch_mlmc_run",
+    "nmlmc_batch_autotune_level",
     "nmlmc_source_digest", "nmlmc_sample_noalloc", "nmlmc_sampler_model_init", "nmlmc_sampler_model_sample",
     "nmlmc_set_level_coupled_paths",
 ]
@@ -174,6 +175,7 @@ def load(path=LIB_PATH):
         "nmlmc_batch_begin": (i32, [u32]),
         "nmlmc_batch_stage": (i32, [u32, C.POINTER(PortfolioData), u64, dp, u32]),
         "nmlmc_batch_sample": (i32, [u32, u32, C.POINTER(u64), C.POINTER(SampleOutput)]),
+        "nmlmc_batch_autotune_level": (i32, [u32, u32, C.POINTER(u64), C.POINTER(u32), C.POINTER(u32), dp]),
         "nmlmc_batch_mlmc_run": (i32, [u32, dp, u64, dbl, dbl, dbl, u32, C.POINTER(LevelStats),
                                        C.POINTER(u32), dp]),
         "nmlmc_mlmc_run": (i32, [dbl, u64, dbl, dbl, dbl, i32, u32, C.POINTER(LevelStats),
@@ -552,6 +554,16 @@ def mlmc_estimate(r):
     for lv in r["levels"][s + 1:]:
         e += lv["sums_diff"][0] / lv["M"]
     return e
+
+
+def batch_autotune_level(ell, Ms):
+    """nmlmc_batch_autotune_level: time the batch level call (ell, Ms) over launch
+    shapes and keep the fastest for later batch calls at this level; returns
+    (segment, blocks_per_cu, ms)."""
+    n = len(Ms)
+    S, B, ms = C.c_uint32(), C.c_uint32(), C.c_double()
+    check(load().nmlmc_batch_autotune_level(ell, n, (C.c_uint64 * n)(*Ms), C.byref(S), C.byref(B), C.byref(ms)))
+    return S.value, B.value, ms.value
 
 
 def batch_sample(ell, Ms, moments=4, phi_size=1):

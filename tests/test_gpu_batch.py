@@ -83,6 +83,37 @@ def test_batch_matches_instances_alone(nm, form, chunk, monkeypatch):
     nm.set_form(nm.FORM_PLAIN)
 
 
+@pytest.mark.parametrize("form", [1, 2])
+def test_batch_autotune_keeps_results(nm, form):
+    """The batch launch-shape autotuner times its candidates on outer ids set
+    apart for tuning: the batch calls after it equal the instances' lone calls
+    (the instances' outer ids did not move), whatever shape it chose."""
+    books = _books(nm)
+    Ms = [[600, 250, 900], [200, 90, 300]]
+    ells = [0, 2]
+    alone = [_alone(nm, form, b, [(e, M[i]) for e, M in zip(ells, Ms)]) for i, b in enumerate(books)]
+    nm.init_nestedmc_sampler(1, 16, 2, 0.0, True)
+    nm.set_adaptive_rule(1, 3)
+    nm.set_form(form)
+    nm.batch_begin(len(books))
+    keep = []
+    for i, (d, sc, L, seed) in enumerate(books):
+        pf, _, _ = to_library(nm, d, **sc)
+        keep.append(pf)
+        nm.batch_stage(i, pf, seed, L)
+    for ell, M in zip(ells, Ms):
+        S, B, ms = nm.batch_autotune_level(ell, M)
+        assert S in (4, 8, 16, 32, 64, 128) and B in (4, 8, 16) and ms > 0
+    for c, (ell, M) in enumerate(zip(ells, Ms)):
+        got = nm.batch_sample(ell, M)
+        for i in range(len(books)):
+            assert _key(got[i]) == _key(alone[i][c]), (c, i)
+    with pytest.raises(nm.NmlmcError):
+        nm.batch_autotune_level(0, [100, 100])  # two entries for three instances
+    nm.cleanup_nestedmc_sampler()
+    nm.set_form(nm.FORM_PLAIN)
+
+
 def _asian_book(nm):
     """A general book: Exact options and Unbiased ones of which about half are
     arithmetic-average Asian calls / puts."""
