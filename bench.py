@@ -30,16 +30,16 @@ scaled figures are kept in profiles/hbm_traffic.json and read from there.
 cpu_baseline: the oracle (C restatement, FP64, same keying and workload) on the
 host cores with OpenMP, over a bounded sample of the sweep (M_l scaled down).
 Its calibration against the reference's own OpenMP engine (oracle/_ref, where
-present) on the workload both run (the non-adaptive sweep of config1) is
+present) on the workload both run (the non-adaptive model sweep below) is
 reported beside it.
 
 Also reported: per-level comparisons of the mixed form with the batched FP64
 form on the same outer samples (count flips, indicator flips, FP64 redos and
 rechecked decisions, level-mean differences), for the model and for the large
 book; the portfolio legs' rooflines (VALU lane-ops of the Exact inner samples,
-opcount.json); config1 (the non-adaptive model sweep of the CPU path, levels
+opcount.json); the non-adaptive model sweep (the reference's CPU path, levels
 0..8, M_l = ceil(1e6 2^-1.5l)) on the GPU forms and on the reference's own
-OpenMP engine; config3 (the small book) beside the large one.
+OpenMP engine; the small book beside the large one.
 """
 import argparse
 import json
@@ -220,7 +220,7 @@ def cpu_baseline(M0_cpu, levels):
                       % (levels, M0_cpu, total, dt)}
 
 
-def cpu_config1(M0, levels=8):
+def cpu_nonadaptive_sweep(M0, levels=8):
     """Config 1 on the host cores: the non-adaptive shift-model sweep (N = 32 2^l,
     antithetic, seed 1, M_l = ceil(M0 2^-1.5l)) with the reference keying, on
     the reference's own OpenMP engine (oracle/_ref, where it was built) and on
@@ -274,7 +274,7 @@ def cpu_config1(M0, levels=8):
     return out
 
 
-def gpu_config1(nm, M0, levels, plain_cores):
+def gpu_nonadaptive_sweep(nm, M0, levels, plain_cores):
     """Config 1's sweep on the GPU forms: per-level outer samples/s and the whole
     sweep's rate (non-adaptive: the reference rule with r = 0)."""
     out = {}
@@ -577,9 +577,9 @@ def main():
                     help="a second, tighter large-book tolerance (x (1 - q)); 0: skip")
     ap.add_argument("--cmp-M0", type=float, default=1e6,
                     help="M0 of the per-level mixed-vs-FP64 comparison sweep of the model")
-    ap.add_argument("--config1-M0", type=float, default=1e6)
-    ap.add_argument("--no-config1", action="store_true")
-    ap.add_argument("--no-small", action="store_true", help="skip the small book (config 3)")
+    ap.add_argument("--nonadaptive-M0", type=float, default=1e6)
+    ap.add_argument("--no-nonadaptive", action="store_true")
+    ap.add_argument("--no-small", action="store_true", help="skip the small book")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (for scaling rocprofv3 byte counters)")
     ap.add_argument("--book-sweep-form", choices=["mixed", "plain"], default="mixed",
@@ -695,15 +695,15 @@ def main():
                                              args.pf_tte_M0, tune=not args.no_autotune,
                                              rel_eps2=args.pf_rel_eps2 or None)
     if not args.no_small:
-        extra["config3_small_book"] = portfolio_bench(nm, "small", args.pf_levels, args.pf_M0, args.pf_rel_eps,
+        extra["small_book"] = portfolio_bench(nm, "small", args.pf_levels, args.pf_M0, args.pf_rel_eps,
                                                       not args.no_tte, args.plain_cores, args.pf_tte_M0,
                                                       tune=not args.no_autotune)
-    if not args.no_config1:
-        extra["config1"] = {"workload": "shift model, adaptive refinement off (N = 32 2^l), antithetic, seed 1, "
-                                        "levels 0..8, M_l = ceil(%g 2^-1.5l)" % args.config1_M0,
-                            "gpu": gpu_config1(nm, args.config1_M0, 8, args.plain_cores)}
+    if not args.no_nonadaptive:
+        extra["nonadaptive_sweep"] = {"workload": "shift model, adaptive refinement off (N = 32 2^l), antithetic, seed 1, "
+                                        "levels 0..8, M_l = ceil(%g 2^-1.5l)" % args.nonadaptive_M0,
+                            "gpu": gpu_nonadaptive_sweep(nm, args.nonadaptive_M0, 8, args.plain_cores)}
         if not args.no_cpu:
-            extra["config1"]["cpu"] = cpu_config1(args.config1_M0)
+            extra["nonadaptive_sweep"]["cpu"] = cpu_nonadaptive_sweep(args.nonadaptive_M0)
     if args.batch_n > 0 and not args.no_tte:
         # The batch drivers' level calls keep the default launch shapes: a shape
         # tuned at one M per instance holds near that M, and the drivers' calls
@@ -717,9 +717,9 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args.cpu_M0, args.levels)
-        c1 = extra.get("config1", {}).get("cpu")
+        c1 = extra.get("nonadaptive_sweep", {}).get("cpu")
         if c1 and "reference" in c1:
-            cpu["calibration"] = {"workload": "config1 sweep (non-adaptive shift model), same threads",
+            cpu["calibration"] = {"workload": "non-adaptive shift-model sweep, levels 0..8, same threads",
                                   "cores": c1["cores"], "port_outer_samples_per_s": c1["port"],
                                   "reference_outer_samples_per_s": c1["reference"],
                                   "port_over_reference": c1["port_over_reference"],

@@ -322,7 +322,7 @@ def test_mixed_reference_rule_and_model(nm, oracle_f64, anti):
     nm.set_form(nm.FORM_PLAIN)
 
 
-def test_config1_plain_nonadaptive_sweep(nm, oracle_f64):
+def test_plain_nonadaptive_shift_sweep(nm, oracle_f64):
     """The CPU-path workload on the GPU: shift model (X = Y + Z, threshold 2),
     plain FP64 form, adaptive refinement off (N = 32 2^l), antithetic, seed 1,
     8 virtual threads as with 8 OpenMP threads, levels 0..8 with
